@@ -1,0 +1,167 @@
+/*
+ * zccl_amd.h -- C ABI of the MI355X (gfx950) SZp fastmode1 codec and hZCCL
+ * homomorphic sum.  Library: zccl_amd/libzccl_amd.so.
+ *
+ * Two layers:
+ *
+ *  1. Drop-in entry points with the reference's own names and signatures.
+ *     A program that links the reference's libZCCL for these calls can link
+ *     this library instead.  Pointers may be host memory (copied through the
+ *     GPU) or device memory (used in place).  Calls are synchronous.  Like the
+ *     reference they return void; the status of the last call on the calling
+ *     thread is available from zccl_last_error().  The chunk count T that
+ *     shapes the stream (the reference's OpenMP team size,
+ *     ZCCL/src/ZCCL_float.c:533) is omp_get_max_threads() of the process's
+ *     libgomp, or the value set with zccl_set_num_chunks().
+ *
+ *  2. A device API (zccl_*_dev): device pointers only, explicit chunk count,
+ *     asynchronous on a caller-supplied hipStream_t, caller-owned workspace.
+ *     Every device stream travels with an index (block widths, tile and seed
+ *     offsets, stream size) that the kernels use instead of walking the bytes.
+ *
+ * Streams are byte-identical to the reference's for the same input, error
+ * bound, block length and chunk count.
+ */
+#ifndef ZCCL_AMD_H
+#define ZCCL_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------ status */
+
+#define ZCCL_OK 0
+#define ZCCL_EINVAL (-1)     /* bad argument: nbEle < chunks, blockSize out of range, null pointer */
+#define ZCCL_ENOSPC (-2)     /* output capacity too small */
+#define ZCCL_ECORRUPT (-3)   /* stream malformed: width > 32, index/stream mismatch, overrun */
+#define ZCCL_EDEVICE (-4)    /* HIP runtime error */
+#define ZCCL_ENOMEM (-5)     /* device allocation failed */
+#define ZCCL_EWORKSPACE (-6) /* workspace too small or not initialised */
+
+int zccl_last_error(void);
+const char *zccl_strerror(int status);
+
+/* Chunk count used by the drop-in entry points; 0 restores the default
+ * (omp_get_max_threads() of libgomp, i.e. the reference's team size). */
+void zccl_set_num_chunks(int nchunks);
+int zccl_get_num_chunks(void);
+
+/* ------------------------------------------------- drop-in (reference ABI) */
+
+/* ZCCL/include/ZCCL_float.h:31-32 (ZCCL/src/ZCCL_float.c:501-807) */
+void ZCCL_float_openmp_threadblock_arg(unsigned char *outputBytes, float *oriData, size_t *outSize,
+    float absErrBound, size_t nbEle, int blockSize);
+
+/* ZCCL/include/ZCCL_float.h:34-35 (ZCCL/src/ZCCL_float.c:809-988): chunk count 1 */
+void ZCCL_float_single_thread_arg(unsigned char *outputBytes, float *oriData, size_t *outSize,
+    float absErrBound, size_t nbEle, int blockSize);
+
+/* ZCCL/include/ZCCL_float.h:27-29: allocating form; result from malloc(),
+ * caller frees.  Produces the same bytes as the _arg form (the reference's
+ * allocating form mis-sizes the last chunk's remainder block,
+ * ZCCL/src/ZCCL_float.c:452,456; that defect is not reproduced). */
+unsigned char *ZCCL_float_openmp_threadblock(float *oriData, size_t *outSize, float absErrBound,
+    size_t nbEle, int blockSize);
+
+/* ZCCL/include/ZCCLd_float.h:21 (ZCCL/src/ZCCLd_float.c:538-754) */
+void ZCCL_float_decompress_openmp_threadblock_arg(float *newData, size_t nbEle, float absErrBound,
+    int blockSize, unsigned char *cmpBytes);
+
+/* ZCCL/include/ZCCLd_float.h:23 (ZCCL/src/ZCCLd_float.c:247-374): chunk count 1 */
+void ZCCL_float_decompress_single_thread_arg(float *newData, size_t nbEle, float absErrBound,
+    int blockSize, unsigned char *cmpBytes);
+
+/* ZCCL/include/ZCCLd_float.h:19: allocating form, *newData from malloc(). */
+void ZCCL_float_decompress_openmp_threadblock(float **newData, size_t nbEle, float absErrBound,
+    int blockSize, unsigned char *cmpBytes);
+
+/* ZCCL/include/ZCCLd_float.h:29 (ZCCL/src/ZCCLd_float.c:988-1679).
+ * *final_cmpSize is incremented by the stream size, as in the reference
+ * (:1027, :1655).  final_cmpBytes may alias cmpBytes2 (in-place ring use,
+ * ZCCL/src/ZCCL_ring_ho.c:281-287). */
+void ZCCL_float_homomophic_add_openmp_threadblock(unsigned char *final_cmpBytes,
+    size_t *final_cmpSize, size_t nbEle, float absErrBound, int blockSize, unsigned char *cmpBytes,
+    unsigned char *cmpBytes2);
+
+/* ZCCL/include/ZCCLd_float.h:33 (ZCCL/src/ZCCLd_float.c:1681-2066): chunk count 1 */
+void ZCCL_float_homomophic_add_single_thread(unsigned char *final_cmpBytes, size_t *final_cmpSize,
+    size_t nbEle, float absErrBound, int blockSize, unsigned char *cmpBytes,
+    unsigned char *cmpBytes2);
+
+/* ZCCL/include/ZCCLd_float.h:31 (ZCCL/src/ZCCLd_float.c:2067-2800): as the
+ * add above, and the counters are incremented by the number of blocks where
+ * both widths are zero / only the first is zero / only the second is zero /
+ * neither is zero. */
+void ZCCL_float_homomophic_add_openmp_threadblock_record(unsigned char *final_cmpBytes,
+    size_t *final_cmpSize, size_t nbEle, float absErrBound, int blockSize, unsigned char *cmpBytes,
+    unsigned char *cmpBytes2, int *total_counter_1, int *total_counter_2, int *total_counter_3,
+    int *total_counter_4);
+
+/* Drop-in variants with the stream size stated (no upper bound inferred from
+ * the header): reads exactly cmpSize bytes of each stream.  No reference
+ * counterpart; for callers that know their sizes. */
+int zccl_decompress_sized(float *newData, size_t nbEle, float absErrBound, int blockSize,
+    int nchunks, const unsigned char *cmpBytes, size_t cmpSize);
+int zccl_homomorphic_add_sized(unsigned char *final_cmpBytes, size_t final_cap, size_t *final_size,
+    size_t nbEle, int blockSize, int nchunks, const unsigned char *cmpBytes, size_t cmpSize,
+    const unsigned char *cmpBytes2, size_t cmpSize2);
+
+/* ------------------------------------------------------------ sizes */
+
+size_t zccl_worst_case_bytes(size_t nbEle, int blockSize, int nchunks);
+size_t zccl_index_bytes(size_t nbEle, int blockSize, int nchunks);
+size_t zccl_num_tiles(size_t nbEle, int blockSize, int nchunks);
+size_t zccl_num_blocks(size_t nbEle, int blockSize, int nchunks);
+/* Byte offset, inside an index, of the u64 stream size. */
+size_t zccl_index_size_offset(size_t nbEle, int blockSize, int nchunks);
+/* Workspace able to serve launches of up to max_tiles tiles. */
+size_t zccl_workspace_bytes(size_t max_tiles);
+
+/* ------------------------------------------------------------ device API */
+
+/* Zero a workspace (device memory of zccl_workspace_bytes(max_tiles) bytes)
+ * before its first use.  A workspace serves one launch at a time: launches
+ * that share one must be ordered on one stream. */
+int zccl_workspace_init(void *workspace, size_t workspace_bytes, void *stream);
+/* Synchronise `stream`, read (and optionally clear) the workspace's sticky
+ * error bits: 1 lookback wait bound hit, 2 output capacity, 4 width > 32,
+ * 8 index/stream mismatch, 16 stream overrun, 32 workspace capacity. */
+int zccl_workspace_errors(void *workspace, unsigned long long *errors, int clear, void *stream);
+
+/* Compress nbEle floats at x into out (capacity out_cap).  Writes the index
+ * and, if d_size is not null, the stream size to device memory. */
+int zccl_compress_dev(const float *x, size_t nbEle, float absErrBound, int blockSize, int nchunks,
+    unsigned char *out, size_t out_cap, void *index, unsigned long long *d_size, void *workspace,
+    size_t workspace_bytes, void *stream);
+
+/* Decompress a device stream described by `index` into out[nbEle]. */
+int zccl_decompress_dev(const unsigned char *in, const void *index, size_t nbEle, float absErrBound,
+    int blockSize, int nchunks, float *out, void *workspace, size_t workspace_bytes, void *stream);
+
+/* Fused homomorphic sum of k streams (1 <= k <= 32) sharing (nbEle,
+ * blockSize, nchunks): one pass over all operands, output canonical (equal to
+ * the reference's chain of k-1 pairwise adds in any order).  `ins` and
+ * `indices` are host arrays of k device pointers.  `out` must not alias an
+ * input. */
+int zccl_homsum_dev(const unsigned char *const *ins, const void *const *indices, int k,
+    size_t nbEle, int blockSize, int nchunks, unsigned char *out, size_t out_cap, void *index,
+    unsigned long long *d_size, void *workspace, size_t workspace_bytes, void *stream);
+
+/* Rebuild the index of a device stream that has none (e.g. produced by the
+ * CPU library); reads at most in_cap bytes of `in`. */
+int zccl_parse_dev(const unsigned char *in, size_t in_cap, size_t nbEle, int blockSize,
+    int nchunks, void *index, unsigned long long *d_size, void *workspace, size_t workspace_bytes,
+    void *stream);
+
+/* Block-case histogram of two indexed streams (see the _record add). */
+int zccl_case_count_dev(const void *index1, const void *index2, size_t nbEle, int blockSize,
+    int nchunks, unsigned long long *d_counts4, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ZCCL_AMD_H */

@@ -1,0 +1,55 @@
+"""CPU-side checks of the C ABI library: it loads without a GPU, exports every
+symbol include/zccl_amd.h declares, and its size queries agree with the oracle's
+geometry.  No compute entry point is called here."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from oracle import oracle as O
+from zccl_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "zccl_amd.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    text = re.sub(r"^#.*$", "", text, flags=re.M)
+    return sorted(set(re.findall(r"\b((?:zccl|ZCCL)_\w+)\s*\(", text)))
+
+
+def test_library_loads_and_exports_every_declared_symbol():
+    L = _lib.lib()
+    decl = declared_symbols()
+    assert len(decl) >= 25
+    for name in decl:
+        assert hasattr(L, name), name
+    assert set(decl) == set(_lib.EXPORTS)
+
+
+@pytest.mark.parametrize("n,L,T", [(1 << 20, 36, 16), (1000, 7, 3), (8, 36, 8), (9, 36, 8), (5000, 128, 1),
+                                   (100000, 4096, 4), (33, 1, 2)])
+def test_sizes_agree_with_oracle(n, L, T):
+    lib = _lib.lib()
+    assert lib.zccl_worst_case_bytes(n, L, T) == O.worst_case_bytes(n, L, T)
+    assert lib.zccl_num_blocks(n, L, T) == O.geometry(n, L, T).nblocks
+    assert lib.zccl_index_bytes(n, L, T) > lib.zccl_num_blocks(n, L, T)
+
+
+def test_invalid_geometry_reports_zero():
+    lib = _lib.lib()
+    assert lib.zccl_worst_case_bytes(3, 36, 8) == 0      # n < T
+    assert lib.zccl_worst_case_bytes(100, 0, 1) == 0     # L < 1
+    assert lib.zccl_worst_case_bytes(100, 4097, 1) == 0  # L above the tile kernels' limit
+
+
+def test_chunk_count_defaults_to_openmp_team_and_can_be_overridden():
+    lib = _lib.lib()
+    gomp = ctypes.CDLL("libgomp.so.1")
+    assert lib.zccl_get_num_chunks() == gomp.omp_get_max_threads()
+    lib.zccl_set_num_chunks(5)
+    assert lib.zccl_get_num_chunks() == 5
+    lib.zccl_set_num_chunks(0)
+    assert lib.zccl_get_num_chunks() == gomp.omp_get_max_threads()

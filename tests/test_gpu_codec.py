@@ -1,0 +1,268 @@
+"""GPU parity of the HIP codec against the CPU oracle and the committed golden
+vectors.  Bar: bit-exact streams, bit-exact decompressed FP32, bit-exact
+homomorphic sums, identical indexes.  Every kernel is reached through the C ABI
+(libzccl_amd.so) -- directly via ctypes for the drop-in entry points, via
+zccl_amd.codec for the device API."""
+import ctypes
+import json
+import os
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from oracle import oracle as O  # noqa: E402
+from tests.golden import gen  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+with open(os.path.join(GOLD, "fixtures_small.json")) as _f:
+    SMALL_META = json.load(_f)
+SMALL = np.load(os.path.join(GOLD, "fixtures_small.npz"), allow_pickle=False)
+
+
+def say(*a):
+    print("[gpu]", *a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def codec():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from zccl_amd import codec as c
+
+    torch.cuda.set_device(0)
+    return c
+
+
+def bits(a):
+    return np.ascontiguousarray(a, dtype=np.float32).view(np.uint32)
+
+
+def dev(x):
+    return torch.from_numpy(np.ascontiguousarray(x)).cuda()
+
+
+def check_index(codec, s, ref_stream):
+    w, bo, so, tot = O.parse(ref_stream, s.n, s.L, s.T)
+    assert s.size == tot == ref_stream.size
+    assert np.array_equal(s.widths().cpu().numpy(), w)
+
+
+# ------------------------------------------------------------ fixtures
+
+@pytest.mark.parametrize("case", SMALL_META, ids=[m["name"] for m in SMALL_META])
+def test_fixture_compress_decompress_add(codec, case):
+    name, n, eb, L, T = case["name"], case["n"], case["eb"], case["L"], case["T"]
+    say("fixture", name, n, L, T, eb)
+    x, y = SMALL[f"{name}.x"], SMALL[f"{name}.y"]
+    sx = codec.compress(dev(x), eb, L, T)
+    sy = codec.compress(dev(y), eb, L, T)
+    dx = codec.decompress(sx, eb)
+    h = codec.homsum([sx, sy])
+    torch.cuda.synchronize()
+    codec.check_errors()
+    assert np.array_equal(sx.to_host(), SMALL[f"{name}.cx"])
+    check_index(codec, sx, SMALL[f"{name}.cx"])
+    assert np.array_equal(bits(dx.cpu().numpy()), bits(SMALL[f"{name}.dx"]))
+    assert np.array_equal(h.to_host(), SMALL[f"{name}.hxy"])
+    check_index(codec, h, SMALL[f"{name}.hxy"])
+
+
+# ------------------------------------------------------------ random vs oracle
+
+CASES = [
+    # kind, n, eb, L, T
+    ("walk", 1 << 16, 1e-3, 36, 8),
+    ("walk", 100_003, 1e-2, 36, 16),
+    ("sines", 77_777, 1e-4, 32, 3),
+    ("grf", 1 << 17, 1e-3, 128, 5),
+    ("outliers", 50_001, 1e-3, 36, 7),
+    ("special", 20_000, 1e-3, 36, 4),
+    ("wide", 40_000, 1.0, 36, 4),
+    ("denormal", 30_000, 3e-39, 36, 4),
+    ("slabs", 60_000, 1e-3, 36, 8),
+    ("zeros", 10_000, 1e-3, 36, 8),
+    ("walk", 12_345, 1e-2, 1, 2),
+    ("walk", 12_345, 1e-2, 5, 2),
+    ("walk", 50_000, 1e-2, 100, 3),
+    ("walk", 70_000, 1e-2, 4096, 3),
+    ("walk", 16, 1e-2, 36, 16),      # n == T: seeds only
+    ("walk", 17, 1e-2, 36, 16),      # remainder of 1
+    ("walk", 40, 1e-2, 36, 16),
+    ("walk", 3, 1e-2, 36, 1),
+]
+
+
+@pytest.mark.parametrize("kind,n,eb,L,T", CASES)
+def test_random_against_oracle(codec, kind, n, eb, L, T):
+    say("random", kind, n, eb, L, T)
+    x = gen.make(kind, n, n + L)
+    y = gen.perturb(x, n + 1)
+    sx = codec.compress(dev(x), eb, L, T)
+    sy = codec.compress(dev(y), eb, L, T)
+    dx = codec.decompress(sx, eb)
+    h = codec.homsum([sx, sy])
+    dh = codec.decompress(h, eb)
+    torch.cuda.synchronize()
+    codec.check_errors()
+    cx, cy = O.compress(x, eb, L, T), O.compress(y, eb, L, T)
+    assert np.array_equal(sx.to_host(), cx)
+    assert np.array_equal(sy.to_host(), cy)
+    check_index(codec, sx, cx)
+    assert np.array_equal(bits(dx.cpu().numpy()), bits(O.decompress(cx, n, eb, L, T)))
+    ch = O.homsum([cx, cy], n, L, T)
+    assert np.array_equal(h.to_host(), ch)
+    assert np.array_equal(bits(dh.cpu().numpy()), bits(O.decompress(ch, n, eb, L, T)))
+
+
+@pytest.mark.parametrize("k", [1, 3, 8, 17, 32, 33, 40])
+def test_fused_k_operand_sum(codec, k):
+    n, eb, L, T = 200_001, 1e-3, 36, 16
+    say("homsum k", k)
+    base = gen.make("grf", n, 3)
+    xs = [gen.perturb(base, 100 + r) for r in range(k)]
+    ss = [codec.compress(dev(x), eb, L, T) for x in xs]
+    h = codec.homsum(ss)
+    torch.cuda.synchronize()
+    codec.check_errors()
+    ref = O.homsum([O.compress(x, eb, L, T) for x in xs], n, L, T)
+    assert np.array_equal(h.to_host(), ref)
+
+
+# ------------------------------------------------------------ foreign streams
+
+@pytest.mark.parametrize("kind,n,eb,L,T", [("walk", 100_003, 1e-2, 36, 16), ("sines", 5000, 1e-4, 7, 3),
+                                           ("walk", 17, 1e-2, 36, 16), ("zeros", 999, 1e-2, 36, 4)])
+def test_parse_cpu_stream(codec, kind, n, eb, L, T):
+    say("parse", kind, n, L, T)
+    x = gen.make(kind, n, 5)
+    cx = O.compress(x, eb, L, T)
+    s = codec.upload(cx, n, L, T)
+    y = codec.decompress(s, eb)
+    torch.cuda.synchronize()
+    codec.check_errors()
+    check_index(codec, s, cx)
+    assert np.array_equal(bits(y.cpu().numpy()), bits(O.decompress(cx, n, eb, L, T)))
+
+
+# ------------------------------------------------------------ drop-in ABI
+
+@pytest.fixture(scope="module")
+def capi(codec):
+    from zccl_amd import _lib
+
+    L = _lib.lib()
+    yield L
+    L.zccl_set_num_chunks(0)
+
+
+def test_dropin_host_pointers(capi):
+    n, eb, L, T = 300_007, 1e-3, 36, 8
+    say("dropin host", n, L, T)
+    capi.zccl_set_num_chunks(T)
+    x = gen.make("sines", n, 9)
+    y = gen.perturb(x, 10)
+    out = np.zeros(4 * n, dtype=np.uint8)
+    size = ctypes.c_size_t(0)
+    capi.ZCCL_float_openmp_threadblock_arg(out.ctypes.data, x.ctypes.data, ctypes.byref(size), eb, n, L)
+    assert capi.zccl_last_error() == 0
+    cx = O.compress(x, eb, L, T)
+    assert size.value == cx.size and np.array_equal(out[: size.value], cx)
+    dec = np.zeros(n, dtype=np.float32)
+    capi.ZCCL_float_decompress_openmp_threadblock_arg(dec.ctypes.data, n, eb, L, out.ctypes.data)
+    assert capi.zccl_last_error() == 0
+    assert np.array_equal(bits(dec), bits(O.decompress(cx, n, eb, L, T)))
+    # in-place add (final == second operand) with the size accumulated as the reference does
+    cy = O.compress(y, eb, L, T)
+    buf2 = np.zeros(4 * n, dtype=np.uint8)
+    buf2[: cy.size] = cy
+    fsize = ctypes.c_size_t(1000)
+    capi.ZCCL_float_homomophic_add_openmp_threadblock(buf2.ctypes.data, ctypes.byref(fsize), n, eb, L,
+                                                      out.ctypes.data, buf2.ctypes.data)
+    assert capi.zccl_last_error() == 0
+    ref = O.homsum([cx, cy], n, L, T)
+    assert fsize.value == 1000 + ref.size
+    assert np.array_equal(buf2[: ref.size], ref)
+
+
+def test_dropin_record_counters(capi):
+    n, eb, L, T = 100_000, 1e-3, 36, 4
+    capi.zccl_set_num_chunks(T)
+    x = gen.make("slabs", n, 2)
+    y = gen.perturb(x, 3)
+    cx, cy = O.compress(x, eb, L, T), O.compress(y, eb, L, T)
+    a = np.zeros(4 * n, dtype=np.uint8); a[: cx.size] = cx
+    b = np.zeros(4 * n, dtype=np.uint8); b[: cy.size] = cy
+    out = np.zeros(4 * n, dtype=np.uint8)
+    size = ctypes.c_size_t(0)
+    c = [ctypes.c_int(1) for _ in range(4)]
+    capi.ZCCL_float_homomophic_add_openmp_threadblock_record(
+        out.ctypes.data, ctypes.byref(size), n, eb, L, a.ctypes.data, b.ctypes.data, *[ctypes.byref(v) for v in c])
+    assert capi.zccl_last_error() == 0
+    wa = O.parse(cx, n, L, T)[0] != 0
+    wb = O.parse(cy, n, L, T)[0] != 0
+    expect = [(~wa & ~wb).sum(), (~wa & wb).sum(), (wa & ~wb).sum(), (wa & wb).sum()]
+    assert [v.value - 1 for v in c] == [int(e) for e in expect]
+    assert min(expect) > 0
+
+
+def test_dropin_device_pointers(capi, codec):
+    n, eb, L, T = 123_457, 1e-2, 36, 6
+    say("dropin device", n, L, T)
+    capi.zccl_set_num_chunks(T)
+    x = gen.make("walk", n, 4)
+    dx = dev(x)
+    out = torch.zeros(4 * n, dtype=torch.uint8, device="cuda")
+    size = ctypes.c_size_t(0)
+    capi.ZCCL_float_openmp_threadblock_arg(out.data_ptr(), dx.data_ptr(), ctypes.byref(size), eb, n, L)
+    assert capi.zccl_last_error() == 0
+    cx = O.compress(x, eb, L, T)
+    assert np.array_equal(out[: size.value].cpu().numpy(), cx)
+    dec = torch.empty(n, dtype=torch.float32, device="cuda")
+    capi.ZCCL_float_decompress_openmp_threadblock_arg(dec.data_ptr(), n, eb, L, out.data_ptr())
+    assert capi.zccl_last_error() == 0
+    assert np.array_equal(bits(dec.cpu().numpy()), bits(O.decompress(cx, n, eb, L, T)))
+
+
+def test_single_thread_twins(capi):
+    n, eb, L = 20_000, 1e-2, 36
+    x = SMALL["single_thread.x"]
+    n = x.size
+    out = np.zeros(4 * n + 64, dtype=np.uint8)
+    size = ctypes.c_size_t(0)
+    capi.ZCCL_float_single_thread_arg(out.ctypes.data, x.ctypes.data, ctypes.byref(size), eb, n, L)
+    assert np.array_equal(out[: size.value], SMALL["single_thread.cx"])
+    dec = np.zeros(n, dtype=np.float32)
+    capi.ZCCL_float_decompress_single_thread_arg(dec.ctypes.data, n, eb, L, out.ctypes.data)
+    assert np.array_equal(bits(dec), bits(SMALL["single_thread.dx"]))
+
+
+# ------------------------------------------------------------ errors
+
+def test_errors_are_reported_not_faulted(codec, capi):
+    n, eb, L, T = 50_000, 1e-3, 36, 4
+    x = gen.make("walk", n, 1)
+    # output capacity too small
+    small = codec.empty_stream(n, L, T, capacity=1000)
+    codec.compress(dev(x), eb, L, T, out=small)
+    with pytest.raises(Exception):
+        codec.check_errors()
+    # corrupt width byte (> 32) in a CPU stream: parse stops and reports
+    cx = O.compress(x, eb, L, T)
+    w, bo, so, tot = O.parse(cx, n, L, T)
+    bad = cx.copy()
+    bad[int(bo[5])] = 77
+    codec.upload(bad, n, L, T)
+    with pytest.raises(Exception):
+        codec.check_errors()
+    # n < T
+    with pytest.raises(Exception):
+        codec.compress(dev(x[:3]), eb, L, T)
+    # drop-in status
+    capi.zccl_set_num_chunks(8)
+    dec = np.zeros(4, dtype=np.float32)
+    capi.ZCCL_float_decompress_openmp_threadblock_arg(dec.ctypes.data, 4, eb, L, bad.ctypes.data)
+    assert capi.zccl_last_error() != 0

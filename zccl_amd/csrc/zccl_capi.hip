@@ -1,0 +1,702 @@
+// zccl_capi.hip -- C ABI: the reference's drop-in entry points and the device
+// API declared in include/zccl_amd.h.  Host code only; all stream work runs in
+// the kernels of zccl_kernels.hip.
+#include <dlfcn.h>
+#include <hip/hip_runtime.h>
+#include <sched.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <atomic>
+#include <mutex>
+#include <vector>
+
+#include "zccl_amd.h"
+#include "zccl_internal.h"
+#include "zccl_kernels.h"
+
+using namespace zk;
+
+// ------------------------------------------------------------ status
+
+static thread_local int g_last = ZCCL_OK;
+
+static int set_last(int s)
+{
+    g_last = s;
+    return s;
+}
+
+extern "C" int zccl_last_error(void) { return g_last; }
+
+extern "C" const char* zccl_strerror(int status)
+{
+    switch (status) {
+    case ZCCL_OK: return "ok";
+    case ZCCL_EINVAL: return "invalid argument";
+    case ZCCL_ENOSPC: return "output capacity too small";
+    case ZCCL_ECORRUPT: return "malformed stream";
+    case ZCCL_EDEVICE: return "HIP runtime error";
+    case ZCCL_ENOMEM: return "device allocation failed";
+    case ZCCL_EWORKSPACE: return "workspace too small or not initialised";
+    default: return "unknown status";
+    }
+}
+
+static int map_errors(unsigned long long e)
+{
+    if (e & ZK_ERR_CAPACITY)
+        return ZCCL_EWORKSPACE;
+    if (e & ZK_ERR_NOSPACE)
+        return ZCCL_ENOSPC;
+    if (e & (ZK_ERR_WIDTH | ZK_ERR_MISMATCH | ZK_ERR_OVERRUN))
+        return ZCCL_ECORRUPT;
+    if (e & ZK_ERR_SPIN)
+        return ZCCL_EDEVICE;
+    return ZCCL_OK;
+}
+
+#define HIP_TRY(expr)                                                                              \
+    do {                                                                                           \
+        if ((expr) != hipSuccess) {                                                                \
+            (void) hipGetLastError();                                                              \
+            return ZCCL_EDEVICE;                                                                   \
+        }                                                                                          \
+    } while (0)
+
+// ------------------------------------------------------------ chunk count
+
+static std::atomic<int> g_chunks_override{0};
+
+// The reference's chunk count is the size of its OpenMP team, i.e.
+// omp_get_max_threads() of libgomp in the calling process
+// (ZCCL/src/ZCCL_float.c:533).  Ask libgomp itself so both libraries agree.
+static int omp_team_size()
+{
+    using fn_t = int (*)(void);
+    static fn_t fn = []() -> fn_t {
+        void* h = dlopen("libgomp.so.1", RTLD_LAZY | RTLD_NOLOAD);
+        if (!h)
+            h = dlopen("libgomp.so.1", RTLD_LAZY);
+        return h ? (fn_t) dlsym(h, "omp_get_max_threads") : nullptr;
+    }();
+    if (fn)
+        return fn();
+    const char* env = getenv("OMP_NUM_THREADS");
+    if (env && atoi(env) > 0)
+        return atoi(env);
+    cpu_set_t set;
+    if (sched_getaffinity(0, sizeof(set), &set) == 0)
+        return CPU_COUNT(&set);
+    return 1;
+}
+
+extern "C" void zccl_set_num_chunks(int nchunks) { g_chunks_override = nchunks > 0 ? nchunks : 0; }
+
+extern "C" int zccl_get_num_chunks(void)
+{
+    int o = g_chunks_override.load();
+    return o > 0 ? o : omp_team_size();
+}
+
+// ------------------------------------------------------------ sizes
+
+static bool geom_of(size_t n, int L, int T, Geom* g)
+{
+    return L > 0 && T > 0 && make_geom((uint64_t) n, (uint32_t) L, (uint32_t) T, g);
+}
+
+extern "C" size_t zccl_worst_case_bytes(size_t n, int L, int T)
+{
+    Geom g;
+    return geom_of(n, L, T, &g) ? (size_t) worst_case_bytes(g) : 0;
+}
+
+extern "C" size_t zccl_index_bytes(size_t n, int L, int T)
+{
+    Geom g;
+    return geom_of(n, L, T, &g) ? (size_t) g.ix_bytes : 0;
+}
+
+extern "C" size_t zccl_num_tiles(size_t n, int L, int T)
+{
+    Geom g;
+    return geom_of(n, L, T, &g) ? (size_t) g.ntiles : 0;
+}
+
+extern "C" size_t zccl_num_blocks(size_t n, int L, int T)
+{
+    Geom g;
+    return geom_of(n, L, T, &g) ? (size_t) g.nblocks : 0;
+}
+
+extern "C" size_t zccl_index_size_offset(size_t n, int L, int T)
+{
+    Geom g;
+    return geom_of(n, L, T, &g) ? (size_t) g.ix_meta : 0;
+}
+
+extern "C" size_t zccl_workspace_bytes(size_t max_tiles) { return (size_t) workspace_bytes(max_tiles); }
+
+// Worst-case bytes of segment s (seed included).
+static uint64_t worst_segment(const Geom& g, uint32_t s)
+{
+    uint64_t len = s < g.T ? g.tb : g.R;
+    if (len < 2)
+        return 4;
+    uint64_t nfull = (len - 1) / g.L;
+    uint32_t tail = (uint32_t) ((len - 1) % g.L);
+    return 4 + nfull * block_bytes(32, g.L) + (tail ? block_bytes(32, tail) : 0);
+}
+
+// ------------------------------------------------------------ device API
+
+static int check_ws(void* ws, size_t ws_bytes, const Geom& g)
+{
+    if (!ws)
+        return ZCCL_EWORKSPACE;
+    if (ws_bytes < workspace_bytes(g.ntiles))
+        return ZCCL_EWORKSPACE;
+    return ZCCL_OK;
+}
+
+extern "C" int zccl_workspace_init(void* ws, size_t ws_bytes, void* stream)
+{
+    if (!ws || ws_bytes < sizeof(WsHeader))
+        return ZCCL_EINVAL;
+    uint64_t cap = (ws_bytes - sizeof(WsHeader)) / 16;
+    HIP_TRY(launch_ws_init((WsHeader*) ws, cap, (hipStream_t) stream));
+    return ZCCL_OK;
+}
+
+extern "C" int zccl_workspace_errors(void* ws, unsigned long long* errors, int clear, void* stream)
+{
+    if (!ws || !errors)
+        return ZCCL_EINVAL;
+    HIP_TRY(hipStreamSynchronize((hipStream_t) stream));
+    HIP_TRY(hipMemcpy(errors, &((WsHeader*) ws)->errors, 8, hipMemcpyDeviceToHost));
+    if (clear && *errors)
+        HIP_TRY(hipMemset(&((WsHeader*) ws)->errors, 0, 8));
+    return ZCCL_OK;
+}
+
+extern "C" int zccl_compress_dev(const float* x, size_t n, float eb, int L, int T, unsigned char* out,
+    size_t out_cap, void* index, unsigned long long* d_size, void* ws, size_t ws_bytes, void* stream)
+{
+    Geom g;
+    if (!x || !out || !index || !geom_of(n, L, T, &g))
+        return ZCCL_EINVAL;
+    int rc = check_ws(ws, ws_bytes, g);
+    if (rc)
+        return rc;
+    CompressArgs a;
+    a.g = g;
+    a.x = x;
+    volatile float one = 1.0f;
+    float invf = one / eb;  // FP32 division, as the reference's int / float
+    a.inv = (double) invf;
+    a.out = out;
+    a.out_cap = out_cap;
+    a.index = (uint8_t*) index;
+    a.ws = (WsHeader*) ws;
+    a.size_out = d_size;
+    HIP_TRY(launch_compress(a, (hipStream_t) stream));
+    return ZCCL_OK;
+}
+
+extern "C" int zccl_decompress_dev(const unsigned char* in, const void* index, size_t n, float eb,
+    int L, int T, float* out, void* ws, size_t ws_bytes, void* stream)
+{
+    Geom g;
+    if (!in || !index || !out || !geom_of(n, L, T, &g))
+        return ZCCL_EINVAL;
+    int rc = check_ws(ws, ws_bytes, g);
+    if (rc)
+        return rc;
+    DecompressArgs a;
+    a.g = g;
+    a.bytes = in;
+    a.index = (const uint8_t*) index;
+    a.eb = eb;
+    a.out = out;
+    a.ws = (WsHeader*) ws;
+    HIP_TRY(launch_decompress(a, (hipStream_t) stream));
+    return ZCCL_OK;
+}
+
+extern "C" int zccl_homsum_dev(const unsigned char* const* ins, const void* const* indices, int k,
+    size_t n, int L, int T, unsigned char* out, size_t out_cap, void* index,
+    unsigned long long* d_size, void* ws, size_t ws_bytes, void* stream)
+{
+    Geom g;
+    if (!ins || !indices || k < 1 || k > kMaxOperands || !out || !index || !geom_of(n, L, T, &g))
+        return ZCCL_EINVAL;
+    int rc = check_ws(ws, ws_bytes, g);
+    if (rc)
+        return rc;
+    HomsumArgs a;
+    memset(&a, 0, sizeof(a));
+    a.g = g;
+    a.k = k;
+    for (int r = 0; r < k; r++) {
+        if (!ins[r] || !indices[r] || ins[r] == out)
+            return ZCCL_EINVAL;
+        a.in[r].bytes = ins[r];
+        a.in[r].index = (const uint8_t*) indices[r];
+    }
+    a.out = out;
+    a.out_cap = out_cap;
+    a.index = (uint8_t*) index;
+    a.ws = (WsHeader*) ws;
+    a.size_out = d_size;
+    HIP_TRY(launch_homsum(a, (hipStream_t) stream));
+    return ZCCL_OK;
+}
+
+extern "C" int zccl_parse_dev(const unsigned char* in, size_t in_cap, size_t n, int L, int T,
+    void* index, unsigned long long* d_size, void* ws, size_t ws_bytes, void* stream)
+{
+    Geom g;
+    if (!in || !index || !geom_of(n, L, T, &g))
+        return ZCCL_EINVAL;
+    if (!ws || ws_bytes < sizeof(WsHeader))
+        return ZCCL_EWORKSPACE;
+    ParseArgs a;
+    a.g = g;
+    a.bytes = in;
+    a.in_cap = in_cap;
+    a.index = (uint8_t*) index;
+    a.ws = (WsHeader*) ws;
+    a.size_out = d_size;
+    HIP_TRY(launch_parse(a, (hipStream_t) stream));
+    return ZCCL_OK;
+}
+
+extern "C" int zccl_case_count_dev(const void* index1, const void* index2, size_t n, int L, int T,
+    unsigned long long* d_counts4, void* stream)
+{
+    Geom g;
+    if (!index1 || !index2 || !d_counts4 || !geom_of(n, L, T, &g))
+        return ZCCL_EINVAL;
+    HIP_TRY(launch_case_count((const uint8_t*) index1, (const uint8_t*) index2, g.nblocks, d_counts4,
+        (hipStream_t) stream));
+    return ZCCL_OK;
+}
+
+// ------------------------------------------------------------ drop-in context
+
+namespace {
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+};
+
+struct Ctx {
+    std::mutex mu;
+    hipStream_t stream = nullptr;
+    DevBuf x, out, in[2], ix[3], ws, small;
+    uint64_t ws_tiles = 0;
+};
+
+int ensure(DevBuf& b, size_t bytes)
+{
+    if (b.cap >= bytes && b.p)
+        return ZCCL_OK;
+    if (b.p) {
+        (void) hipFree(b.p);
+        b.p = nullptr;
+        b.cap = 0;
+    }
+    size_t want = std::max<size_t>(bytes, 256);
+    if (hipMalloc(&b.p, want) != hipSuccess) {
+        (void) hipGetLastError();
+        b.p = nullptr;
+        return ZCCL_ENOMEM;
+    }
+    b.cap = want;
+    return ZCCL_OK;
+}
+
+Ctx* ctx_current(int* rc)
+{
+    static std::mutex mu;
+    static std::vector<Ctx*> ctxs;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) {
+        (void) hipGetLastError();
+        *rc = ZCCL_EDEVICE;
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lk(mu);
+    if ((int) ctxs.size() <= dev)
+        ctxs.resize(dev + 1, nullptr);
+    if (!ctxs[dev]) {
+        Ctx* c = new Ctx();
+        if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+            (void) hipGetLastError();
+            delete c;
+            *rc = ZCCL_EDEVICE;
+            return nullptr;
+        }
+        ctxs[dev] = c;
+    }
+    *rc = ZCCL_OK;
+    return ctxs[dev];
+}
+
+int ensure_ws(Ctx& c, uint64_t tiles)
+{
+    if (c.ws.p && c.ws_tiles >= tiles)
+        return ZCCL_OK;
+    uint64_t want = std::max<uint64_t>(tiles, 1024);
+    int rc = ensure(c.ws, workspace_bytes(want));
+    if (rc)
+        return rc;
+    HIP_TRY(launch_ws_init((WsHeader*) c.ws.p, want, c.stream));
+    c.ws_tiles = want;
+    return ZCCL_OK;
+}
+
+// Device pointer?  If so, *avail receives the bytes from p to the end of its
+// allocation (so kernels never read or write past it).
+bool device_ptr(const void* p, size_t* avail)
+{
+    hipPointerAttribute_t a;
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void) hipGetLastError();
+        return false;
+    }
+    if (a.type != hipMemoryTypeDevice)
+        return false;
+    hipDeviceptr_t base = nullptr;
+    size_t size = 0;
+    if (hipMemGetAddressRange(&base, &size, (hipDeviceptr_t) p) != hipSuccess) {
+        (void) hipGetLastError();
+        *avail = 0;
+        return true;
+    }
+    *avail = (size_t) ((char*) base + size - (char*) p);
+    return true;
+}
+
+// Synchronise, then turn the workspace's sticky error bits into a status
+// (and clear them for the next call).
+int finish(Ctx& c)
+{
+    HIP_TRY(hipStreamSynchronize(c.stream));
+    unsigned long long e = 0;
+    HIP_TRY(hipMemcpy(&e, &((WsHeader*) c.ws.p)->errors, 8, hipMemcpyDeviceToHost));
+    if (e)
+        HIP_TRY(hipMemset(&((WsHeader*) c.ws.p)->errors, 0, 8));
+    return map_errors(e);
+}
+
+unsigned long long* small_slot(Ctx& c, int i) { return (unsigned long long*) c.small.p + i; }
+
+// Make stream bytes available on the device and build their index.  For a
+// host stream at most `cap` bytes are copied; for a device stream the
+// allocation bounds the read.
+int stage_stream(Ctx& c, const Geom& g, const unsigned char* bytes, size_t cap, bool cap_exact,
+    int slot, const unsigned char** dptr)
+{
+    size_t avail = 0;
+    bool dev = device_ptr(bytes, &avail);
+    uint64_t hdr_last = 0;
+    if (8ull * g.T > (dev ? avail : SIZE_MAX))
+        return ZCCL_ECORRUPT;
+    {
+        uint64_t h = 0;
+        if (dev)
+            HIP_TRY(hipMemcpy(&h, bytes + 8ull * (g.T - 1), 8, hipMemcpyDeviceToHost));
+        else
+            memcpy(&h, bytes + 8ull * (g.T - 1), 8);
+        hdr_last = 8ull * g.T + h;
+    }
+    uint64_t bound = hdr_last + worst_segment(g, g.T - 1) + (g.R ? worst_segment(g, g.T) : 0);
+    if (!cap_exact)
+        cap = (size_t) std::min<uint64_t>(bound, std::max<uint64_t>(4ull * g.n, hdr_last + 8));
+    if (dev)
+        cap = std::min(cap, avail);
+    const unsigned char* d = bytes;
+    if (!dev) {
+        int rc = ensure(c.in[slot], cap + 16);
+        if (rc)
+            return rc;
+        HIP_TRY(hipMemcpyAsync(c.in[slot].p, bytes, cap, hipMemcpyHostToDevice, c.stream));
+        d = (const unsigned char*) c.in[slot].p;
+    }
+    int rc = ensure(c.ix[slot], g.ix_bytes);
+    if (rc)
+        return rc;
+    ParseArgs a;
+    a.g = g;
+    a.bytes = d;
+    a.in_cap = cap;
+    a.index = (uint8_t*) c.ix[slot].p;
+    a.ws = (WsHeader*) c.ws.p;
+    a.size_out = nullptr;
+    HIP_TRY(launch_parse(a, c.stream));
+    *dptr = d;
+    return ZCCL_OK;
+}
+
+int dropin_compress(unsigned char* outputBytes, const float* x, size_t* outSize, float eb, size_t n,
+    int L, int T)
+{
+    if (!outputBytes || !x || !outSize)
+        return ZCCL_EINVAL;
+    *outSize = 0;
+    Geom g;
+    if (!geom_of(n, L, T, &g))
+        return ZCCL_EINVAL;
+    int rc;
+    Ctx* c = ctx_current(&rc);
+    if (!c)
+        return rc;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if ((rc = ensure_ws(*c, g.ntiles)) || (rc = ensure(c->small, 64)) || (rc = ensure(c->ix[0], g.ix_bytes)))
+        return rc;
+    size_t avail_x = 0, avail_o = 0;
+    bool xdev = device_ptr(x, &avail_x), odev = device_ptr(outputBytes, &avail_o);
+    const float* dx = x;
+    if (!xdev) {
+        if ((rc = ensure(c->x, 4 * n)))
+            return rc;
+        HIP_TRY(hipMemcpyAsync(c->x.p, x, 4 * n, hipMemcpyHostToDevice, c->stream));
+        dx = (const float*) c->x.p;
+    } else if (avail_x < 4 * n) {
+        return ZCCL_EINVAL;
+    }
+    unsigned char* dout = outputBytes;
+    size_t dcap = avail_o;
+    if (!odev) {
+        dcap = worst_case_bytes(g);
+        if ((rc = ensure(c->out, dcap)))
+            return rc;
+        dout = (unsigned char*) c->out.p;
+    }
+    rc = zccl_compress_dev(dx, n, eb, L, T, dout, dcap, c->ix[0].p, small_slot(*c, 0), c->ws.p,
+        c->ws.cap, c->stream);
+    if (rc)
+        return rc;
+    if ((rc = finish(*c)))
+        return rc;
+    unsigned long long size = 0;
+    HIP_TRY(hipMemcpy(&size, small_slot(*c, 0), 8, hipMemcpyDeviceToHost));
+    if (!odev)
+        HIP_TRY(hipMemcpy(outputBytes, dout, size, hipMemcpyDeviceToHost));
+    *outSize = size;
+    return ZCCL_OK;
+}
+
+int dropin_decompress(float* newData, size_t n, float eb, int L, int T, const unsigned char* cmp,
+    size_t cmpSize, bool sized)
+{
+    if (!newData || !cmp)
+        return ZCCL_EINVAL;
+    Geom g;
+    if (!geom_of(n, L, T, &g))
+        return ZCCL_EINVAL;
+    int rc;
+    Ctx* c = ctx_current(&rc);
+    if (!c)
+        return rc;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if ((rc = ensure_ws(*c, g.ntiles)))
+        return rc;
+    const unsigned char* din = nullptr;
+    if ((rc = stage_stream(*c, g, cmp, cmpSize, sized, 0, &din)))
+        return rc;
+    size_t avail = 0;
+    bool odev = device_ptr(newData, &avail);
+    float* dout = newData;
+    if (odev && avail < 4 * n)
+        return ZCCL_EINVAL;
+    if (!odev) {
+        if ((rc = ensure(c->x, 4 * n)))
+            return rc;
+        dout = (float*) c->x.p;
+    }
+    rc = zccl_decompress_dev(din, c->ix[0].p, n, eb, L, T, dout, c->ws.p, c->ws.cap, c->stream);
+    if (rc)
+        return rc;
+    if ((rc = finish(*c)))
+        return rc;
+    if (!odev)
+        HIP_TRY(hipMemcpy(newData, dout, 4 * n, hipMemcpyDeviceToHost));
+    return ZCCL_OK;
+}
+
+int dropin_add(unsigned char* out, size_t out_cap, bool out_cap_known, size_t* out_size, bool accumulate,
+    size_t n, int L, int T, const unsigned char* a, size_t asz, const unsigned char* b, size_t bsz,
+    bool sized, int* counters)
+{
+    if (!out || !out_size || !a || !b)
+        return ZCCL_EINVAL;
+    Geom g;
+    if (!geom_of(n, L, T, &g))
+        return ZCCL_EINVAL;
+    int rc;
+    Ctx* c = ctx_current(&rc);
+    if (!c)
+        return rc;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if ((rc = ensure_ws(*c, g.ntiles)) || (rc = ensure(c->small, 64)) || (rc = ensure(c->ix[2], g.ix_bytes)))
+        return rc;
+    const unsigned char* da = nullptr;
+    const unsigned char* db = nullptr;
+    if ((rc = stage_stream(*c, g, a, asz, sized, 0, &da)) || (rc = stage_stream(*c, g, b, bsz, sized, 1, &db)))
+        return rc;
+    uint64_t cap = worst_case_bytes(g);
+    if ((rc = ensure(c->out, cap)))
+        return rc;
+    const unsigned char* ins[2] = {da, db};
+    const void* ixs[2] = {c->ix[0].p, c->ix[1].p};
+    rc = zccl_homsum_dev(ins, ixs, 2, n, L, T, (unsigned char*) c->out.p, cap, c->ix[2].p,
+        small_slot(*c, 0), c->ws.p, c->ws.cap, c->stream);
+    if (rc)
+        return rc;
+    if (counters)
+        HIP_TRY(launch_case_count((const uint8_t*) c->ix[0].p, (const uint8_t*) c->ix[1].p, g.nblocks,
+            small_slot(*c, 2), c->stream));
+    if ((rc = finish(*c)))
+        return rc;
+    unsigned long long size = 0;
+    HIP_TRY(hipMemcpy(&size, small_slot(*c, 0), 8, hipMemcpyDeviceToHost));
+    size_t avail = 0;
+    bool odev = device_ptr(out, &avail);
+    if ((odev && size > avail) || (out_cap_known && size > out_cap))
+        return ZCCL_ENOSPC;
+    HIP_TRY(hipMemcpy(out, c->out.p, size, odev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost));
+    if (counters) {
+        unsigned long long cnt[4];
+        HIP_TRY(hipMemcpy(cnt, small_slot(*c, 2), 32, hipMemcpyDeviceToHost));
+        for (int i = 0; i < 4; i++)
+            counters[i] += (int) cnt[i];
+    }
+    *out_size = accumulate ? *out_size + size : size;
+    return ZCCL_OK;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------ drop-in ABI
+
+extern "C" void ZCCL_float_openmp_threadblock_arg(unsigned char* outputBytes, float* oriData,
+    size_t* outSize, float absErrBound, size_t nbEle, int blockSize)
+{
+    set_last(dropin_compress(outputBytes, oriData, outSize, absErrBound, nbEle, blockSize,
+        zccl_get_num_chunks()));
+}
+
+extern "C" void ZCCL_float_single_thread_arg(unsigned char* outputBytes, float* oriData,
+    size_t* outSize, float absErrBound, size_t nbEle, int blockSize)
+{
+    set_last(dropin_compress(outputBytes, oriData, outSize, absErrBound, nbEle, blockSize, 1));
+}
+
+extern "C" unsigned char* ZCCL_float_openmp_threadblock(float* oriData, size_t* outSize,
+    float absErrBound, size_t nbEle, int blockSize)
+{
+    int T = zccl_get_num_chunks();
+    size_t cap = zccl_worst_case_bytes(nbEle, blockSize, T);
+    if (!cap || !outSize) {
+        set_last(ZCCL_EINVAL);
+        return nullptr;
+    }
+    unsigned char* out = (unsigned char*) malloc(cap);
+    if (!out) {
+        set_last(ZCCL_ENOMEM);
+        return nullptr;
+    }
+    int rc = dropin_compress(out, oriData, outSize, absErrBound, nbEle, blockSize, T);
+    set_last(rc);
+    if (rc) {
+        free(out);
+        return nullptr;
+    }
+    return out;
+}
+
+extern "C" void ZCCL_float_decompress_openmp_threadblock_arg(float* newData, size_t nbEle,
+    float absErrBound, int blockSize, unsigned char* cmpBytes)
+{
+    set_last(dropin_decompress(newData, nbEle, absErrBound, blockSize, zccl_get_num_chunks(), cmpBytes, 0,
+        false));
+}
+
+extern "C" void ZCCL_float_decompress_single_thread_arg(float* newData, size_t nbEle,
+    float absErrBound, int blockSize, unsigned char* cmpBytes)
+{
+    set_last(dropin_decompress(newData, nbEle, absErrBound, blockSize, 1, cmpBytes, 0, false));
+}
+
+extern "C" void ZCCL_float_decompress_openmp_threadblock(float** newData, size_t nbEle,
+    float absErrBound, int blockSize, unsigned char* cmpBytes)
+{
+    if (!newData) {
+        set_last(ZCCL_EINVAL);
+        return;
+    }
+    *newData = (float*) malloc(sizeof(float) * (nbEle ? nbEle : 1));
+    if (!*newData) {
+        set_last(ZCCL_ENOMEM);
+        return;
+    }
+    set_last(dropin_decompress(*newData, nbEle, absErrBound, blockSize, zccl_get_num_chunks(), cmpBytes, 0,
+        false));
+}
+
+extern "C" void ZCCL_float_homomophic_add_openmp_threadblock(unsigned char* final_cmpBytes,
+    size_t* final_cmpSize, size_t nbEle, float absErrBound, int blockSize, unsigned char* cmpBytes,
+    unsigned char* cmpBytes2)
+{
+    (void) absErrBound;  // unused by the reference too
+    set_last(dropin_add(final_cmpBytes, 0, false, final_cmpSize, true, nbEle, blockSize,
+        zccl_get_num_chunks(), cmpBytes, 0, cmpBytes2, 0, false, nullptr));
+}
+
+extern "C" void ZCCL_float_homomophic_add_single_thread(unsigned char* final_cmpBytes,
+    size_t* final_cmpSize, size_t nbEle, float absErrBound, int blockSize, unsigned char* cmpBytes,
+    unsigned char* cmpBytes2)
+{
+    (void) absErrBound;
+    set_last(dropin_add(final_cmpBytes, 0, false, final_cmpSize, true, nbEle, blockSize, 1, cmpBytes, 0,
+        cmpBytes2, 0, false, nullptr));
+}
+
+extern "C" void ZCCL_float_homomophic_add_openmp_threadblock_record(unsigned char* final_cmpBytes,
+    size_t* final_cmpSize, size_t nbEle, float absErrBound, int blockSize, unsigned char* cmpBytes,
+    unsigned char* cmpBytes2, int* total_counter_1, int* total_counter_2, int* total_counter_3,
+    int* total_counter_4)
+{
+    (void) absErrBound;
+    int cnt[4] = {0, 0, 0, 0};
+    int rc = dropin_add(final_cmpBytes, 0, false, final_cmpSize, true, nbEle, blockSize,
+        zccl_get_num_chunks(), cmpBytes, 0, cmpBytes2, 0, false, cnt);
+    set_last(rc);
+    if (rc == ZCCL_OK) {
+        if (total_counter_1) *total_counter_1 += cnt[0];
+        if (total_counter_2) *total_counter_2 += cnt[1];
+        if (total_counter_3) *total_counter_3 += cnt[2];
+        if (total_counter_4) *total_counter_4 += cnt[3];
+    }
+}
+
+extern "C" int zccl_decompress_sized(float* newData, size_t nbEle, float absErrBound, int blockSize,
+    int nchunks, const unsigned char* cmpBytes, size_t cmpSize)
+{
+    return set_last(dropin_decompress(newData, nbEle, absErrBound, blockSize, nchunks, cmpBytes, cmpSize,
+        true));
+}
+
+extern "C" int zccl_homomorphic_add_sized(unsigned char* final_cmpBytes, size_t final_cap,
+    size_t* final_size, size_t nbEle, int blockSize, int nchunks, const unsigned char* cmpBytes,
+    size_t cmpSize, const unsigned char* cmpBytes2, size_t cmpSize2)
+{
+    return set_last(dropin_add(final_cmpBytes, final_cap, true, final_size, false, nbEle, blockSize, nchunks,
+        cmpBytes, cmpSize, cmpBytes2, cmpSize2, true, nullptr));
+}
