@@ -1,0 +1,289 @@
+"""Benchmark: fused homomorphic sum (hZCCL reduce-scatter) on one MI355X.
+
+Workload (one step): the reduce-scatter of the hZCCL allreduce data flow
+(ZCCL/src/ZCCL_ring_ho.c:258-290) for 8 simulated ranks, each holding a 512^3
+FP32 Gaussian random field with P(k) ~ k^-3 (shared seed) plus an independent
+N(0, 0.05 std) perturbation per rank; relative error bound 1e-3 of the largest
+value range; block length L = 36; chunk count T = 16.  Every rank's field is
+compressed into 8 segments of 2^24 values beforehand (inputs resident in HBM as
+compressed streams); a step is 8 fused homomorphic-sum launches, one per
+segment, each adding the 8 ranks' streams.  ``value`` = GB/s of input
+represented (8 ranks x 512 MiB per step) / step time.
+
+Also reported: the plain per-call path (the reference's ring order, 7 pairwise
+adds per segment), compress and decompress rates, the roofline of the fused
+kernel (algorithmic bytes per launch / average launch time from HIP events,
+against 8 TB/s), HBM traffic per launch from rocprofv3 counters when
+profiles/homsum_traffic.json exists, and the compiled reference's OpenMP
+homomorphic add (the same 8-operand chain on segment 0) timed on the host
+cores, checked byte for byte against the GPU output.
+
+    python bench.py [--gpus 1] [--steps 10] [--warmup 3]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBPS = 8000.0
+METRIC = "GB/s input (fused homomorphic sum, 1 GPU) vs plain; serve p50/p99 ms"
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def parse_args():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--side", type=int, default=512, help="grid side of each rank's field")
+    ap.add_argument("--ranks", type=int, default=8)
+    ap.add_argument("--rel-eb", type=float, default=1e-3)
+    ap.add_argument("--L", type=int, default=36)
+    ap.add_argument("--T", type=int, default=16)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="bound on the CPU baseline sample")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 256 MiB device copy (rocprof counter calibration)")
+    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "homsum_traffic.json"))
+    return ap.parse_args()
+
+
+def cpu_baseline(streams0, n, L, T, gpu_sum, seconds):
+    """Time the compiled reference's pairwise homomorphic add (OpenMP team of
+    T) over the ring chain of segment 0's 8 streams; bounded to ~`seconds`."""
+    import ctypes
+
+    import numpy as np
+
+    from oracle import oracle as O
+
+    kind = "reference" if O.ref_available() else "port"
+    hs = [s.to_host() for s in streams0]
+    if kind == "reference":
+        R = O.ref()
+        O._set_T(T)
+        pad = [np.zeros(h.size + 256, dtype=np.uint8) for h in hs]
+        for p, h in zip(pad, hs):
+            p[: h.size] = h
+        cap = max(O.worst_case_bytes(n, L, T), 4 * n) + 64
+        acc_a = np.zeros(cap, dtype=np.uint8)
+        acc_b = np.zeros(cap, dtype=np.uint8)
+        if not O.ref_buffer_fits(gpu_sum, n, T):
+            return None
+
+        def chain():
+            size = ctypes.c_size_t(0)
+            cur = pad[0]
+            bufs = (acc_a, acc_b)
+            for i, p in enumerate(pad[1:]):
+                out = bufs[i % 2]
+                size.value = 0
+                R.ZCCL_float_homomophic_add_openmp_threadblock(
+                    out.ctypes.data, ctypes.byref(size), n, ctypes.c_float(0.0), L, cur.ctypes.data, p.ctypes.data)
+                cur = out
+            return cur[: size.value]
+    else:
+        def chain():
+            return O.homsum(hs, n, L, T)
+
+    out = chain()  # warm-up, and the parity check
+    bitexact = bool(np.array_equal(out, gpu_sum))
+    times = []
+    t_end = time.perf_counter() + seconds
+    while len(times) < 3 or (time.perf_counter() < t_end and len(times) < 50):
+        t0 = time.perf_counter()
+        chain()
+        times.append(time.perf_counter() - t0)
+    med = float(np.median(times))
+    gb = len(hs) * 4.0 * n / 1e9
+    return {
+        "value": round(gb / med, 3),
+        "unit": "GB/s",
+        "cores": T,
+        "kind": kind,
+        "sample": (f"segment 0 only: ring chain of {len(hs) - 1} pairwise homomorphic adds over "
+                   f"{len(hs)} streams of {n} values (L={L}, T={T}), median of {len(times)} runs"),
+        "bitexact_vs_gpu": bitexact,
+    }
+
+
+def main():
+    args = parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported "
+                                   "(multi-GPU sharding is described in DESIGN.md, not built)"}))
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from zccl_amd import codec, synth
+    from zccl_amd.allreduce import StagedAllreduce
+
+    torch.cuda.set_device(0)
+    if args.copy_calibration:
+        a = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        for _ in range(3):
+            a.clone()
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_per_launch": a.numel() * 4}))
+        return
+
+    P, side, L, T = args.ranks, args.side, args.L, args.T
+    count = side ** 3
+    log(f"generating {P} fields of {side}^3")
+    base = synth.grf3d(side, seed=20261202)
+    xs = [synth.perturb(base, 0.05, seed=1000 + r) for r in range(P)]
+    del base
+    eb = synth.rel_to_abs_eb(xs, args.rel_eb)
+    ar = StagedAllreduce(P=P, count=count, eb=eb, L=L, T=T)
+    seg_n = ar.cnts[0]
+    log(f"eb={eb:.6g} segments={ar.cnts[0]}x{P}")
+
+    def timed(fn, reps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps
+
+    # compress every rank's segments (inputs of the timed step)
+    t_comp = timed(lambda: ar.compress_all(xs), 2)
+    streams = ar.streams
+    codec.check_errors()
+    in_sizes = [[s.size for s in row] for row in streams]
+    total_c = sum(sum(r) for r in in_sizes)
+    cr = P * count * 4 / total_c
+    log(f"compressed: total {total_c / 1e6:.1f} MB, CR {cr:.2f}, compress {t_comp * 1e3:.2f} ms")
+
+    outs = [codec.empty_stream(seg_n, L, T) for _ in range(P)]
+
+    def step_fused():
+        return ar.reduce_scatter_fused(outs)
+
+    # warm-up
+    for _ in range(args.warmup):
+        step_fused()
+    torch.cuda.synchronize()
+    codec.check_errors()
+
+    # timed region: K steps, per-launch HIP events on the launch stream
+    stream = torch.cuda.current_stream()
+    ev = [[(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(P)]
+          for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        for j in range(P):
+            ev[k][j][0].record(stream)
+            codec.homsum([streams[r][j] for r in range(P)], out=outs[j])
+            ev[k][j][1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    codec.check_errors()
+    ms_per_step = wall / args.steps * 1e3
+    launch_ms = [ev[k][j][0].elapsed_time(ev[k][j][1]) for k in range(args.steps) for j in range(P)]
+    avg_launch_s = float(np.mean(launch_ms)) / 1e3
+    out_sizes = [o.size for o in outs]
+    alg_bytes = [sum(in_sizes[r][j] for r in range(P)) + out_sizes[j] for j in range(P)]
+    alg_per_launch = float(np.mean(alg_bytes))
+    achieved = alg_per_launch / avg_launch_s / 1e9
+    input_gb = P * count * 4 / 1e9
+    value = input_gb / (ms_per_step / 1e3)
+    log(f"fused: {ms_per_step:.3f} ms/step, launch {avg_launch_s * 1e6:.1f} us, {achieved:.0f} GB/s algorithmic")
+
+    # plain per-call path: reference ring order, pairwise launches
+    scratch = ar.ring_scratch()
+    ring_res = ar.reduce_scatter_ring(scratch)
+    torch.cuda.synchronize()
+    codec.check_errors()
+    same = all(torch.equal(a.data[: a.size], b.data[: b.size]) for a, b in zip(ring_res, outs))
+    t_ring = timed(lambda: ar.reduce_scatter_ring(scratch), max(2, args.steps // 2))
+    plain_value = input_gb / t_ring
+
+    # decompress + quality of the reduced field
+    y = torch.empty(count, dtype=torch.float32, device="cuda")
+    t_dec = timed(lambda: ar.decompress_all(outs, out=y), 3)
+    codec.check_errors()
+    truth = torch.zeros(count, dtype=torch.float64, device="cuda")
+    for x in xs:
+        truth += x.double()
+    err = (y.double() - truth).abs()
+    max_err = float(err.max().item())
+    n_over = int((err > P * eb).sum().item())
+    del truth, err
+
+    # full staged allreduce (compress all + fused reduce-scatter + decompress all)
+    t_all = timed(lambda: (ar.compress_all(xs), ar.reduce_scatter_fused(outs), ar.decompress_all(outs, out=y)), 2)
+
+    roof = {
+        "bound": "hbm",
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBPS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBPS, 4),
+        "traffic": None,
+        "kernel": "zk::k_homsum",
+        "alg_bytes_per_launch": int(alg_per_launch),
+        "avg_launch_us": round(avg_launch_s * 1e6, 2),
+    }
+    if os.path.exists(args.traffic):
+        with open(args.traffic) as f:
+            tr = json.load(f)
+        roof["traffic"] = tr.get("hbm_bytes_per_launch")
+        roof["traffic_source"] = os.path.relpath(args.traffic, ROOT)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline")
+        cpu = cpu_baseline([streams[r][0] for r in range(P)], seg_n, L, T,
+                           outs[0].to_host(), args.cpu_seconds)
+
+    res = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"hZCCL fused homomorphic reduce-scatter: {P} simulated ranks x {side}^3 fp32 "
+                         f"GRF k^-3 (+N(0,0.05 std) per rank), rel eb {args.rel_eb:g}, L={L}, T={T}"),
+            "ranks": P, "elements_per_rank": count, "segment_elements": seg_n,
+            "L": L, "T": T, "abs_eb": eb, "rel_eb": args.rel_eb,
+        },
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "plain_value": round(plain_value, 2),
+        "plain_ms_per_step": round(t_ring * 1e3, 3),
+        "fused_vs_plain": round(value / plain_value, 2),
+        "ring_bytes_equal_fused": bool(same),
+        "compression_ratio": round(cr, 3),
+        "compress_GBps_input": round(P * count * 4 / 1e9 / t_comp, 1),
+        "decompress_GBps_output": round(count * 4 / 1e9 / t_dec, 1),
+        "allreduce_ms": round(t_all * 1e3, 3),
+        "max_abs_err": max_err,
+        "max_err_over_P_eb": round(max_err / (P * eb), 4),
+        "n_err_over_P_eb": n_over,
+    }
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

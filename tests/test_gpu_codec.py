@@ -191,8 +191,13 @@ def test_dropin_host_pointers(capi):
 def test_dropin_record_counters(capi):
     n, eb, L, T = 100_000, 1e-3, 36, 4
     capi.zccl_set_num_chunks(T)
-    x = gen.make("slabs", n, 2)
-    y = gen.perturb(x, 3)
+    # x is zero on the first half, y on the second and fourth quarters, so
+    # blocks meet all four cases: (0,0), (0,y), (x,0), (x,y)
+    x = gen.make("walk", n, 2)
+    y = gen.make("walk", n, 3)
+    x[: n // 2] = 0.0
+    y[n // 4: n // 2] = 0.0
+    y[3 * n // 4:] = 0.0
     cx, cy = O.compress(x, eb, L, T), O.compress(y, eb, L, T)
     a = np.zeros(4 * n, dtype=np.uint8); a[: cx.size] = cx
     b = np.zeros(4 * n, dtype=np.uint8); b[: cy.size] = cy

@@ -1,0 +1,118 @@
+"""hZCCL allreduce data flow staged on one GPU.
+
+The reference's MPI_Allreduce_ZCCL_RI2_mt_oa_ho_record
+(ZCCL/src/ZCCL_ring_ho.c:177-317) runs, on each of P ranks:
+
+1. split ``count`` values into P segments of ceil(count/P) (the last short,
+   :218-233) and compress every segment of its own data (:247-256);
+2. P-1 ring steps: receive the partial compressed sum of segment
+   (rank-2-i) mod P from the left neighbour, add the own compressed segment
+   homomorphically in place (:258-290) -- rank r ends owning segment r;
+3. compressed ring allgather of the owned segments (:292, :12-175) and
+   decompression of all P segments (:163-169).
+
+Here the P ranks are P sets of device-resident buffers on one GPU and no
+bytes travel between GPUs.  Two forms of step 2:
+
+* ``reduce_scatter_ring``  -- the reference's order: P-1 pairwise adds per
+  segment, one launch each (the plain per-call path);
+* ``reduce_scatter_fused`` -- one k=P homomorphic-sum launch per segment.
+  The encoding is canonical, so both give the same bytes.
+
+Step 3's allgather moves the P owned streams to every rank; on one GPU every
+"rank" reads the same resident streams, so the allgather is the identity and
+``decompress_all`` decompresses each segment once into the full output.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+from typing import List, Optional, Sequence
+
+import torch
+
+from . import codec
+
+
+def segments(count: int, P: int):
+    """(cnts, displs) as ZCCL/src/ZCCL_ring_ho.c:218-233 computes them."""
+    cnts = [0] * P
+    total = 0
+    for i in range(P):
+        c = (count + P - 1) // P
+        if total + c > count:
+            cnts[i] = count - total
+            break
+        cnts[i] = c
+        total += c
+    displs = [0] * P
+    for i in range(1, P):
+        displs[i] = displs[i - 1] + cnts[i - 1]
+    return cnts, displs
+
+
+def ring_order(P: int, j: int) -> List[int]:
+    """Ranks in the order the ring adds their copy of segment j: rank j+1
+    sends its raw segment at step 0, ranks j+2, ..., j+P (= j) each add theirs
+    (recv_rank = (r-2-i) mod P at step i, ZCCL/src/ZCCL_ring_ho.c:259-260)."""
+    return [(j + 1 + i) % P for i in range(P)]
+
+
+@dataclass
+class StagedAllreduce:
+    P: int
+    count: int
+    eb: float
+    L: int = 36
+    T: int = 16
+
+    def __post_init__(self):
+        self.cnts, self.displs = segments(self.count, self.P)
+        # a segment of 0 elements carries no stream (count < P); the
+        # reference would also produce none for it
+        self.active = [j for j in range(self.P) if self.cnts[j] >= self.T]
+        if len(self.active) != self.P:
+            raise ValueError("every segment must hold at least T elements")
+        self.streams: Optional[List[List[codec.DeviceStream]]] = None
+
+    # step 1
+    def compress_all(self, xs: Sequence[torch.Tensor]) -> List[List[codec.DeviceStream]]:
+        if len(xs) != self.P:
+            raise ValueError("one input per rank")
+        if self.streams is None:
+            self.streams = [[codec.empty_stream(self.cnts[j], self.L, self.T) for j in range(self.P)]
+                            for _ in range(self.P)]
+        for r, x in enumerate(xs):
+            for j in range(self.P):
+                seg = x[self.displs[j]: self.displs[j] + self.cnts[j]]
+                codec.compress(seg, self.eb, self.L, self.T, out=self.streams[r][j])
+        return self.streams
+
+    # step 2, fused
+    def reduce_scatter_fused(self, outs: Optional[List[codec.DeviceStream]] = None) -> List[codec.DeviceStream]:
+        res = []
+        for j in range(self.P):
+            ops = [self.streams[r][j] for r in range(self.P)]
+            res.append(codec.homsum(ops, out=outs[j] if outs else None))
+        return res
+
+    # step 2, reference order
+    def reduce_scatter_ring(self, scratch: Optional[List[List[codec.DeviceStream]]] = None) -> List[codec.DeviceStream]:
+        res = []
+        for j in range(self.P):
+            order = ring_order(self.P, j)
+            acc = self.streams[order[0]][j]
+            for i, r in enumerate(order[1:]):
+                out = scratch[j][i % 2] if scratch else None
+                acc = codec.homsum([acc, self.streams[r][j]], out=out)
+            res.append(acc)
+        return res
+
+    def ring_scratch(self) -> List[List[codec.DeviceStream]]:
+        return [[codec.empty_stream(self.cnts[j], self.L, self.T) for _ in range(2)] for j in range(self.P)]
+
+    # step 3
+    def decompress_all(self, sums: Sequence[codec.DeviceStream], out: Optional[torch.Tensor] = None) -> torch.Tensor:
+        y = out if out is not None else torch.empty(self.count, dtype=torch.float32, device="cuda")
+        for j in range(self.P):
+            codec.decompress(sums[j], self.eb, out=y[self.displs[j]: self.displs[j] + self.cnts[j]])
+        return y
