@@ -156,6 +156,33 @@ int zccl_parse_dev(const unsigned char *in, size_t in_cap, size_t nbEle, int blo
     int nchunks, void *index, unsigned long long *d_size, void *workspace, size_t workspace_bytes,
     void *stream);
 
+/* ------------------------------------------------------------ batches
+ *
+ * A batch plan puts many independent operations of one kind into a single
+ * launch: every job keeps its own stream geometry and lookback chain, a
+ * block-to-job table maps workgroups to jobs.  create() validates the jobs
+ * and copies their descriptors to device memory once (it synchronises);
+ * launch() is one kernel launch on `stream` that may be repeated or captured
+ * in a hipGraph while the buffers the plan names stay alive.  The workspace
+ * must hold zccl_batch_tiles() tiles.  No reference counterpart: the
+ * reference runs each codec call separately. */
+typedef struct zccl_batch zccl_batch;
+
+/* njobs fused sums of k operands each; ins / indices are job-major
+ * (job j operand r at [j*k + r]). */
+zccl_batch *zccl_batch_homsum_create(int njobs, int k, const size_t *nbEle, int blockSize,
+    int nchunks, const unsigned char *const *ins, const void *const *indices,
+    unsigned char *const *outs, const size_t *out_caps, void *const *out_indices, int *status);
+zccl_batch *zccl_batch_compress_create(int njobs, const size_t *nbEle, const float *absErrBounds,
+    int blockSize, int nchunks, const float *const *xs, unsigned char *const *outs,
+    const size_t *out_caps, void *const *out_indices, int *status);
+zccl_batch *zccl_batch_decompress_create(int njobs, const size_t *nbEle, const float *absErrBounds,
+    int blockSize, int nchunks, const unsigned char *const *ins, const void *const *indices,
+    float *const *outs, int *status);
+size_t zccl_batch_tiles(const zccl_batch *batch);
+int zccl_batch_launch(const zccl_batch *batch, void *workspace, size_t workspace_bytes, void *stream);
+void zccl_batch_destroy(zccl_batch *batch);
+
 /* Block-case histogram of two indexed streams (see the _record add). */
 int zccl_case_count_dev(const void *index1, const void *index2, size_t nbEle, int blockSize,
     int nchunks, unsigned long long *d_counts4, void *stream);

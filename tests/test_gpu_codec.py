@@ -271,3 +271,38 @@ def test_errors_are_reported_not_faulted(codec, capi):
     dec = np.zeros(4, dtype=np.float32)
     capi.ZCCL_float_decompress_openmp_threadblock_arg(dec.ctypes.data, 4, eb, L, bad.ctypes.data)
     assert capi.zccl_last_error() != 0
+
+
+# ------------------------------------------------------------ batch plans
+
+def test_batch_plans_uniform_and_mixed(codec):
+    from zccl_amd import batch
+
+    L, T, eb = 36, 8, 1e-3
+    for sizes in ([100_000] * 5, [7, 40, 1000, 123_457, 9, 60_001]):
+        say("batch sizes", sizes)
+        xs = [gen.make("grf", n, 11 + i) for i, n in enumerate(sizes) if n >= T]
+        ns = [x.size for x in xs]
+        dxs = [dev(x) for x in xs]
+        ebs = [eb * (1 + i) for i in range(len(xs))]
+        outs = [codec.empty_stream(n, L, T) for n in ns]
+        plan = batch.compress_plan(dxs, ebs, outs)
+        plan.launch()
+        plan.launch()  # replays must give the same bytes
+        ys = [gen.perturb(x, 50 + i) for i, x in enumerate(xs)]
+        sy = [codec.compress(dev(y), e, L, T) for y, e in zip(ys, ebs)]
+        sums = [codec.empty_stream(n, L, T) for n in ns]
+        hp = batch.homsum_plan([[a, b] for a, b in zip(outs, sy)], sums)
+        hp.launch()
+        decs = [torch.empty(n, dtype=torch.float32, device="cuda") for n in ns]
+        dp = batch.decompress_plan(sums, ebs, decs)
+        dp.launch()
+        torch.cuda.synchronize()
+        codec.check_errors()
+        for i, n in enumerate(ns):
+            cx = O.compress(xs[i], ebs[i], L, T)
+            cy = O.compress(ys[i], ebs[i], L, T)
+            assert np.array_equal(outs[i].to_host(), cx), (sizes, i)
+            ch = O.homsum([cx, cy], n, L, T)
+            assert np.array_equal(sums[i].to_host(), ch), (sizes, i)
+            assert np.array_equal(bits(decs[i].cpu().numpy()), bits(O.decompress(ch, n, ebs[i], L, T)))

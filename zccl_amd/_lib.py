@@ -36,6 +36,8 @@ EXPORTS = (
     "zccl_workspace_init", "zccl_workspace_errors",
     "zccl_compress_dev", "zccl_decompress_dev", "zccl_homsum_dev", "zccl_parse_dev",
     "zccl_case_count_dev",
+    "zccl_batch_homsum_create", "zccl_batch_compress_create", "zccl_batch_decompress_create",
+    "zccl_batch_tiles", "zccl_batch_launch", "zccl_batch_destroy",
 )
 
 
@@ -94,6 +96,12 @@ def lib() -> ctypes.CDLL:
         "zccl_homsum_dev": ([vp, vp, i32, sz, i32, i32, vp, sz, vp, vp, vp, sz, vp], i32),
         "zccl_parse_dev": ([vp, sz, sz, i32, i32, vp, vp, vp, sz, vp], i32),
         "zccl_case_count_dev": ([vp, vp, sz, i32, i32, vp, vp], i32),
+        "zccl_batch_homsum_create": ([i32, i32, vp, i32, i32, vp, vp, vp, vp, vp, i32_p], vp),
+        "zccl_batch_compress_create": ([i32, vp, vp, i32, i32, vp, vp, vp, vp, i32_p], vp),
+        "zccl_batch_decompress_create": ([i32, vp, vp, i32, i32, vp, vp, vp, i32_p], vp),
+        "zccl_batch_tiles": ([vp], sz),
+        "zccl_batch_launch": ([vp, vp, sz, vp], i32),
+        "zccl_batch_destroy": ([vp], None),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

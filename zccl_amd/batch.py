@@ -1,0 +1,105 @@
+"""Batch plans: many codec operations in one launch (zccl_batch_* in
+include/zccl_amd.h).
+
+A plan is built once (descriptors uploaded to device memory) and launched any
+number of times; each launch is a single kernel on the current stream, so it
+can be replayed inside a captured CUDA/HIP graph.  The plan keeps references to
+the tensors it names so they outlive it.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import List, Optional, Sequence
+
+import torch
+
+from . import _lib
+from ._lib import check, lib
+from .codec import DeviceStream, Workspace, _stream_handle, default_workspace
+
+
+class BatchPlan:
+    def __init__(self, handle: int, keep: list, outs: Optional[List[DeviceStream]] = None):
+        if not handle:
+            raise _lib.ZcclError(-1, "batch create")
+        self.handle = ctypes.c_void_p(handle)
+        self.tiles = int(lib().zccl_batch_tiles(self.handle))
+        self._keep = keep
+        self.outs = outs or []
+
+    def launch(self, ws: Optional[Workspace] = None) -> None:
+        w = ws if ws is not None else default_workspace(self.tiles)
+        if w.max_tiles < self.tiles:
+            raise _lib.ZcclError(-6, f"workspace holds {w.max_tiles} tiles, batch needs {self.tiles}")
+        for o in self.outs:
+            o.invalidate()
+        check(lib().zccl_batch_launch(self.handle, w.ptr, w.nbytes, _stream_handle()), "batch launch")
+
+    def close(self) -> None:
+        if self.handle is not None:
+            lib().zccl_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _ptrs(ts) -> ctypes.Array:
+    return (ctypes.c_void_p * len(ts))(*[t.data_ptr() for t in ts])
+
+
+def homsum_plan(jobs: Sequence[Sequence[DeviceStream]], outs: Sequence[DeviceStream]) -> BatchPlan:
+    """jobs[j] is the list of k operand streams summed into outs[j]."""
+    nj = len(jobs)
+    k = len(jobs[0])
+    if any(len(j) != k for j in jobs) or len(outs) != nj:
+        raise ValueError("every job needs k operands and one output")
+    L, T = jobs[0][0].L, jobs[0][0].T
+    for j, ops in enumerate(jobs):
+        for s in ops:
+            if (s.n, s.L, s.T) != (outs[j].n, L, T):
+                raise ValueError("operands of a job must share (n, L, T) with its output")
+    n = (ctypes.c_size_t * nj)(*[o.n for o in outs])
+    ins = _ptrs([s.data for ops in jobs for s in ops])
+    ixs = _ptrs([s.index for ops in jobs for s in ops])
+    od = _ptrs([o.data for o in outs])
+    ocap = (ctypes.c_size_t * nj)(*[o.data.numel() for o in outs])
+    oix = _ptrs([o.index for o in outs])
+    st = ctypes.c_int(0)
+    h = lib().zccl_batch_homsum_create(nj, k, n, L, T, ins, ixs, od, ocap, oix, ctypes.byref(st))
+    check(st.value, "batch homsum create")
+    return BatchPlan(h, [list(jobs), list(outs)], list(outs))
+
+
+def compress_plan(xs: Sequence[torch.Tensor], ebs: Sequence[float], outs: Sequence[DeviceStream]) -> BatchPlan:
+    nj = len(xs)
+    L, T = outs[0].L, outs[0].T
+    for x, o in zip(xs, outs):
+        if x.numel() != o.n or x.dtype != torch.float32 or not x.is_contiguous() or (o.L, o.T) != (L, T):
+            raise ValueError("inputs must be contiguous float32 matching their output streams")
+    n = (ctypes.c_size_t * nj)(*[o.n for o in outs])
+    eb = (ctypes.c_float * nj)(*ebs)
+    st = ctypes.c_int(0)
+    h = lib().zccl_batch_compress_create(nj, n, eb, L, T, _ptrs(xs), _ptrs([o.data for o in outs]),
+                                          (ctypes.c_size_t * nj)(*[o.data.numel() for o in outs]),
+                                          _ptrs([o.index for o in outs]), ctypes.byref(st))
+    check(st.value, "batch compress create")
+    return BatchPlan(h, [list(xs), list(outs)], list(outs))
+
+
+def decompress_plan(streams: Sequence[DeviceStream], ebs: Sequence[float], outs: Sequence[torch.Tensor]) -> BatchPlan:
+    nj = len(streams)
+    L, T = streams[0].L, streams[0].T
+    for s, y in zip(streams, outs):
+        if y.numel() != s.n or y.dtype != torch.float32 or not y.is_contiguous() or (s.L, s.T) != (L, T):
+            raise ValueError("outputs must be contiguous float32 matching their streams")
+    n = (ctypes.c_size_t * nj)(*[s.n for s in streams])
+    eb = (ctypes.c_float * nj)(*ebs)
+    st = ctypes.c_int(0)
+    h = lib().zccl_batch_decompress_create(nj, n, eb, L, T, _ptrs([s.data for s in streams]),
+                                            _ptrs([s.index for s in streams]), _ptrs(outs), ctypes.byref(st))
+    check(st.value, "batch decompress create")
+    return BatchPlan(h, [list(streams), list(outs)])

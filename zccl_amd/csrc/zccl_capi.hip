@@ -120,10 +120,16 @@ extern "C" size_t zccl_index_bytes(size_t n, int L, int T)
     return geom_of(n, L, T, &g) ? (size_t) g.ix_bytes : 0;
 }
 
+// Tiles of the finest tiling any kernel uses for (n, L, T): a workspace of
+// this many tiles serves every operation on such a stream.
 extern "C" size_t zccl_num_tiles(size_t n, int L, int T)
 {
     Geom g;
-    return geom_of(n, L, T, &g) ? (size_t) g.ntiles : 0;
+    if (!geom_of(n, L, T, &g))
+        return 0;
+    uint64_t a = make_tiling(g, kCompressGroups).ntiles, b = make_tiling(g, kHomsumGroups).ntiles,
+             c = make_tiling(g, kDecompressGroups).ntiles;
+    return (size_t) std::max(a, std::max(b, c));
 }
 
 extern "C" size_t zccl_num_blocks(size_t n, int L, int T)
@@ -153,20 +159,37 @@ static uint64_t worst_segment(const Geom& g, uint32_t s)
 
 // ------------------------------------------------------------ device API
 
-static int check_ws(void* ws, size_t ws_bytes, const Geom& g)
+static int check_ws(void* ws, size_t ws_bytes, uint64_t tiles)
 {
     if (!ws)
         return ZCCL_EWORKSPACE;
-    if (ws_bytes < workspace_bytes(g.ntiles))
+    if (ws_bytes < workspace_bytes(tiles))
         return ZCCL_EWORKSPACE;
     return ZCCL_OK;
+}
+
+static double inverse_bound(float eb)
+{
+    volatile float one = 1.0f;
+    float invf = one / eb;  // FP32 division, as the reference's int / float
+    return (double) invf;
+}
+
+static OpParams op_params(const Tiling& tl, double inv, float eb, int k)
+{
+    OpParams p;
+    p.inv = inv;
+    p.eb = eb;
+    p.k = k;
+    p.stage_bytes = stage_bytes_for(tl);
+    return p;
 }
 
 extern "C" int zccl_workspace_init(void* ws, size_t ws_bytes, void* stream)
 {
     if (!ws || ws_bytes < sizeof(WsHeader))
         return ZCCL_EINVAL;
-    uint64_t cap = (ws_bytes - sizeof(WsHeader)) / 16;
+    uint64_t cap = workspace_capacity(ws_bytes);
     HIP_TRY(launch_ws_init((WsHeader*) ws, cap, (hipStream_t) stream));
     return ZCCL_OK;
 }
@@ -188,20 +211,19 @@ extern "C" int zccl_compress_dev(const float* x, size_t n, float eb, int L, int 
     Geom g;
     if (!x || !out || !index || !geom_of(n, L, T, &g))
         return ZCCL_EINVAL;
-    int rc = check_ws(ws, ws_bytes, g);
+    CompressArgs a;
+    a.job.g = g;
+    a.job.tl = make_tiling(g, kCompressGroups);
+    int rc = check_ws(ws, ws_bytes, a.job.tl.ntiles);
     if (rc)
         return rc;
-    CompressArgs a;
-    a.g = g;
-    a.x = x;
-    volatile float one = 1.0f;
-    float invf = one / eb;  // FP32 division, as the reference's int / float
-    a.inv = (double) invf;
-    a.out = out;
-    a.out_cap = out_cap;
-    a.index = (uint8_t*) index;
+    a.job.p = op_params(a.job.tl, inverse_bound(eb), eb, 1);
+    a.job.io.x = x;
+    a.job.io.out = out;
+    a.job.io.out_cap = out_cap;
+    a.job.io.index = (uint8_t*) index;
+    a.job.io.size_out = d_size;
     a.ws = (WsHeader*) ws;
-    a.size_out = d_size;
     HIP_TRY(launch_compress(a, (hipStream_t) stream));
     return ZCCL_OK;
 }
@@ -212,15 +234,16 @@ extern "C" int zccl_decompress_dev(const unsigned char* in, const void* index, s
     Geom g;
     if (!in || !index || !out || !geom_of(n, L, T, &g))
         return ZCCL_EINVAL;
-    int rc = check_ws(ws, ws_bytes, g);
+    DecompressArgs a;
+    a.job.g = g;
+    a.job.tl = make_tiling(g, kDecompressGroups);
+    int rc = check_ws(ws, ws_bytes, a.job.tl.ntiles);
     if (rc)
         return rc;
-    DecompressArgs a;
-    a.g = g;
-    a.bytes = in;
-    a.index = (const uint8_t*) index;
-    a.eb = eb;
-    a.out = out;
+    a.job.p = op_params(a.job.tl, 0.0, eb, 1);
+    a.job.io.in[0].bytes = in;
+    a.job.io.in[0].index = (const uint8_t*) index;
+    a.job.io.out = out;
     a.ws = (WsHeader*) ws;
     HIP_TRY(launch_decompress(a, (hipStream_t) stream));
     return ZCCL_OK;
@@ -233,26 +256,235 @@ extern "C" int zccl_homsum_dev(const unsigned char* const* ins, const void* cons
     Geom g;
     if (!ins || !indices || k < 1 || k > kMaxOperands || !out || !index || !geom_of(n, L, T, &g))
         return ZCCL_EINVAL;
-    int rc = check_ws(ws, ws_bytes, g);
-    if (rc)
-        return rc;
     HomsumArgs a;
     memset(&a, 0, sizeof(a));
-    a.g = g;
-    a.k = k;
+    a.job.g = g;
+    a.job.tl = make_tiling(g, kHomsumGroups);
+    int rc = check_ws(ws, ws_bytes, a.job.tl.ntiles);
+    if (rc)
+        return rc;
+    a.job.p = op_params(a.job.tl, 0.0, 0.0f, k);
     for (int r = 0; r < k; r++) {
         if (!ins[r] || !indices[r] || ins[r] == out)
             return ZCCL_EINVAL;
-        a.in[r].bytes = ins[r];
-        a.in[r].index = (const uint8_t*) indices[r];
+        a.job.io.in[r].bytes = ins[r];
+        a.job.io.in[r].index = (const uint8_t*) indices[r];
     }
-    a.out = out;
-    a.out_cap = out_cap;
-    a.index = (uint8_t*) index;
+    a.job.io.out = out;
+    a.job.io.out_cap = out_cap;
+    a.job.io.index = (uint8_t*) index;
+    a.job.io.size_out = d_size;
     a.ws = (WsHeader*) ws;
-    a.size_out = d_size;
     HIP_TRY(launch_homsum(a, (hipStream_t) stream));
     return ZCCL_OK;
+}
+
+// ------------------------------------------------------------ batch plans
+
+struct zccl_batch {
+    BatchKind kind;
+    BatchArgs args;
+    uint32_t threads;
+    size_t lds;
+    void* d_jobs;
+    unsigned long long* d_tile0;
+};
+
+namespace {
+
+template <typename IO, typename Job>
+zccl_batch* batch_upload(BatchKind kind, const std::vector<Job>& jobs, uint32_t chains, int* status)
+{
+    const size_t nj = jobs.size();
+    bool uniform = true;
+    for (size_t j = 1; j < nj; j++) {
+        const Job& a = jobs[0];
+        const Job& b = jobs[j];
+        if (memcmp(&a.g, &b.g, sizeof(Geom)) || memcmp(&a.tl, &b.tl, sizeof(Tiling)) ||
+            memcmp(&a.p, &b.p, sizeof(OpParams)))
+            uniform = false;
+    }
+    std::vector<unsigned long long> tile0(nj + 1, 0);
+    uint32_t threads = 64;
+    size_t lds = 0;
+    for (size_t j = 0; j < nj; j++) {
+        tile0[j + 1] = tile0[j] + jobs[j].tl.ntiles;
+        threads = std::max(threads, jobs[j].tl.threads);
+        size_t need = kind == BatchKind::Compress ? compress_lds_bytes(jobs[j].tl)
+                      : kind == BatchKind::Homsum ? operand_lds_bytes(jobs[j].tl, jobs[j].p.k, jobs[j].p.stage_bytes)
+                                                  : operand_lds_bytes(jobs[j].tl, 1, jobs[j].p.stage_bytes);
+        lds = std::max(lds, need);
+    }
+    zccl_batch* b = new zccl_batch();
+    b->kind = kind;
+    b->threads = threads;
+    b->lds = lds;
+    b->d_jobs = nullptr;
+    b->d_tile0 = nullptr;
+    size_t desc_bytes = uniform ? nj * sizeof(IO) : nj * sizeof(Job);
+    if (hipMalloc(&b->d_jobs, desc_bytes) != hipSuccess || hipMalloc(&b->d_tile0, 8 * (nj + 1)) != hipSuccess) {
+        (void) hipGetLastError();
+        zccl_batch_destroy(b);
+        *status = ZCCL_ENOMEM;
+        return nullptr;
+    }
+    hipError_t e;
+    if (uniform) {
+        std::vector<IO> ios(nj);
+        for (size_t j = 0; j < nj; j++)
+            ios[j] = jobs[j].io;
+        e = hipMemcpy(b->d_jobs, ios.data(), desc_bytes, hipMemcpyHostToDevice);
+    } else {
+        e = hipMemcpy(b->d_jobs, jobs.data(), desc_bytes, hipMemcpyHostToDevice);
+    }
+    if (e == hipSuccess)
+        e = hipMemcpy(b->d_tile0, tile0.data(), 8 * (nj + 1), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void) hipGetLastError();
+        zccl_batch_destroy(b);
+        *status = ZCCL_EDEVICE;
+        return nullptr;
+    }
+    BatchArgs& a = b->args;
+    memset(&a, 0, sizeof(a));
+    a.jobs = b->d_jobs;
+    a.job_tile0 = b->d_tile0;
+    a.njobs = (uint32_t) nj;
+    a.uniform = uniform;
+    a.interleave = uniform && nj > 1;
+    a.total_chains = chains;
+    a.total_tiles = tile0[nj];
+    if (uniform) {
+        a.g = jobs[0].g;
+        a.tl = jobs[0].tl;
+        a.p = jobs[0].p;
+    }
+    *status = ZCCL_OK;
+    return b;
+}
+
+}  // namespace
+
+extern "C" zccl_batch* zccl_batch_homsum_create(int njobs, int k, const size_t* nbEle, int L, int T,
+    const unsigned char* const* ins, const void* const* indices, unsigned char* const* outs,
+    const size_t* out_caps, void* const* out_indices, int* status)
+{
+    int dummy;
+    int* st = status ? status : &dummy;
+    if (njobs < 1 || k < 1 || k > kMaxOperands || !nbEle || !ins || !indices || !outs || !out_caps ||
+        !out_indices) {
+        *st = ZCCL_EINVAL;
+        return nullptr;
+    }
+    std::vector<HomsumJob> jobs(njobs);
+    for (int j = 0; j < njobs; j++) {
+        HomsumJob& h = jobs[j];
+        memset(&h, 0, sizeof(h));
+        if (!geom_of(nbEle[j], L, T, &h.g) || !outs[j] || !out_indices[j]) {
+            *st = ZCCL_EINVAL;
+            return nullptr;
+        }
+        h.tl = make_tiling(h.g, kHomsumGroups);
+        h.p = op_params(h.tl, 0.0, 0.0f, k);
+        for (int r = 0; r < k; r++) {
+            const unsigned char* in = ins[(size_t) j * k + r];
+            if (!in || !indices[(size_t) j * k + r] || in == outs[j]) {
+                *st = ZCCL_EINVAL;
+                return nullptr;
+            }
+            h.io.in[r].bytes = in;
+            h.io.in[r].index = (const uint8_t*) indices[(size_t) j * k + r];
+        }
+        h.io.out = outs[j];
+        h.io.out_cap = out_caps[j];
+        h.io.index = (uint8_t*) out_indices[j];
+        h.io.size_out = nullptr;
+    }
+    return batch_upload<HomsumIO>(BatchKind::Homsum, jobs, (uint32_t) njobs, st);
+}
+
+extern "C" zccl_batch* zccl_batch_compress_create(int njobs, const size_t* nbEle, const float* ebs, int L,
+    int T, const float* const* xs, unsigned char* const* outs, const size_t* out_caps,
+    void* const* out_indices, int* status)
+{
+    int dummy;
+    int* st = status ? status : &dummy;
+    if (njobs < 1 || !nbEle || !ebs || !xs || !outs || !out_caps || !out_indices) {
+        *st = ZCCL_EINVAL;
+        return nullptr;
+    }
+    std::vector<CompressJob> jobs(njobs);
+    for (int j = 0; j < njobs; j++) {
+        CompressJob& c = jobs[j];
+        memset(&c, 0, sizeof(c));
+        if (!geom_of(nbEle[j], L, T, &c.g) || !xs[j] || !outs[j] || !out_indices[j]) {
+            *st = ZCCL_EINVAL;
+            return nullptr;
+        }
+        c.tl = make_tiling(c.g, kCompressGroups);
+        c.p = op_params(c.tl, inverse_bound(ebs[j]), ebs[j], 1);
+        c.io.x = xs[j];
+        c.io.out = outs[j];
+        c.io.out_cap = out_caps[j];
+        c.io.index = (uint8_t*) out_indices[j];
+        c.io.size_out = nullptr;
+    }
+    return batch_upload<CompressIO>(BatchKind::Compress, jobs, (uint32_t) njobs, st);
+}
+
+extern "C" zccl_batch* zccl_batch_decompress_create(int njobs, const size_t* nbEle, const float* ebs,
+    int L, int T, const unsigned char* const* ins, const void* const* indices, float* const* outs,
+    int* status)
+{
+    int dummy;
+    int* st = status ? status : &dummy;
+    if (njobs < 1 || !nbEle || !ebs || !ins || !indices || !outs) {
+        *st = ZCCL_EINVAL;
+        return nullptr;
+    }
+    std::vector<DecompressJob> jobs(njobs);
+    uint32_t chains = 0;
+    for (int j = 0; j < njobs; j++) {
+        DecompressJob& d = jobs[j];
+        memset(&d, 0, sizeof(d));
+        if (!geom_of(nbEle[j], L, T, &d.g) || !ins[j] || !indices[j] || !outs[j]) {
+            *st = ZCCL_EINVAL;
+            return nullptr;
+        }
+        d.tl = make_tiling(d.g, kDecompressGroups);
+        d.p = op_params(d.tl, 0.0, ebs[j], 1);
+        d.io.in[0].bytes = ins[j];
+        d.io.in[0].index = (const uint8_t*) indices[j];
+        d.io.out = outs[j];
+        chains += d.g.S;
+    }
+    return batch_upload<DecompressIO>(BatchKind::Decompress, jobs, chains, st);
+}
+
+extern "C" size_t zccl_batch_tiles(const zccl_batch* b) { return b ? (size_t) b->args.total_tiles : 0; }
+
+extern "C" int zccl_batch_launch(const zccl_batch* b, void* ws, size_t ws_bytes, void* stream)
+{
+    if (!b)
+        return ZCCL_EINVAL;
+    int rc = check_ws(ws, ws_bytes, b->args.total_tiles);
+    if (rc)
+        return rc;
+    BatchArgs a = b->args;
+    a.ws = (WsHeader*) ws;
+    HIP_TRY(launch_batch(b->kind, a, b->threads, b->lds, (hipStream_t) stream));
+    return ZCCL_OK;
+}
+
+extern "C" void zccl_batch_destroy(zccl_batch* b)
+{
+    if (!b)
+        return;
+    if (b->d_jobs)
+        (void) hipFree(b->d_jobs);
+    if (b->d_tile0)
+        (void) hipFree(b->d_tile0);
+    delete b;
 }
 
 extern "C" int zccl_parse_dev(const unsigned char* in, size_t in_cap, size_t n, int L, int T,
@@ -457,7 +689,7 @@ int dropin_compress(unsigned char* outputBytes, const float* x, size_t* outSize,
     if (!c)
         return rc;
     std::lock_guard<std::mutex> lk(c->mu);
-    if ((rc = ensure_ws(*c, g.ntiles)) || (rc = ensure(c->small, 64)) || (rc = ensure(c->ix[0], g.ix_bytes)))
+    if ((rc = ensure_ws(*c, zccl_num_tiles(n, L, T))) || (rc = ensure(c->small, 64)) || (rc = ensure(c->ix[0], g.ix_bytes)))
         return rc;
     size_t avail_x = 0, avail_o = 0;
     bool xdev = device_ptr(x, &avail_x), odev = device_ptr(outputBytes, &avail_o);
@@ -505,7 +737,7 @@ int dropin_decompress(float* newData, size_t n, float eb, int L, int T, const un
     if (!c)
         return rc;
     std::lock_guard<std::mutex> lk(c->mu);
-    if ((rc = ensure_ws(*c, g.ntiles)))
+    if ((rc = ensure_ws(*c, zccl_num_tiles(n, L, T))))
         return rc;
     const unsigned char* din = nullptr;
     if ((rc = stage_stream(*c, g, cmp, cmpSize, sized, 0, &din)))
@@ -544,7 +776,7 @@ int dropin_add(unsigned char* out, size_t out_cap, bool out_cap_known, size_t* o
     if (!c)
         return rc;
     std::lock_guard<std::mutex> lk(c->mu);
-    if ((rc = ensure_ws(*c, g.ntiles)) || (rc = ensure(c->small, 64)) || (rc = ensure(c->ix[2], g.ix_bytes)))
+    if ((rc = ensure_ws(*c, zccl_num_tiles(n, L, T))) || (rc = ensure(c->small, 64)) || (rc = ensure(c->ix[2], g.ix_bytes)))
         return rc;
     const unsigned char* da = nullptr;
     const unsigned char* db = nullptr;

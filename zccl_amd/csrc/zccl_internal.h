@@ -1,5 +1,5 @@
-// zccl_internal.h -- stream geometry, tile mapping and buffer layouts shared by
-// the host launchers and the CDNA4 kernels.
+// zccl_internal.h -- stream geometry, tiling and buffer layouts shared by the
+// host launchers and the CDNA4 kernels.
 //
 // Wire format (reference: ZCCL/src/ZCCL_float.c:501-807):
 //   [T x u64 chunk offsets][chunk 0 payload]...[chunk T-1 payload]
@@ -12,15 +12,17 @@
 //
 // Device layout chosen here:
 //   * a "segment" is one seeded run of the Lorenzo chain (T regular chunks plus
-//     the optional remainder); segments are cut into tiles of G blocks and a
-//     tile never straddles two segments;
-//   * every segment owns at least one tile (an empty tile carries only the
-//     seed when the segment has no blocks);
+//     the optional remainder);
+//   * a segment's blocks are cut into index groups of IG blocks (IG = 64 for
+//     L <= 36); every segment owns at least one group (an empty one when it
+//     has no blocks);
+//   * a kernel tile is up to TG consecutive groups of one segment (a tile never
+//     straddles two segments);
 //   * a device-resident stream travels with an index: widths[nblocks] (one
-//     byte per block), tile_off[ntiles] (byte offset of each tile's first
+//     byte per block), group_off[ngroups] (byte offset of each group's first
 //     block), seed_off[S] (byte offset of each seed) and a meta word holding
-//     the stream size.  The index is what lets every kernel find its bytes
-//     without a sequential walk of the stream.
+//     the stream size.  With the index every kernel finds its bytes without a
+//     sequential walk of the stream.
 #pragma once
 
 #include <stddef.h>
@@ -34,9 +36,13 @@
 
 namespace zk {
 
-constexpr int kMaxOperands = 32;   // operands of one fused homomorphic-sum launch
-constexpr uint32_t kMaxL = 4096;   // block length supported by the tile kernels
-constexpr uint32_t kMaxThreads = 1024;
+constexpr int kMaxOperands = 32;     // operands of one fused homomorphic-sum launch
+constexpr uint32_t kMaxL = 4096;     // block length supported by the tile kernels
+constexpr uint32_t kUnitsPerThread = 4;
+constexpr uint32_t kTileUnits = 2304; // 4-element units per tile (at most)
+constexpr uint32_t kGroupUnits = 576;  // units per index group (at most)
+constexpr uint32_t kMaxTileBlocks = 256;
+constexpr uint32_t kStatusStride = 16; // u64 words between lookback status entries (one 128 B line each)
 
 struct Geom {
     uint64_t n;
@@ -46,12 +52,11 @@ struct Geom {
     uint64_t nbT, nbR;   // blocks per regular / remainder segment
     uint64_t nblocks;
     uint32_t U;          // 4-element units per block = ceil(L/4)
-    uint32_t G;          // blocks per tile
-    uint32_t threads;    // workgroup size: G*U rounded up to a wave
-    uint64_t tilesT, tilesR, ntiles;
-    uint32_t tile_bytes_max;  // worst-case encoded bytes of one tile
+    uint32_t IG;         // blocks per index group
+    uint64_t gT, gR;     // groups per regular / remainder segment
+    uint64_t ngroups;
     // index layout (byte offsets inside an index buffer)
-    uint64_t ix_tile, ix_seed, ix_meta, ix_bytes;
+    uint64_t ix_group, ix_seed, ix_meta, ix_bytes;
 };
 
 ZK_HD uint64_t ceil_div(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
@@ -66,16 +71,8 @@ ZK_HD uint32_t block_bytes(uint32_t b, uint32_t n)
     return 1 + ((n + 7) >> 3) + (b >> 3) * n + (((b & 7) * n + 7) >> 3);
 }
 
-// Blocks per tile for block length L: 64 while a tile fits 1024 threads.
-ZK_HD uint32_t tile_blocks(uint32_t L)
-{
-    uint32_t U = (L + 3) / 4;
-    uint32_t G = kMaxThreads / U;
-    return G > 64 ? 64 : (G < 1 ? 1 : G);
-}
-
-// Fill g for (n, L, T); returns false when the reference would be undefined
-// (n < T: ZCCL/src/ZCCL_float.c:543-544 underflows) or L is out of range.
+// Fill g for (n, L, T); false when the reference would be undefined (n < T:
+// ZCCL/src/ZCCL_float.c:543-544 underflows) or L is out of range.
 ZK_HD bool make_geom(uint64_t n, uint32_t L, uint32_t T, Geom* g)
 {
     if (L < 1 || L > kMaxL || T < 1 || n < T)
@@ -90,15 +87,13 @@ ZK_HD bool make_geom(uint64_t n, uint32_t L, uint32_t T, Geom* g)
     g->nbR = g->R ? ceil_div(g->R - 1, L) : 0;
     g->nblocks = (uint64_t) T * g->nbT + g->nbR;
     g->U = (L + 3) / 4;
-    g->G = tile_blocks(L);
-    uint32_t t = g->G * g->U;
-    g->threads = (uint32_t) align_up(t < 64 ? 64 : t, 64);
-    g->tilesT = g->nbT ? ceil_div(g->nbT, g->G) : 1;
-    g->tilesR = g->R ? (g->nbR ? ceil_div(g->nbR, g->G) : 1) : 0;
-    g->ntiles = (uint64_t) T * g->tilesT + g->tilesR;
-    g->tile_bytes_max = g->G * block_bytes(32, L);
-    g->ix_tile = align_up(g->nblocks, 16);
-    g->ix_seed = g->ix_tile + 8 * g->ntiles;
+    uint32_t ig = kGroupUnits / g->U;
+    g->IG = ig > 64 ? 64 : (ig < 1 ? 1 : ig);
+    g->gT = g->nbT ? ceil_div(g->nbT, g->IG) : 1;
+    g->gR = g->R ? (g->nbR ? ceil_div(g->nbR, g->IG) : 1) : 0;
+    g->ngroups = (uint64_t) T * g->gT + g->gR;
+    g->ix_group = align_up(g->nblocks, 16);
+    g->ix_seed = g->ix_group + 8 * g->ngroups;
     g->ix_meta = align_up(g->ix_seed + 8ull * g->S, 16);
     g->ix_bytes = g->ix_meta + 32;
     return true;
@@ -121,41 +116,86 @@ ZK_HD uint64_t worst_case_bytes(const Geom& g)
     return tot;
 }
 
+// Per-launch tiling: TG index groups per tile, up to kTileUnits units.
+struct Tiling {
+    uint32_t TG;         // groups per tile
+    uint32_t G;          // blocks per tile (TG * IG)
+    uint32_t threads;    // workgroup size
+    uint64_t tilesT, tilesR, ntiles;
+    uint32_t tile_bytes_max;  // worst-case encoded bytes of one tile
+};
+
+ZK_HD Tiling make_tiling(const Geom& g, uint32_t max_groups = 4)
+{
+    Tiling t;
+    uint32_t tg = kTileUnits / (g.IG * g.U);
+    if (tg > max_groups)
+        tg = max_groups;
+    if (tg < 1)
+        tg = 1;
+    while (tg > 1 && tg * g.IG > kMaxTileBlocks)
+        tg--;
+    t.TG = tg;
+    t.G = tg * g.IG;
+    uint32_t units = t.G * g.U;
+    uint32_t th = (uint32_t) ceil_div(units, kUnitsPerThread);
+    t.threads = (uint32_t) align_up(th < 64 ? 64 : th, 64);
+    t.tilesT = ceil_div(g.gT, tg);
+    t.tilesR = g.gR ? ceil_div(g.gR, tg) : 0;
+    t.ntiles = (uint64_t) g.T * t.tilesT + t.tilesR;
+    t.tile_bytes_max = t.G * block_bytes(32, g.L);
+    return t;
+}
+
 struct Tile {
     uint32_t s;              // segment
     uint32_t nblk;           // blocks in this tile
+    uint32_t ngrp;           // index groups in this tile
     uint64_t lt;             // tile index inside the segment
     uint64_t g0;             // global index of the tile's first block
+    uint64_t grp0;           // global index of the tile's first group
     uint64_t seg_a, seg_end; // segment element range [seg_a, seg_end)
     uint64_t e0;             // first element of the tile's first block
     uint64_t seg_tile0;      // index of the segment's first tile
+    uint64_t seg_tiles;      // tiles in the segment
 };
 
-ZK_HD Tile tile_of(const Geom& g, uint64_t t)
+ZK_HD Tile tile_of(const Geom& g, const Tiling& tl, uint64_t t)
 {
     Tile x;
-    uint64_t nb_s;
-    uint64_t regular = (uint64_t) g.T * g.tilesT;
+    uint64_t nb_s, ng_s;
+    uint64_t regular = (uint64_t) g.T * tl.tilesT;
     if (t < regular) {
-        x.s = (uint32_t) (t / g.tilesT);
-        x.lt = t % g.tilesT;
+        x.s = (uint32_t) (t / tl.tilesT);
+        x.lt = t % tl.tilesT;
         x.seg_a = (uint64_t) x.s * g.tb;
         x.seg_end = x.seg_a + g.tb;
         nb_s = g.nbT;
-        x.seg_tile0 = (uint64_t) x.s * g.tilesT;
-        x.g0 = (uint64_t) x.s * g.nbT + x.lt * g.G;
+        ng_s = g.gT;
+        x.seg_tile0 = (uint64_t) x.s * tl.tilesT;
+        x.seg_tiles = tl.tilesT;
+        x.g0 = (uint64_t) x.s * g.nbT;
+        x.grp0 = (uint64_t) x.s * g.gT;
     } else {
         x.s = g.T;
         x.lt = t - regular;
         x.seg_a = (uint64_t) g.T * g.tb;
         x.seg_end = g.n;
         nb_s = g.nbR;
+        ng_s = g.gR;
         x.seg_tile0 = regular;
-        x.g0 = (uint64_t) g.T * g.nbT + x.lt * g.G;
+        x.seg_tiles = tl.tilesR;
+        x.g0 = (uint64_t) g.T * g.nbT;
+        x.grp0 = (uint64_t) g.T * g.gT;
     }
-    uint64_t first = x.lt * g.G;
+    uint64_t first = x.lt * tl.G;
     uint64_t left = nb_s > first ? nb_s - first : 0;
-    x.nblk = (uint32_t) (left < g.G ? left : g.G);
+    x.nblk = (uint32_t) (left < tl.G ? left : tl.G);
+    uint64_t gfirst = x.lt * tl.TG;
+    uint64_t gleft = ng_s > gfirst ? ng_s - gfirst : 0;
+    x.ngrp = (uint32_t) (gleft < tl.TG ? gleft : tl.TG);
+    x.g0 += first;
+    x.grp0 += gfirst;
     x.e0 = x.seg_a + 1 + first * g.L;
     return x;
 }
@@ -169,17 +209,29 @@ ZK_HD uint32_t tile_block_len(const Geom& g, const Tile& x, uint32_t j)
 }
 
 // Workspace: decoupled-lookback state.  Two status arrays alternate between
-// consecutive launches (parity kept in the ticket word); every launch clears
-// the array the next launch will use, so nothing is reset from the host.
+// consecutive launches; the parity flips when the last lookback chain of a
+// launch completes, and every launch clears the array the next one uses, so
+// nothing is reset from the host between launches (graph replays included).
 struct WsHeader {
-    unsigned long long ticket;   // [63:32] parity, [31:0] next tile ticket
+    unsigned int parity;         // status array in use
+    unsigned int chains_done;    // chains of the current launch that completed
     unsigned long long capacity; // status entries per array
     unsigned long long errors;   // sticky error bits (ZK_ERR_*)
     unsigned long long pad[29];
 };
 static_assert(sizeof(WsHeader) == 256, "workspace header is 256 bytes");
 
-ZK_HD uint64_t workspace_bytes(uint64_t max_tiles) { return sizeof(WsHeader) + 16 * max_tiles; }
+// Bytes of lookback state per tile: two parity arrays of one kStatusStride
+// slot each.
+constexpr uint64_t kWsBytesPerTile = 2ull * 8 * kStatusStride;
+
+ZK_HD uint64_t workspace_bytes(uint64_t max_tiles) { return sizeof(WsHeader) + kWsBytesPerTile * max_tiles; }
+
+// Tiles a workspace of `bytes` bytes can serve (inverse of workspace_bytes).
+ZK_HD uint64_t workspace_capacity(uint64_t bytes)
+{
+    return bytes < sizeof(WsHeader) ? 0 : (bytes - sizeof(WsHeader)) / kWsBytesPerTile;
+}
 
 enum : unsigned long long {
     ZK_ERR_SPIN = 1ull << 0,      // lookback wait exceeded its bound
@@ -189,8 +241,5 @@ enum : unsigned long long {
     ZK_ERR_OVERRUN = 1ull << 4,   // stream walk left its stated size
     ZK_ERR_CAPACITY = 1ull << 5,  // workspace too small for the launch
 };
-
-// Index meta words (u64 at ix_meta): [0] stream size, [1] block-case counters
-// are not kept; [1..3] reserved.
 
 }  // namespace zk

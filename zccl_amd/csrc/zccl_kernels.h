@@ -7,39 +7,112 @@
 
 namespace zk {
 
-struct CompressArgs {
-    Geom g;
-    const float* x;
-    double inv;                    // (double)(1.0f / eb), FP32 division done on the host
-    uint8_t* out;
-    uint64_t out_cap;
-    uint8_t* index;                // output index (Geom::ix_bytes)
-    WsHeader* ws;
-    unsigned long long* size_out;  // optional device copy of the stream size
-};
+// Workgroup size bound of every tile kernel: a tile holds at most kTileUnits
+// 4-value units and a thread owns four of them.
+constexpr uint32_t kMaxTileThreads = kTileUnits / kUnitsPerThread;
+
+// Index groups per tile: compress and decompress use 4 (256 blocks at
+// L = 36); the homomorphic sum keeps more state per thread and uses 2 so
+// three of its workgroups fit one CU.
+constexpr uint32_t kCompressGroups = 4;
+constexpr uint32_t kDecompressGroups = 4;
+constexpr uint32_t kHomsumGroups = 2;
+
+// LDS stage area of the operand-reading kernels (bytes).  Sized so three
+// 576-thread workgroups fit one CU's 160 KB of LDS with 8 operands' tables.
+constexpr uint32_t kStageMin = 44 * 1024;
 
 struct Operand {
     const uint8_t* bytes;
     const uint8_t* index;
 };
 
-struct HomsumArgs {
-    Geom g;
-    int k;
+// Pointers of one operation; geometry travels separately so a launch whose
+// jobs share one geometry keeps it in kernel arguments.
+struct CompressIO {
+    const float* x;
+    uint8_t* out;
+    uint64_t out_cap;
+    uint8_t* index;                // output index (Geom::ix_bytes)
+    unsigned long long* size_out;  // optional device copy of the stream size
+};
+
+struct HomsumIO {
     Operand in[kMaxOperands];
     uint8_t* out;
     uint64_t out_cap;
     uint8_t* index;
-    WsHeader* ws;
     unsigned long long* size_out;
 };
 
-struct DecompressArgs {
-    Geom g;
-    const uint8_t* bytes;
-    const uint8_t* index;
-    float eb;
+struct DecompressIO {
+    Operand in[1];
     float* out;
+};
+
+// Scalar parameters of an operation.
+struct OpParams {
+    double inv;           // compress: (double)(1.0f / eb), FP32 division done on the host
+    float eb;             // decompress: error bound
+    int k;                // homomorphic sum: operands
+    uint32_t stage_bytes; // sum / decompress: LDS stage area
+};
+
+struct CompressJob {
+    Geom g;
+    Tiling tl;
+    OpParams p;
+    CompressIO io;
+};
+
+struct HomsumJob {
+    Geom g;
+    Tiling tl;
+    OpParams p;
+    HomsumIO io;
+};
+
+struct DecompressJob {
+    Geom g;
+    Tiling tl;
+    OpParams p;
+    DecompressIO io;
+};
+
+struct CompressArgs {
+    CompressJob job;
+    WsHeader* ws;
+};
+
+struct HomsumArgs {
+    HomsumJob job;
+    WsHeader* ws;
+};
+
+struct DecompressArgs {
+    DecompressJob job;
+    WsHeader* ws;
+};
+
+// Many jobs of one kind in one launch.  Job j owns tiles [job_tile0[j],
+// job_tile0[j+1]) of the launch.  With `uniform` every job shares g / tl / p
+// (kernel arguments) and `jobs` holds only their IO blocks; otherwise `jobs`
+// holds full CompressJob / HomsumJob / DecompressJob descriptors.  With
+// `interleave` (uniform jobs) workgroup b serves tile b / njobs of job
+// b % njobs, running the jobs' lookback chains side by side.
+enum class BatchKind { Compress, Homsum, Decompress };
+
+struct BatchArgs {
+    const void* jobs;
+    const unsigned long long* job_tile0;  // njobs + 1 entries
+    uint32_t njobs;
+    uint32_t uniform;
+    uint32_t interleave;
+    uint32_t total_chains;
+    uint64_t total_tiles;
+    Geom g;
+    Tiling tl;
+    OpParams p;
     WsHeader* ws;
 };
 
@@ -52,9 +125,14 @@ struct ParseArgs {
     unsigned long long* size_out;
 };
 
+size_t compress_lds_bytes(const Tiling& tl);
+uint32_t stage_bytes_for(const Tiling& tl);
+size_t operand_lds_bytes(const Tiling& tl, int k, uint32_t stage);
+
 hipError_t launch_compress(const CompressArgs& a, hipStream_t s);
 hipError_t launch_homsum(const HomsumArgs& a, hipStream_t s);
 hipError_t launch_decompress(const DecompressArgs& a, hipStream_t s);
+hipError_t launch_batch(BatchKind kind, const BatchArgs& b, uint32_t threads, size_t lds, hipStream_t s);
 hipError_t launch_parse(const ParseArgs& a, hipStream_t s);
 hipError_t launch_case_count(const uint8_t* w1, const uint8_t* w2, uint64_t nblocks,
     unsigned long long* counts, hipStream_t s);

@@ -1,16 +1,18 @@
 // zccl_kernels.hip -- CDNA4 (gfx950) kernels for the SZp fastmode1 codec and
 // the hZCCL homomorphic sum.
 //
-// One workgroup encodes or decodes one tile of G blocks (G = 64 for L <= 64).
-// Thread (j, u) owns the 4-element unit u of block j, so a wave64 covers 256
-// consecutive elements and the tile's elements are contiguous in thread order.
-// Stream offsets come from a single-pass decoupled-lookback scan over tiles
-// (tickets drawn in launch order), so the input is read once and the output
-// written once.  Bytes are staged in LDS: encoders OR each unit's sign nibble,
-// high bytes and MSB-first low-bit field into a zeroed tile buffer, then the
-// tile leaves with aligned 16-byte stores (byte stores only at its two ends);
-// decoders pull the tile's bytes in with aligned 16-byte loads and extract
-// fields with alignbyte / byte-swap / shifts.
+// One workgroup encodes or decodes one tile of up to 256 blocks (9216 values
+// at L = 36).  Thread tid owns the 4-element units v = i*threads + tid
+// (i < 4) of the tile, so each wave instruction touches 256 consecutive
+// values.  Stream offsets (compress, sum) and running integer values
+// (decompress) come from a single-pass decoupled-lookback scan over tiles in
+// workgroup-id order (every XCD dispatches its workgroups in order, so the
+// lowest unfinished tile is always resident and every wait ends); the input is
+// read once and the output written once.  Bytes are staged in LDS: encoders
+// OR each unit's sign nibble, high bytes and MSB-first low-bit field into a
+// zeroed tile buffer and write the tile out with 16-byte stores; decoders
+// pull the tile's bytes in with 16-byte loads and extract fields with
+// alignbyte / byte swap / shifts.
 //
 // Arithmetic follows the reference bit for bit:
 //   quantize   q = (int)((double)x * (double)(1.0f/eb))      ZCCL/src/ZCCL_float.c:540,563,580
@@ -58,6 +60,21 @@ __device__ __forceinline__ float dequant(uint32_t q, float eb)
 }
 
 __device__ __forceinline__ uint32_t umag(int32_t d) { return d < 0 ? 0u - (uint32_t) d : (uint32_t) d; }
+
+// 16 bytes from any byte address (gfx950 global loads need no alignment).
+__device__ __forceinline__ uint4 ld16(const void* p)
+{
+    uint4 v;
+    __builtin_memcpy(&v, p, 16);
+    return v;
+}
+
+__device__ __forceinline__ float4 ldf4(const float* p)
+{
+    float4 v;
+    __builtin_memcpy(&v, p, 16);
+    return v;
+}
 
 // ------------------------------------------------------------ LDS helpers
 
@@ -184,33 +201,37 @@ __device__ __forceinline__ void raise_error(WsHeader* ws, unsigned long long bit
 }
 
 // Decoupled lookback, executed by wave 0 of the workgroup.  Tiles t0..t form
-// one scan; tile t0 starts from `init`.  Returns init + sum of the aggregates
-// of tiles t0..t-1 (modulo 2^62; callers mask further) and publishes tile t's
-// inclusive value.  Every wait is bounded: past kSpinLimit polls the tile
-// records ZK_ERR_SPIN and proceeds, so the grid always drains.
+// one scan (status entry of tile i at status[i * kStatusStride]); tile t0
+// starts from `init`.  Returns init + the aggregates of tiles t0..t-1 (mod
+// 2^62; callers mask further) and publishes tile t's inclusive value.  The
+// nearest predecessor is polled alone first; a 64-wide window is read only
+// when it has published just its aggregate.  Every wait is bounded: past
+// kSpinLimit polls the tile records ZK_ERR_SPIN and proceeds, so the grid
+// always drains.
 __device__ unsigned long long lookback(unsigned long long* status, uint64_t t, uint64_t t0,
     unsigned long long agg, unsigned long long init, WsHeader* ws)
 {
     const int lane = threadIdx.x & 63;
     if (t == t0) {
         if (lane == 0)
-            st_publish(&status[t], kFlagInc | ((init + agg) & kValMask));
+            st_publish(&status[t * kStatusStride], kFlagInc | ((init + agg) & kValMask));
         return init;
     }
     if (lane == 0)
-        st_publish(&status[t], kFlagAgg | (agg & kValMask));
+        st_publish(&status[t * kStatusStride], kFlagAgg | (agg & kValMask));
     unsigned long long excl = 0;
     int64_t end = (int64_t) t;
     uint32_t spins = 0;
     bool timed_out = false;
     while (true) {
+        const int width = end == (int64_t) t ? 1 : 64;
         int64_t idx = end - 1 - lane;
-        bool valid = idx >= (int64_t) t0;
+        bool act = lane < width && idx >= (int64_t) t0;
         unsigned long long st = 0;
         while (true) {
-            if (valid)
-                st = st_poll(&status[idx]);
-            bool waiting = valid && (st >> 62) == 0;
+            if (act)
+                st = st_poll(&status[(uint64_t) idx * kStatusStride]);
+            bool waiting = act && (st >> 62) == 0;
             if (!__any(waiting))
                 break;
             if (++spins > kSpinLimit) {
@@ -221,8 +242,8 @@ __device__ unsigned long long lookback(unsigned long long* status, uint64_t t, u
             }
             __builtin_amdgcn_s_sleep(1);
         }
-        unsigned long long pm = __ballot(valid && (st >> 62) == 2);
-        unsigned long long val = valid ? (st & kValMask) : 0ull;
+        unsigned long long pm = __ballot(act && (st >> 62) == 2);
+        unsigned long long val = act ? (st & kValMask) : 0ull;
         if (pm) {
             int first = __ffsll((long long) pm) - 1;
             if (lane > first)
@@ -231,117 +252,49 @@ __device__ unsigned long long lookback(unsigned long long* status, uint64_t t, u
             break;
         }
         excl += wave_sum64(val);
-        end -= 64;
+        end -= width;
     }
     if (timed_out && lane == 0)
         raise_error(ws, ZK_ERR_SPIN);
     excl += init;
     if (lane == 0)
-        st_publish(&status[t], kFlagInc | ((excl + agg) & kValMask));
+        st_publish(&status[t * kStatusStride], kFlagInc | ((excl + agg) & kValMask));
     return excl;
 }
 
-// Ticket in launch order plus the duty of clearing the status array the next
-// launch will use.  Returns false when this workgroup has nothing to do.
-__device__ bool take_ticket(WsHeader* ws, uint64_t ntiles, uint64_t* s_bc, uint64_t& t,
-    unsigned long long*& status)
+// Status arrays of the current launch.  The workgroup also clears its share
+// of the array the next launch will use.  Returns false when the workspace is
+// too small (then no workgroup of the launch does anything).
+__device__ bool launch_status(WsHeader* ws, uint64_t ntiles_total, uint64_t t, unsigned long long*& status)
 {
-    if (threadIdx.x == 0) {
-        unsigned long long old = atomicAdd(&ws->ticket, 1ull);
-        uint32_t tk = (uint32_t) old;
-        uint32_t par = (uint32_t) (old >> 32) & 1u;
-        if ((uint64_t) tk == ntiles - 1)
-            atomicExch(&ws->ticket, (unsigned long long) (par ^ 1u) << 32);
-        s_bc[0] = old;
-        s_bc[1] = ws->capacity;
-    }
-    __syncthreads();
-    unsigned long long old = s_bc[0];
-    uint64_t cap = s_bc[1];
-    t = (uint32_t) old;
-    uint32_t par = (uint32_t) (old >> 32) & 1u;
-    unsigned long long* base = (unsigned long long*) ((char*) ws + sizeof(WsHeader));
-    if (cap < ntiles || t >= ntiles) {
-        if (threadIdx.x == 0)
+    const uint64_t cap = ws->capacity;
+    if (cap < ntiles_total) {
+        if (threadIdx.x == 0 && t == 0)
             raise_error(ws, ZK_ERR_CAPACITY);
         return false;
     }
-    status = base + (uint64_t) par * cap;
-    unsigned long long* other = base + (uint64_t) (par ^ 1u) * cap;
-    uint64_t per = ceil_div(cap, ntiles);
+    const uint32_t par = ws->parity & 1u;
+    unsigned long long* base = (unsigned long long*) ((char*) ws + sizeof(WsHeader));
+    status = base + (uint64_t) par * cap * kStatusStride;
+    unsigned long long* other = base + (uint64_t) (par ^ 1u) * cap * kStatusStride;
+    uint64_t per = ceil_div(cap, ntiles_total);
     uint64_t lo = t * per, hi = lo + per < cap ? lo + per : cap;
     for (uint64_t i = lo + threadIdx.x; i < hi; i += blockDim.x)
-        other[i] = 0;
+        other[i * kStatusStride] = 0;
     return true;
 }
 
-// ------------------------------------------------------------ global <-> LDS
-
-// Copy src[lo, hi) into LDS so that LDS byte i holds byte (a0 + i) where
-// a0 = (src + lo) rounded down to 16 bytes; returns (src + lo) - a0.  Whole
-// 16-byte loads are used only when they lie inside [src, src + size); the
-// rest is read byte by byte, so nothing outside the stream is touched.
-__device__ uint32_t load_tile(uint32_t* buf, const uint8_t* src, uint64_t lo, uint64_t hi,
-    uint64_t size)
+// Called by one thread of the last tile of a lookback chain once that tile
+// has published.  A finished chain implies every tile of the chain published
+// (and so read the parity); when the last chain of the launch finishes the
+// parity flips for the next launch.
+__device__ void chain_finished(WsHeader* ws, uint32_t nchains)
 {
-    uintptr_t b0 = (uintptr_t) (src + lo), b1 = (uintptr_t) (src + hi);
-    uintptr_t a0 = b0 & ~(uintptr_t) 15;
-    uintptr_t v0 = (uintptr_t) src, v1 = v0 + size;
-    uint32_t nch = (uint32_t) ((b1 - a0 + 15) >> 4);
-    for (uint32_t c = threadIdx.x; c < nch; c += blockDim.x) {
-        uintptr_t ca = a0 + 16ull * c;
-        uint4 v;
-        if (ca >= v0 && ca + 16 <= v1) {
-            v = *(const uint4*) ca;
-        } else {
-            uint32_t w[4] = {0, 0, 0, 0};
-            uintptr_t s = ca > b0 ? ca : b0, e = ca + 16 < b1 ? ca + 16 : b1;
-            for (uintptr_t p = s; p < e; p++)
-                w[(p - ca) >> 2] |= (uint32_t) (*(const uint8_t*) p) << (8 * ((p - ca) & 3));
-            v = make_uint4(w[0], w[1], w[2], w[3]);
-        }
-        *(uint4*) &buf[4 * c] = v;
+    unsigned int old = atomicAdd(&ws->chains_done, 1u);
+    if (old == nchains - 1) {
+        ws->chains_done = 0;
+        __hip_atomic_store(&ws->parity, ws->parity ^ 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    return (uint32_t) (b0 - a0);
-}
-
-// Write nbytes of LDS (starting at LDS byte 0) to dst.  Interior 16-byte
-// aligned chunks go out as whole stores; the partial first and last chunk are
-// written byte by byte so neighbouring tiles' bytes are never touched.
-__device__ void store_tile(const uint32_t* buf, uint8_t* dst, uint32_t nbytes)
-{
-    uintptr_t d0 = (uintptr_t) dst, d1 = d0 + nbytes;
-    uintptr_t c0 = d0 & ~(uintptr_t) 15;
-    uint32_t nch = nbytes ? (uint32_t) ((d1 - c0 + 15) >> 4) : 0u;
-    for (uint32_t c = threadIdx.x; c < nch; c += blockDim.x) {
-        uintptr_t ca = c0 + 16ull * c;
-        if (ca >= d0 && ca + 16 <= d1) {
-            uint32_t o[4];
-            lds_rd16(buf, (uint32_t) (ca - d0), o);
-            *(uint4*) ca = make_uint4(o[0], o[1], o[2], o[3]);
-        } else {
-            uintptr_t s = ca > d0 ? ca : d0, e = ca + 16 < d1 ? ca + 16 : d1;
-            for (uintptr_t p = s; p < e; p++)
-                *(uint8_t*) p = (uint8_t) lds_u8(buf, (uint32_t) (p - d0));
-        }
-    }
-}
-
-__device__ __forceinline__ void store_bytes(uint8_t* dst, const void* src, int n)
-{
-    const uint8_t* s = (const uint8_t*) src;
-    for (int i = 0; i < n; i++)
-        dst[i] = s[i];
-}
-
-__device__ __forceinline__ uint32_t load_u32_bytes(const uint8_t* p)
-{
-    return (uint32_t) p[0] | ((uint32_t) p[1] << 8) | ((uint32_t) p[2] << 16) | ((uint32_t) p[3] << 24);
-}
-
-__device__ __forceinline__ uint64_t load_u64_bytes(const uint8_t* p)
-{
-    return (uint64_t) load_u32_bytes(p) | ((uint64_t) load_u32_bytes(p + 4) << 32);
 }
 
 // ------------------------------------------------------------ unit codec
@@ -394,19 +347,26 @@ __device__ __forceinline__ void decode_unit(const uint32_t* buf, uint32_t bo, ui
         d[c] = ((nib >> (3 - c)) & 1u) ? 0u - mag[c] : mag[c];
 }
 
-// OR the encoding of unit u (cnt deltas, magnitudes m[], sign nibble sg with
-// element 0 in bit 3) of a block of nj elements at width w whose width byte
-// sits at LDS byte bo.  ZCCL/src/ZCCL_TypeManager.c:12-91 and :824-848.
+// OR the encoding of unit u (cnt deltas d[], two's complement) of a block of
+// nj elements at width w whose width byte sits at LDS byte bo.
+// ZCCL/src/ZCCL_TypeManager.c:12-91 and :824-848.
 __device__ __forceinline__ void encode_unit(uint32_t* buf, uint32_t bo, uint32_t w, uint32_t nj,
-    uint32_t u, uint32_t cnt, const uint32_t m[4], uint32_t sg)
+    uint32_t u, uint32_t cnt, const uint32_t d[4])
 {
     const uint32_t B = w >> 3, r = w & 7, sb = (nj + 7) >> 3;
+    uint32_t m[4], sg = 0;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        bool live = (uint32_t) c < cnt;
+        m[c] = live ? umag((int32_t) d[c]) : 0u;
+        sg |= (live && (int32_t) d[c] < 0 ? 1u : 0u) << (3 - c);
+    }
     lds_or4(buf, bo + 1 + (u >> 1), (u & 1) ? sg : (sg << 4));
     if (B) {
         uint32_t h[4];
 #pragma unroll
         for (int c = 0; c < 4; c++)
-            h[c] = (uint32_t) c < cnt ? (m[c] >> r) : 0u;
+            h[c] = m[c] >> r;
         uint32_t W[4] = {0, 0, 0, 0};
         switch (B) {
         case 1:
@@ -441,53 +401,148 @@ __device__ __forceinline__ void encode_unit(uint32_t* buf, uint32_t bo, uint32_t
 
 struct IndexView {
     uint8_t* widths;
-    unsigned long long* tile_off;
+    unsigned long long* group_off;
     unsigned long long* seed_off;
     unsigned long long* meta;
 };
 
-__device__ __forceinline__ IndexView index_view(const Geom& g, uint8_t* ix)
+__device__ __forceinline__ IndexView index_view(const Geom& g, const uint8_t* ix)
 {
     IndexView v;
-    v.widths = ix;
-    v.tile_off = (unsigned long long*) (ix + g.ix_tile);
+    v.widths = (uint8_t*) ix;
+    v.group_off = (unsigned long long*) (ix + g.ix_group);
     v.seed_off = (unsigned long long*) (ix + g.ix_seed);
     v.meta = (unsigned long long*) (ix + g.ix_meta);
     return v;
 }
 
-// ------------------------------------------------------------ shared LDS
+__device__ __forceinline__ uint32_t load_u32_bytes(const uint8_t* p)
+{
+    return (uint32_t) p[0] | ((uint32_t) p[1] << 8) | ((uint32_t) p[2] << 16) | ((uint32_t) p[3] << 24);
+}
+
+__device__ __forceinline__ uint64_t load_u64_bytes(const uint8_t* p)
+{
+    return (uint64_t) load_u32_bytes(p) | ((uint64_t) load_u32_bytes(p + 4) << 32);
+}
+
+__device__ __forceinline__ void store_bytes(uint8_t* dst, const void* src, int n)
+{
+    const uint8_t* s = (const uint8_t*) src;
+    for (int i = 0; i < n; i++)
+        dst[i] = s[i];
+}
+
+// Write nbytes of LDS (from LDS byte 0) to dst.  16-byte aligned interior
+// chunks go out as whole stores; the partial first and last chunk are written
+// byte by byte so neighbouring tiles' bytes are never touched.
+__device__ void store_tile(const uint32_t* buf, uint8_t* dst, uint32_t nbytes)
+{
+    uintptr_t d0 = (uintptr_t) dst, d1 = d0 + nbytes;
+    uintptr_t c0 = d0 & ~(uintptr_t) 15;
+    uint32_t nch = nbytes ? (uint32_t) ((d1 - c0 + 15) >> 4) : 0u;
+    for (uint32_t c = threadIdx.x; c < nch; c += blockDim.x) {
+        uintptr_t ca = c0 + 16ull * c;
+        if (ca >= d0 && ca + 16 <= d1) {
+            uint32_t o[4];
+            lds_rd16(buf, (uint32_t) (ca - d0), o);
+            *(uint4*) ca = make_uint4(o[0], o[1], o[2], o[3]);
+        } else {
+            uintptr_t s = ca > d0 ? ca : d0, e = ca + 16 < d1 ? ca + 16 : d1;
+            for (uintptr_t p = s; p < e; p++)
+                *(uint8_t*) p = (uint8_t) lds_u8(buf, (uint32_t) (p - d0));
+        }
+    }
+}
+
+// ------------------------------------------------------------ tile state
 
 extern __shared__ uint32_t zk_lds[];
 
 struct TileShared {
-    uint64_t bc[4];
-    uint32_t wave[16];
-    uint32_t bmax[64];
-    uint32_t boff[64];
-    uint32_t width[64];
+    unsigned long long bc[4];
+    uint32_t wave[64];
+    uint32_t bmax[kMaxTileBlocks];
+    uint32_t boff[kMaxTileBlocks];
+    uint8_t width[kMaxTileBlocks];
 };
 
-// Width, size and tile offsets of the blocks whose magnitudes were folded into
-// sh.bmax; encodes the thread's unit; writes the tile, its seed and header
-// entry, and the output index.  Shared tail of compress and homomorphic sum.
-__device__ void encode_tile(const Geom& g, const Tile& x, uint64_t t, TileShared& sh,
-    unsigned long long* status, WsHeader* ws, uint32_t j, uint32_t u, uint32_t nj, uint32_t cnt,
-    const uint32_t m[4], uint32_t sg, uint32_t seed, uint8_t* out, uint64_t out_cap,
-    uint8_t* index, unsigned long long* size_out)
+// Per-thread unit coordinates: unit v = i*threads + tid of the tile, packed
+// as block j (9 bits) | unit-in-block u (11 bits) | value count (3 bits); a
+// zero count means the thread has no unit i.
+struct Units {
+    uint32_t pk[kUnitsPerThread];
+};
+
+__device__ __forceinline__ uint32_t unit_j(uint32_t p) { return p & 511u; }
+__device__ __forceinline__ uint32_t unit_u(uint32_t p) { return (p >> 9) & 2047u; }
+__device__ __forceinline__ uint32_t unit_cnt(uint32_t p) { return p >> 20; }
+
+__device__ __forceinline__ Units tile_units(const Geom& g, const Tile& x)
+{
+    Units un;
+    const uint32_t nunits = x.nblk * g.U;
+#pragma unroll
+    for (uint32_t i = 0; i < kUnitsPerThread; i++) {
+        uint32_t v = i * blockDim.x + threadIdx.x;
+        un.pk[i] = 0;
+        if (v < nunits) {
+            uint32_t j = v / g.U;
+            uint32_t u = v - j * g.U;
+            uint32_t nj = tile_block_len(g, x, j);
+            uint32_t cnt = 4 * u < nj ? min(4u, nj - 4 * u) : 0u;
+            un.pk[i] = j | (u << 9) | (cnt << 20);
+        }
+    }
+    return un;
+}
+
+// Fold the magnitudes of the thread's units into the per-block maxima.
+__device__ __forceinline__ void fold_block_max(uint32_t* bmax, const Units& un, const uint32_t (&d)[kUnitsPerThread][4])
+{
+#pragma unroll
+    for (uint32_t i = 0; i < kUnitsPerThread; i++) {
+        const uint32_t cnt = unit_cnt(un.pk[i]);
+        uint32_t umax = 0;
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+            if ((uint32_t) c < cnt)
+                umax = max(umax, umag((int32_t) d[i][c]));
+        if (umax)
+            atomicMax(&bmax[unit_j(un.pk[i])], umax);
+    }
+}
+
+// Output side shared by compress and the homomorphic sum: per-block widths
+// from sh.bmax, block offsets, the lookback for the tile's stream offset,
+// encoding into LDS, the tile store, seed / header entry and output index.
+__device__ void encode_tile(const Geom& g, const Tiling& tl, const Tile& x, uint64_t t,
+    TileShared& sh, uint32_t* buf, unsigned long long* status, WsHeader* ws, const Units& un,
+    const uint32_t (&d)[kUnitsPerThread][4], uint32_t seed,
+    uint8_t* out, uint64_t out_cap, uint8_t* index, unsigned long long* size_out, bool last_tile,
+    uint32_t nchains)
 {
     const uint32_t tid = threadIdx.x;
-    uint32_t* buf = zk_lds;
-    uint32_t bsz = 0;
-    if (tid < x.nblk) {
-        uint32_t w = dwidth(sh.bmax[tid]);
-        sh.width[tid] = w;
-        bsz = block_bytes(w, tile_block_len(g, x, tid));
+    // widths and sizes, blocks assigned to threads in contiguous runs
+    const uint32_t cpt = (uint32_t) ceil_div(tl.G, blockDim.x);
+    uint32_t loc[4] = {0, 0, 0, 0};
+    uint32_t run = 0;
+    for (uint32_t c = 0; c < cpt; c++) {
+        uint32_t j = tid * cpt + c;
+        if (j < x.nblk) {
+            uint32_t w = dwidth(sh.bmax[j]);
+            sh.width[j] = (uint8_t) w;
+            loc[c] = run;
+            run += block_bytes(w, tile_block_len(g, x, j));
+        }
     }
     uint32_t total;
-    uint32_t boff = block_excl_scan(bsz, sh.wave, total);
-    if (tid < x.nblk)
-        sh.boff[tid] = boff;
+    uint32_t base = block_excl_scan(run, sh.wave, total);
+    for (uint32_t c = 0; c < cpt; c++) {
+        uint32_t j = tid * cpt + c;
+        if (j < x.nblk)
+            sh.boff[j] = base + loc[c];
+    }
     if (tid < 64) {
         unsigned long long ex = lookback(status, t, 0, total, 0, ws);
         if (tid == 0)
@@ -498,13 +553,18 @@ __device__ void encode_tile(const Geom& g, const Tile& x, uint64_t t, TileShared
         buf[i] = 0;
     __syncthreads();
     const uint64_t P = 8ull * g.T + 4ull * (x.s + 1) + sh.bc[2];
-    if (cnt) {
-        uint32_t w = sh.width[j];
-        if (w)
-            encode_unit(buf, sh.boff[j], w, nj, u, cnt, m, sg);
+#pragma unroll
+    for (uint32_t i = 0; i < kUnitsPerThread; i++) {
+        const uint32_t pk = un.pk[i], cnt = unit_cnt(pk);
+        if (cnt) {
+            const uint32_t j = unit_j(pk);
+            uint32_t w = sh.width[j];
+            if (w)
+                encode_unit(buf, sh.boff[j], w, tile_block_len(g, x, j), unit_u(pk), cnt, d[i]);
+        }
     }
-    if (tid < x.nblk)
-        lds_or4(buf, sh.boff[tid], sh.width[tid]);
+    for (uint32_t j = tid; j < x.nblk; j += blockDim.x)
+        lds_or4(buf, sh.boff[j], sh.width[j]);
     __syncthreads();
     const bool fits = P + total <= out_cap;
     if (fits)
@@ -512,6 +572,10 @@ __device__ void encode_tile(const Geom& g, const Tile& x, uint64_t t, TileShared
     else if (tid == 0)
         raise_error(ws, ZK_ERR_NOSPACE);
     IndexView ix = index_view(g, index);
+    for (uint32_t j = tid; j < x.nblk; j += blockDim.x)
+        ix.widths[x.g0 + j] = sh.width[j];
+    if (tid < x.ngrp)
+        ix.group_off[x.grp0 + tid] = P + (x.nblk ? sh.boff[tid * g.IG] : 0u);
     if (tid == 0) {
         if (x.lt == 0) {
             uint64_t so = P - 4;
@@ -524,250 +588,542 @@ __device__ void encode_tile(const Geom& g, const Tile& x, uint64_t t, TileShared
             }
             ix.seed_off[x.s] = so;
         }
-        ix.tile_off[t] = P;
-        if (t == g.ntiles - 1) {
+        if (last_tile) {
             // the last tile ends the stream: if it fits, every tile fits.  A
             // stream that did not fit records size 0 so no consumer reads it.
             unsigned long long sz = fits ? P + total : 0ull;
             ix.meta[0] = sz;
             if (size_out)
                 *size_out = sz;
+            chain_finished(ws, nchains);
         }
     }
-    if (tid < x.nblk)
-        ix.widths[x.g0 + tid] = (uint8_t) sh.width[tid];
 }
 
 // ------------------------------------------------------------ compress
 
-__global__ void __launch_bounds__(1024) k_compress(CompressArgs a)
+__device__ void compress_tile(const Geom& g, const Tiling& tl, const OpParams& p, const CompressIO& io,
+    uint64_t t, unsigned long long* status, WsHeader* ws, uint32_t nchains)
 {
     __shared__ TileShared sh;
-    const Geom& g = a.g;
-    uint64_t t;
-    unsigned long long* status;
-    if (!take_ticket(a.ws, g.ntiles, sh.bc, t, status))
-        return;
-    const Tile x = tile_of(g, t);
+    uint32_t* buf = zk_lds;
+    const Tile x = tile_of(g, tl, t);
     const uint32_t tid = threadIdx.x;
-    const uint32_t j = tid / g.U, u = tid - j * g.U;
-    if (tid < 64)
-        sh.bmax[tid] = 0;
-    const uint32_t nj = j < x.nblk ? tile_block_len(g, x, j) : 0u;
-    const uint32_t cnt = 4 * u < nj ? min(4u, nj - 4 * u) : 0u;
-    const uint64_t e = x.e0 + (uint64_t) j * g.L + 4 * u;
-    int32_t q[4] = {0, 0, 0, 0};
-    if (cnt == 4 && (((uintptr_t) (a.x + e)) & 15) == 0) {
-        float4 v = *(const float4*) (a.x + e);
-        q[0] = quant(v.x, a.inv); q[1] = quant(v.y, a.inv);
-        q[2] = quant(v.z, a.inv); q[3] = quant(v.w, a.inv);
-    } else {
+    for (uint32_t j = tid; j < tl.G; j += blockDim.x)
+        sh.bmax[j] = 0;
+    const Units un = tile_units(g, x);
+    uint32_t d[kUnitsPerThread][4];
 #pragma unroll
-        for (int c = 0; c < 4; c++)
-            if ((uint32_t) c < cnt)
-                q[c] = quant(a.x[e + c], a.inv);
-    }
-    // previous element: the last element of lane-1's unit (the units of a
-    // tile are contiguous in thread order); lane 0 of each wave reads it
-    int32_t last = cnt ? q[cnt - 1] : 0;
-    int32_t prev = __shfl_up(last, 1, 64);
-    if ((tid & 63) == 0 && cnt)
-        prev = quant(a.x[e - 1], a.inv);
-    uint32_t m[4] = {0, 0, 0, 0}, sg = 0, umax = 0;
-    int32_t p = prev;
+    for (uint32_t i = 0; i < kUnitsPerThread; i++) {
+        const uint32_t pk = un.pk[i], cnt = unit_cnt(pk);
+        const float* xp = io.x + x.e0 + (uint64_t) unit_j(pk) * g.L + 4 * unit_u(pk);
+        if (cnt == 4) {
+            float4 v = ldf4(xp);
+            d[i][0] = quant(v.x, p.inv); d[i][1] = quant(v.y, p.inv);
+            d[i][2] = quant(v.z, p.inv); d[i][3] = quant(v.w, p.inv);
+        } else {
 #pragma unroll
-    for (int c = 0; c < 4; c++) {
-        if ((uint32_t) c < cnt) {
-            int32_t d = (int32_t) ((uint32_t) q[c] - (uint32_t) p);
-            p = q[c];
-            m[c] = umag(d);
-            sg |= (d < 0 ? 1u : 0u) << (3 - c);
-            umax = max(umax, m[c]);
+            for (int c = 0; c < 4; c++)
+                d[i][c] = (uint32_t) c < cnt ? (uint32_t) quant(xp[c], p.inv) : 0u;
         }
+        // previous value: last value of unit v-1, held by lane-1 (units of a
+        // tile are contiguous in v); lane 0 of each wave reads it
+        uint32_t last = cnt == 4 ? d[i][3] : cnt == 3 ? d[i][2] : cnt == 2 ? d[i][1] : d[i][0];
+        uint32_t prev = __shfl_up(last, 1, 64);
+        if ((tid & 63) == 0 && cnt)
+            prev = (uint32_t) quant(xp[-1], p.inv);
+        // deltas in place (int32 wrap)
+#pragma unroll
+        for (int c = 3; c >= 1; c--)
+            d[i][c] -= d[i][c - 1];
+        d[i][0] -= prev;
     }
     __syncthreads();
-    if (umax)
-        atomicMax(&sh.bmax[j], umax);
+    fold_block_max(sh.bmax, un, d);
     __syncthreads();
     uint32_t seed = 0;
     if (x.lt == 0 && tid == 0)
-        seed = (uint32_t) quant(a.x[x.seg_a], a.inv);
-    encode_tile(g, x, t, sh, status, a.ws, j, u, nj, cnt, m, sg, seed, a.out, a.out_cap, a.index,
-        a.size_out);
+        seed = (uint32_t) quant(io.x[x.seg_a], p.inv);
+    encode_tile(g, tl, x, t, sh, buf, status, ws, un, d, seed, io.out, io.out_cap, io.index, io.size_out,
+        t == tl.ntiles - 1, nchains);
 }
 
-// ------------------------------------------------------------ operand decode
+// ------------------------------------------------------------ operand staging
 
-// Stage operand tile bytes in LDS and return each block's LDS offset in
-// sh.boff, width in sh.width.  Widths come from the operand's index; a width
-// byte in the stream that disagrees with the index raises ZK_ERR_MISMATCH.
-__device__ void stage_operand(const Geom& g, const Tile& x, uint64_t t, TileShared& sh,
-    const uint8_t* bytes, const uint8_t* index, WsHeader* ws)
+struct OperandShared {
+    unsigned long long base[kMaxOperands];  // stream offset of the tile's first block
+    unsigned long long size[kMaxOperands];  // stream size
+    unsigned long long seedoff[kMaxOperands];
+    uint32_t tot[kMaxOperands];             // tile bytes
+    uint32_t lds[kMaxOperands];             // LDS position of the operand's bytes
+    uint32_t cstart[kMaxOperands + 1];      // chunk prefix within a load group
+    uint32_t bad[kMaxOperands];
+};
+
+// Read the k operands' widths and offsets for tile x; per-operand scans give
+// each block's byte offset inside its operand's tile bytes.  Tables: widths
+// s_w[k][G] (u8) and offsets s_off[k][G] (u16).
+__device__ void operand_tables(const Geom& g, const Tiling& tl, const Tile& x, int k, const Operand* in,
+    OperandShared& os, uint8_t* s_w, uint16_t* s_off, WsHeader* ws)
 {
-    const uint32_t tid = threadIdx.x;
-    IndexView ix = index_view(g, (uint8_t*) index);
-    uint32_t bsz = 0, w = 0;
-    if (tid < x.nblk) {
-        w = ix.widths[x.g0 + tid];
-        if (w > 32) {
-            raise_error(ws, ZK_ERR_WIDTH);
-            w = 0;
+    const uint32_t tid = threadIdx.x, G = tl.G;
+    if (tid < (uint32_t) k) {
+        IndexView ix = index_view(g, in[tid].index);
+        os.base[tid] = ix.group_off[x.grp0];
+        os.size[tid] = ix.meta[0];
+        os.seedoff[tid] = x.lt == 0 ? ix.seed_off[x.s] : 0ull;
+    }
+    for (uint32_t idx = tid; idx < (uint32_t) k * G; idx += blockDim.x) {
+        uint32_t r = idx / G, j = idx - r * G;
+        uint32_t w = 0;
+        if (j < x.nblk) {
+            w = in[r].index[x.g0 + j];
+            if (w > 32) {
+                raise_error(ws, ZK_ERR_WIDTH);
+                w = 0;
+            }
         }
-        bsz = block_bytes(w, tile_block_len(g, x, tid));
-    }
-    uint32_t total;
-    uint32_t boff = block_excl_scan(bsz, sh.wave, total);
-    const uint64_t P = ix.tile_off[t];
-    const uint64_t size = ix.meta[0];
-    uint32_t shift = 0;
-    if (P + total <= size) {
-        shift = load_tile(zk_lds, bytes, P, P + total, size);
-    } else {
-        if (tid == 0)
-            raise_error(ws, ZK_ERR_OVERRUN);
-        total = 0;
-    }
-    if (tid < x.nblk) {
-        sh.boff[tid] = shift + boff;
-        sh.width[tid] = total ? w : 0u;
+        s_w[idx] = (uint8_t) w;
     }
     __syncthreads();
-    if (tid < x.nblk && total && lds_u8(zk_lds, shift + boff) != w)
-        raise_error(ws, ZK_ERR_MISMATCH);
+    const int lane = tid & 63, wid = tid >> 6, nw = blockDim.x >> 6;
+    for (int r = wid; r < k; r += nw) {
+        uint32_t sz[4], run = 0;
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            uint32_t j = 4 * lane + c;
+            sz[c] = j < x.nblk ? block_bytes(s_w[r * G + j], tile_block_len(g, x, j)) : 0u;
+            run += sz[c];
+        }
+        uint32_t incl = wave_incl_scan(run);
+        uint32_t off = incl - run;
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            uint32_t j = 4 * lane + c;
+            if (j < G)
+                s_off[r * G + j] = (uint16_t) off;
+            off += sz[c];
+        }
+        if (lane == 63) {
+            os.tot[r] = incl;
+            bool bad = os.base[r] + incl > os.size[r];
+            os.bad[r] = bad;
+            if (bad)
+                raise_error(ws, ZK_ERR_OVERRUN);
+        }
+    }
+    __syncthreads();
 }
 
-// Seed of the tile's segment, read only when it lies inside the stream.
-__device__ uint32_t load_seed(const Geom& g, const Tile& x, const uint8_t* bytes, const uint8_t* index,
-    WsHeader* ws)
+// Group operands [r0, r) so their tile bytes fit the stage area; fills
+// os.lds / os.cstart for the group and returns its end.  Executed by one
+// thread; the caller synchronises.
+__device__ int plan_group(OperandShared& os, int r0, int k, uint32_t stage_bytes)
 {
-    IndexView ix = index_view(g, (uint8_t*) index);
-    uint64_t so = ix.seed_off[x.s];
-    if (so + 4 <= ix.meta[0])
-        return load_u32_bytes(bytes + so);
-    raise_error(ws, ZK_ERR_OVERRUN);
-    return 0;
+    uint32_t pos = 0, ch = 0;
+    int r = r0;
+    os.cstart[0] = 0;
+    while (r < k) {
+        uint32_t n = os.bad[r] ? 0u : os.tot[r];
+        uint32_t need = (uint32_t) align_up(n + 16, 16);
+        if (r > r0 && pos + need > stage_bytes)
+            break;
+        os.lds[r] = pos;
+        pos += need;
+        ch += (n + 15) / 16;
+        os.cstart[r + 1 - r0] = ch;
+        r++;
+    }
+    return r;
+}
+
+// Load operands [r0, r1) of a group into LDS: LDS byte os.lds[r] + i holds
+// stream byte os.base[r] + i.  Every thread issues a batch of loads before
+// writing any of them, so a group costs about one memory round trip; only
+// bytes inside each stream are read.
+constexpr int kLoadBatch = 8;
+
+__device__ void load_group(const Operand* in, const OperandShared& os, int r0, int r1, uint32_t* stage)
+{
+    const uint32_t nch = os.cstart[r1 - r0];
+    for (uint32_t c0 = threadIdx.x; c0 < nch; c0 += kLoadBatch * blockDim.x) {
+        uint4 v[kLoadBatch];
+        uint32_t dst[kLoadBatch];
+        int r = r0;
+#pragma unroll
+        for (int b = 0; b < kLoadBatch; b++) {
+            uint32_t c = c0 + b * blockDim.x;
+            dst[b] = ~0u;
+            v[b] = make_uint4(0, 0, 0, 0);
+            if (c < nch) {
+                while (os.cstart[r + 1 - r0] <= c)
+                    r++;
+                uint32_t cl = c - os.cstart[r - r0];
+                uint64_t off = os.base[r] + 16ull * cl;
+                const uint8_t* src = in[r].bytes + off;
+                if (off + 16 <= os.size[r]) {
+                    v[b] = ld16(src);
+                } else {
+                    uint32_t w[4] = {0, 0, 0, 0};
+                    for (uint32_t p = 0; p < 16 && off + p < os.size[r]; p++)
+                        w[p >> 2] |= (uint32_t) src[p] << (8 * (p & 3));
+                    v[b] = make_uint4(w[0], w[1], w[2], w[3]);
+                }
+                dst[b] = (os.lds[r] >> 2) + 4 * cl;
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < kLoadBatch; b++)
+            if (dst[b] != ~0u)
+                *(uint4*) &stage[dst[b]] = v[b];
+    }
 }
 
 // ------------------------------------------------------------ homomorphic sum
 
-__global__ void __launch_bounds__(1024) k_homsum(HomsumArgs a)
+__device__ void homsum_tile(const Geom& g, const Tiling& tl, const OpParams& p, const HomsumIO& io,
+    uint64_t t, unsigned long long* status, WsHeader* ws, uint32_t nchains)
 {
     __shared__ TileShared sh;
-    const Geom& g = a.g;
-    uint64_t t;
-    unsigned long long* status;
-    if (!take_ticket(a.ws, g.ntiles, sh.bc, t, status))
-        return;
-    const Tile x = tile_of(g, t);
+    __shared__ OperandShared os;
+    __shared__ int s_grp_end;
+    const Tile x = tile_of(g, tl, t);
     const uint32_t tid = threadIdx.x;
-    const uint32_t j = tid / g.U, u = tid - j * g.U;
-    const uint32_t nj = j < x.nblk ? tile_block_len(g, x, j) : 0u;
-    const uint32_t cnt = 4 * u < nj ? min(4u, nj - 4 * u) : 0u;
-    uint32_t acc[4] = {0, 0, 0, 0};
-    uint32_t seed = 0;
-    for (int r = 0; r < a.k; r++) {
-        stage_operand(g, x, t, sh, a.in[r].bytes, a.in[r].index, a.ws);
-        if (cnt) {
-            uint32_t w = sh.width[j];
-            if (w) {
-                uint32_t d[4];
-                decode_unit(zk_lds, sh.boff[j], w, nj, u, d);
+    uint32_t* stage = zk_lds;
+    uint8_t* s_w = (uint8_t*) (zk_lds + p.stage_bytes / 4);
+    uint16_t* s_off = (uint16_t*) (s_w + align_up((uint64_t) p.k * tl.G, 16));
+    const Units un = tile_units(g, x);
+    operand_tables(g, tl, x, p.k, io.in, os, s_w, s_off, ws);
+    uint32_t acc[kUnitsPerThread][4];
+#pragma unroll
+    for (uint32_t i = 0; i < kUnitsPerThread; i++)
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+            acc[i][c] = 0;
+    for (int r0 = 0; r0 < p.k;) {
+        if (tid == 0)
+            s_grp_end = plan_group(os, r0, p.k, p.stage_bytes);
+        __syncthreads();
+        const int r1 = s_grp_end;
+        load_group(io.in, os, r0, r1, stage);
+        __syncthreads();
+        for (int r = r0; r < r1; r++) {
+            if (os.bad[r])
+                continue;
+            const uint8_t* wr = s_w + r * tl.G;
+            const uint16_t* orr = s_off + r * tl.G;
+            const uint32_t lbase = os.lds[r];
+#pragma unroll
+            for (uint32_t i = 0; i < kUnitsPerThread; i++) {
+                const uint32_t pk = un.pk[i];
+                if (!unit_cnt(pk))
+                    continue;
+                const uint32_t j = unit_j(pk), u = unit_u(pk);
+                uint32_t w = wr[j];
+                if (!w)
+                    continue;
+                uint32_t dd[4];
+                uint32_t bo = lbase + orr[j];
+                decode_unit(stage, bo, w, tile_block_len(g, x, j), u, dd);
 #pragma unroll
                 for (int c = 0; c < 4; c++)
-                    acc[c] += d[c];
+                    acc[i][c] += dd[c];
+                if (u == 0 && lds_u8(stage, bo) != w)
+                    raise_error(ws, ZK_ERR_MISMATCH);
             }
         }
-        if (x.lt == 0 && tid == 0)
-            seed += load_seed(g, x, a.in[r].bytes, a.in[r].index, a.ws);
         __syncthreads();
+        r0 = r1;
     }
-    if (tid < 64)
-        sh.bmax[tid] = 0;
-    uint32_t m[4] = {0, 0, 0, 0}, sg = 0, umax = 0;
-#pragma unroll
-    for (int c = 0; c < 4; c++) {
-        if ((uint32_t) c < cnt) {
-            int32_t d = (int32_t) acc[c];
-            m[c] = umag(d);
-            sg |= (d < 0 ? 1u : 0u) << (3 - c);
-            umax = max(umax, m[c]);
+    uint32_t seed = 0;
+    if (x.lt == 0 && tid == 0) {
+        for (int r = 0; r < p.k; r++) {
+            uint64_t so = os.seedoff[r];
+            if (so + 4 <= os.size[r])
+                seed += load_u32_bytes(io.in[r].bytes + so);
+            else
+                raise_error(ws, ZK_ERR_OVERRUN);
         }
     }
+    for (uint32_t j = tid; j < tl.G; j += blockDim.x)
+        sh.bmax[j] = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < kUnitsPerThread; i++)
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+            if ((uint32_t) c >= unit_cnt(un.pk[i]))
+                acc[i][c] = 0;
     __syncthreads();
-    if (umax)
-        atomicMax(&sh.bmax[j], umax);
+    fold_block_max(sh.bmax, un, acc);
     __syncthreads();
-    encode_tile(g, x, t, sh, status, a.ws, j, u, nj, cnt, m, sg, seed, a.out, a.out_cap, a.index,
-        a.size_out);
+    encode_tile(g, tl, x, t, sh, stage, status, ws, un, acc, seed, io.out, io.out_cap, io.index, io.size_out,
+        t == tl.ntiles - 1, nchains);
 }
 
 // ------------------------------------------------------------ decompress
 
-__global__ void __launch_bounds__(1024) k_decompress(DecompressArgs a)
+__device__ void decompress_tile(const Geom& g, const Tiling& tl, const OpParams& p, const DecompressIO& io,
+    uint64_t t, unsigned long long* status, WsHeader* ws, uint32_t nchains)
 {
     __shared__ TileShared sh;
-    const Geom& g = a.g;
-    uint64_t t;
-    unsigned long long* status;
-    if (!take_ticket(a.ws, g.ntiles, sh.bc, t, status))
-        return;
-    const Tile x = tile_of(g, t);
+    __shared__ OperandShared os;
+    const Tile x = tile_of(g, tl, t);
     const uint32_t tid = threadIdx.x;
-    const uint32_t j = tid / g.U, u = tid - j * g.U;
-    const uint32_t nj = j < x.nblk ? tile_block_len(g, x, j) : 0u;
-    const uint32_t cnt = 4 * u < nj ? min(4u, nj - 4 * u) : 0u;
-    stage_operand(g, x, t, sh, a.bytes, a.index, a.ws);
-    uint32_t d[4] = {0, 0, 0, 0};
-    if (cnt) {
-        uint32_t w = sh.width[j];
-        if (w)
-            decode_unit(zk_lds, sh.boff[j], w, nj, u, d);
-    }
-    // running sum inside the unit, then across the tile in element order
-    uint32_t pre[4];
-    uint32_t run = 0;
-#pragma unroll
-    for (int c = 0; c < 4; c++) {
-        run += (uint32_t) c < cnt ? d[c] : 0u;
-        pre[c] = run;
-    }
-    uint32_t tile_sum;
-    uint32_t texcl = block_excl_scan(run, sh.wave, tile_sum);
-    if (x.lt == 0 && tid == 0)
-        sh.bc[3] = load_seed(g, x, a.bytes, a.index, a.ws);
+    uint32_t* stage = zk_lds;
+    uint8_t* s_w = (uint8_t*) (zk_lds + p.stage_bytes / 4);
+    uint16_t* s_off = (uint16_t*) (s_w + align_up(tl.G, 16));
+    const Units un = tile_units(g, x);
+    operand_tables(g, tl, x, 1, io.in, os, s_w, s_off, ws);
+    if (tid == 0)
+        plan_group(os, 0, 1, p.stage_bytes);
     __syncthreads();
+    load_group(io.in, os, 0, 1, stage);
+    if (tid == 0) {
+        uint32_t seed = 0;
+        if (x.lt == 0) {
+            uint64_t so = os.seedoff[0];
+            if (so + 4 <= os.size[0])
+                seed = load_u32_bytes(io.in[0].bytes + so);
+            else
+                raise_error(ws, ZK_ERR_OVERRUN);
+        }
+        sh.bc[3] = seed;
+    }
+    __syncthreads();
+    uint32_t pre[kUnitsPerThread][4], us[kUnitsPerThread];
+#pragma unroll
+    for (uint32_t i = 0; i < kUnitsPerThread; i++) {
+        uint32_t d[4] = {0, 0, 0, 0};
+        const uint32_t pk = un.pk[i], cnt = unit_cnt(pk);
+        if (cnt && !os.bad[0]) {
+            const uint32_t j = unit_j(pk), u = unit_u(pk);
+            uint32_t w = s_w[j];
+            if (w) {
+                uint32_t bo = os.lds[0] + s_off[j];
+                decode_unit(stage, bo, w, tile_block_len(g, x, j), u, d);
+                if (u == 0 && lds_u8(stage, bo) != w)
+                    raise_error(ws, ZK_ERR_MISMATCH);
+            }
+        }
+        uint32_t run = 0;
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            run += (uint32_t) c < cnt ? d[c] : 0u;
+            pre[i][c] = run;
+        }
+        us[i] = run;
+    }
+    // scan of unit sums in unit order v = i*threads + tid
+    const int lane = tid & 63, wid = tid >> 6, nw = blockDim.x >> 6;
+    uint32_t winc[kUnitsPerThread];
+#pragma unroll
+    for (uint32_t i = 0; i < kUnitsPerThread; i++) {
+        winc[i] = wave_incl_scan(us[i]);
+        if (lane == 63)
+            sh.wave[i * nw + wid] = winc[i];
+    }
+    __syncthreads();
+    if (wid == 0) {
+        const int nv = kUnitsPerThread * nw;
+        uint32_t v = lane < nv ? sh.wave[lane] : 0u;
+        uint32_t inc = wave_incl_scan(v);
+        if (lane < nv)
+            sh.wave[lane] = inc - v;
+        if (lane == nv - 1)
+            sh.bc[1] = inc;
+    }
+    __syncthreads();
+    const uint32_t tile_sum = (uint32_t) sh.bc[1];
     const uint32_t seed = (uint32_t) sh.bc[3];
     if (tid < 64) {
-        unsigned long long ex = lookback(status, t, x.seg_tile0, tile_sum, x.lt == 0 ? seed : 0u, a.ws);
+        unsigned long long ex = lookback(status, t, x.seg_tile0, tile_sum, x.lt == 0 ? seed : 0u, ws);
         if (tid == 0)
             sh.bc[2] = ex;
     }
     __syncthreads();
-    const uint32_t carry = (uint32_t) sh.bc[2] + texcl;
-    const uint64_t e = x.e0 + (uint64_t) j * g.L + 4 * u;
-    if (cnt == 4 && (((uintptr_t) (a.out + e)) & 15) == 0) {
-        float4 v;
-        v.x = dequant(carry + pre[0], a.eb);
-        v.y = dequant(carry + pre[1], a.eb);
-        v.z = dequant(carry + pre[2], a.eb);
-        v.w = dequant(carry + pre[3], a.eb);
-        *(float4*) (a.out + e) = v;
-    } else {
+    const uint32_t carry = (uint32_t) sh.bc[2];
 #pragma unroll
-        for (int c = 0; c < 4; c++)
-            if ((uint32_t) c < cnt)
-                a.out[e + c] = dequant(carry + pre[c], a.eb);
+    for (uint32_t i = 0; i < kUnitsPerThread; i++) {
+        const uint32_t pk = un.pk[i], cnt = unit_cnt(pk);
+        if (!cnt)
+            continue;
+        const uint32_t c0 = carry + sh.wave[i * nw + wid] + winc[i] - us[i];
+        float* op = io.out + x.e0 + (uint64_t) unit_j(pk) * g.L + 4 * unit_u(pk);
+        if (cnt == 4) {
+            float4 v;
+            v.x = dequant(c0 + pre[i][0], p.eb);
+            v.y = dequant(c0 + pre[i][1], p.eb);
+            v.z = dequant(c0 + pre[i][2], p.eb);
+            v.w = dequant(c0 + pre[i][3], p.eb);
+            __builtin_memcpy(op, &v, 16);
+        } else {
+#pragma unroll
+            for (uint32_t c = 0; c < 3; c++)
+                if (c < cnt)
+                    op[c] = dequant(c0 + pre[i][c], p.eb);
+        }
     }
-    if (x.lt == 0 && tid == 0)
-        a.out[x.seg_a] = dequant(seed, a.eb);
+    if (tid == 0) {
+        if (x.lt == 0)
+            io.out[x.seg_a] = dequant(seed, p.eb);
+        if (x.lt == x.seg_tiles - 1)
+            chain_finished(ws, nchains);
+    }
+}
+
+// ------------------------------------------------------------ kernels
+
+__global__ void __launch_bounds__(kMaxTileThreads, 6) k_compress(CompressArgs a)
+{
+    const uint64_t t = blockIdx.x;
+    unsigned long long* status;
+    if (!launch_status(a.ws, a.job.tl.ntiles, t, status))
+        return;
+    compress_tile(a.job.g, a.job.tl, a.job.p, a.job.io, t, status, a.ws, 1);
+}
+
+__global__ void __launch_bounds__(kMaxTileThreads, 4) k_homsum(HomsumArgs a)
+{
+    const uint64_t t = blockIdx.x;
+    unsigned long long* status;
+    if (!launch_status(a.ws, a.job.tl.ntiles, t, status))
+        return;
+    homsum_tile(a.job.g, a.job.tl, a.job.p, a.job.io, t, status, a.ws, 1);
+}
+
+__global__ void __launch_bounds__(kMaxTileThreads, 7) k_decompress(DecompressArgs a)
+{
+    const uint64_t t = blockIdx.x;
+    unsigned long long* status;
+    if (!launch_status(a.ws, a.job.tl.ntiles, t, status))
+        return;
+    decompress_tile(a.job.g, a.job.tl, a.job.p, a.job.io, t, status, a.ws, a.job.g.S);
+}
+
+// Batched launches (see BatchArgs).
+__device__ __forceinline__ void job_of(const BatchArgs& b, uint64_t gb, uint32_t& jidx, uint64_t& t)
+{
+    // 32-bit arithmetic, results forced into scalar registers (every lane of
+    // the workgroup agrees)
+    const uint32_t g32 = (uint32_t) gb;
+    uint32_t j, tt;
+    if (b.interleave) {
+        j = g32 % b.njobs;
+        tt = g32 / b.njobs;
+    } else {
+        uint32_t lo = 0, hi = b.njobs;
+        while (hi - lo > 1) {
+            uint32_t mid = (lo + hi) / 2;
+            if (b.job_tile0[mid] <= gb)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        j = lo;
+        tt = g32 - (uint32_t) b.job_tile0[lo];
+    }
+    jidx = __builtin_amdgcn_readfirstlane(j);
+    t = __builtin_amdgcn_readfirstlane(tt);
+}
+
+// Copy one descriptor into LDS (the workgroup reads it many times).
+template <typename D>
+__device__ __forceinline__ void stage_desc(const void* base, uint32_t jidx, D* dst)
+{
+    static_assert(sizeof(D) % 8 == 0, "descriptors are 8-byte multiples");
+    const unsigned long long* src = (const unsigned long long*) ((const D*) base + jidx);
+    unsigned long long* d = (unsigned long long*) dst;
+    for (uint32_t i = threadIdx.x; i < sizeof(D) / 8; i += blockDim.x)
+        d[i] = src[i];
+    __syncthreads();
+}
+
+// Uniform batches: geometry in kernel arguments, IO blocks per job.
+__global__ void __launch_bounds__(kMaxTileThreads, 4) k_homsum_batch(BatchArgs b)
+{
+    __shared__ HomsumIO sio;
+    unsigned long long* status;
+    if (!launch_status(b.ws, b.total_tiles, blockIdx.x, status))
+        return;
+    uint32_t jidx;
+    uint64_t t;
+    job_of(b, blockIdx.x, jidx, t);
+    stage_desc(b.jobs, jidx, &sio);
+    homsum_tile(b.g, b.tl, b.p, sio, t, status + b.job_tile0[jidx] * kStatusStride, b.ws, b.total_chains);
+}
+
+__global__ void __launch_bounds__(kMaxTileThreads, 6) k_compress_batch(BatchArgs b)
+{
+    __shared__ CompressIO sio;
+    unsigned long long* status;
+    if (!launch_status(b.ws, b.total_tiles, blockIdx.x, status))
+        return;
+    uint32_t jidx;
+    uint64_t t;
+    job_of(b, blockIdx.x, jidx, t);
+    stage_desc(b.jobs, jidx, &sio);
+    compress_tile(b.g, b.tl, b.p, sio, t, status + b.job_tile0[jidx] * kStatusStride, b.ws, b.total_chains);
+}
+
+__global__ void __launch_bounds__(kMaxTileThreads, 7) k_decompress_batch(BatchArgs b)
+{
+    __shared__ DecompressIO sio;
+    unsigned long long* status;
+    if (!launch_status(b.ws, b.total_tiles, blockIdx.x, status))
+        return;
+    uint32_t jidx;
+    uint64_t t;
+    job_of(b, blockIdx.x, jidx, t);
+    stage_desc(b.jobs, jidx, &sio);
+    decompress_tile(b.g, b.tl, b.p, sio, t, status + b.job_tile0[jidx] * kStatusStride, b.ws, b.total_chains);
+}
+
+// Mixed-geometry batches: whole descriptors per job.
+__global__ void __launch_bounds__(kMaxTileThreads, 4) k_homsum_mixed(BatchArgs b)
+{
+    __shared__ HomsumJob sj;
+    unsigned long long* status;
+    if (!launch_status(b.ws, b.total_tiles, blockIdx.x, status))
+        return;
+    uint32_t jidx;
+    uint64_t t;
+    job_of(b, blockIdx.x, jidx, t);
+    stage_desc(b.jobs, jidx, &sj);
+    homsum_tile(sj.g, sj.tl, sj.p, sj.io, t, status + b.job_tile0[jidx] * kStatusStride, b.ws, b.total_chains);
+}
+
+__global__ void __launch_bounds__(kMaxTileThreads, 6) k_compress_mixed(BatchArgs b)
+{
+    __shared__ CompressJob sj;
+    unsigned long long* status;
+    if (!launch_status(b.ws, b.total_tiles, blockIdx.x, status))
+        return;
+    uint32_t jidx;
+    uint64_t t;
+    job_of(b, blockIdx.x, jidx, t);
+    stage_desc(b.jobs, jidx, &sj);
+    compress_tile(sj.g, sj.tl, sj.p, sj.io, t, status + b.job_tile0[jidx] * kStatusStride, b.ws, b.total_chains);
+}
+
+__global__ void __launch_bounds__(kMaxTileThreads, 7) k_decompress_mixed(BatchArgs b)
+{
+    __shared__ DecompressJob sj;
+    unsigned long long* status;
+    if (!launch_status(b.ws, b.total_tiles, blockIdx.x, status))
+        return;
+    uint32_t jidx;
+    uint64_t t;
+    job_of(b, blockIdx.x, jidx, t);
+    stage_desc(b.jobs, jidx, &sj);
+    decompress_tile(sj.g, sj.tl, sj.p, sj.io, t, status + b.job_tile0[jidx] * kStatusStride, b.ws, b.total_chains);
 }
 
 // ------------------------------------------------------------ parse
 
 // Rebuild the index of a stream that carries none (for example one written
 // by the CPU library).  One wave walks one chunk: the wave pulls a window of
-// the chunk into LDS and lane 0 follows the width bytes through it; the
-// remainder segment continues chunk T-1's walk (ZCCL/src/ZCCLd_float.c:597-745).
+// the chunk into LDS and follows the width bytes through it; the remainder
+// segment continues chunk T-1's walk (ZCCL/src/ZCCLd_float.c:597-745).
 constexpr uint32_t kParseWindow = 8192;
 
 __global__ void __launch_bounds__(64) k_parse(ParseArgs a)
@@ -803,16 +1159,15 @@ __global__ void __launch_bounds__(64) k_parse(ParseArgs a)
         const uint32_t s = chunk + (uint32_t) sidx;
         const uint64_t nb = s < g.T ? g.nbT : g.nbR;
         const uint64_t len = s < g.T ? g.tb : g.R;
-        const uint64_t g0 = (uint64_t) s * g.nbT;
-        const uint64_t tile0 = (uint64_t) s * g.tilesT;
+        const uint64_t b0 = (uint64_t) s * g.nbT;
+        const uint64_t grp0 = (uint64_t) s * g.gT;
         if (lane == 0) {
             ix.seed_off[s] = pos;
             if (nb == 0)
-                ix.tile_off[tile0] = pos + 4;
+                ix.group_off[grp0] = pos + 4;
         }
         pos += 4;
         for (uint64_t jb = 0; jb < nb; jb++) {
-            // refill the window when the width byte is outside it
             if (pos < win_lo || pos >= win_hi) {
                 uint64_t lo = pos & ~(uint64_t) 3;
                 uint64_t hi = lo + kParseWindow;
@@ -841,9 +1196,9 @@ __global__ void __launch_bounds__(64) k_parse(ParseArgs a)
             uint64_t left = len - 1 - jb * g.L;
             uint32_t nbl = (uint32_t) (left < g.L ? left : g.L);
             if (lane == 0) {
-                ix.widths[g0 + jb] = (uint8_t) w;
-                if (jb % g.G == 0)
-                    ix.tile_off[tile0 + jb / g.G] = pos;
+                ix.widths[b0 + jb] = (uint8_t) w;
+                if (jb % g.IG == 0)
+                    ix.group_off[grp0 + jb / g.IG] = pos;
             }
             pos += block_bytes(w, nbl);
             if (pos > cap) {
@@ -878,7 +1233,6 @@ __global__ void k_case_count(const uint8_t* w1, const uint8_t* w2, uint64_t nblo
         uint32_t a = w1[i] != 0, b = w2[i] != 0;
         c[a * 2 + b] += 1;
     }
-    // c index: (a,b) = (0,0)->0, (0,1)->1, (1,0)->2, (1,1)->3
 #pragma unroll
     for (int k = 0; k < 4; k++) {
         uint32_t v = c[k];
@@ -902,7 +1256,8 @@ __global__ void k_fill_u64(unsigned long long* p, uint64_t n, unsigned long long
 __global__ void k_ws_init(WsHeader* ws, uint64_t capacity)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
-        ws->ticket = 0;
+        ws->parity = 0;
+        ws->chains_done = 0;
         ws->capacity = capacity;
         ws->errors = 0;
     }
@@ -910,23 +1265,63 @@ __global__ void k_ws_init(WsHeader* ws, uint64_t capacity)
 
 // ------------------------------------------------------------ launchers
 
-static size_t lds_bytes(const Geom& g) { return (size_t) align_up(g.tile_bytes_max + 64, 16); }
+size_t compress_lds_bytes(const Tiling& tl) { return (size_t) align_up(tl.tile_bytes_max + 64, 16); }
+
+uint32_t stage_bytes_for(const Tiling& tl)
+{
+    uint64_t need = align_up(tl.tile_bytes_max + 64, 16);
+    return (uint32_t) (need > kStageMin ? need : kStageMin);
+}
+
+size_t operand_lds_bytes(const Tiling& tl, int k, uint32_t stage)
+{
+    return (size_t) (stage + align_up((uint64_t) k * tl.G, 16) + align_up(2ull * k * tl.G, 16));
+}
 
 hipError_t launch_compress(const CompressArgs& a, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_compress, dim3((uint32_t) a.g.ntiles), dim3(a.g.threads), lds_bytes(a.g), s, a);
+    hipLaunchKernelGGL(k_compress, dim3((uint32_t) a.job.tl.ntiles), dim3(a.job.tl.threads),
+        compress_lds_bytes(a.job.tl), s, a);
     return hipGetLastError();
 }
 
 hipError_t launch_homsum(const HomsumArgs& a, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_homsum, dim3((uint32_t) a.g.ntiles), dim3(a.g.threads), lds_bytes(a.g), s, a);
+    hipLaunchKernelGGL(k_homsum, dim3((uint32_t) a.job.tl.ntiles), dim3(a.job.tl.threads),
+        operand_lds_bytes(a.job.tl, a.job.p.k, a.job.p.stage_bytes), s, a);
     return hipGetLastError();
 }
 
 hipError_t launch_decompress(const DecompressArgs& a, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_decompress, dim3((uint32_t) a.g.ntiles), dim3(a.g.threads), lds_bytes(a.g), s, a);
+    hipLaunchKernelGGL(k_decompress, dim3((uint32_t) a.job.tl.ntiles), dim3(a.job.tl.threads),
+        operand_lds_bytes(a.job.tl, 1, a.job.p.stage_bytes), s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch(BatchKind kind, const BatchArgs& b, uint32_t threads, size_t lds, hipStream_t s)
+{
+    dim3 grid((uint32_t) b.total_tiles), block(threads);
+    switch (kind) {
+    case BatchKind::Compress:
+        if (b.uniform)
+            hipLaunchKernelGGL(k_compress_batch, grid, block, lds, s, b);
+        else
+            hipLaunchKernelGGL(k_compress_mixed, grid, block, lds, s, b);
+        break;
+    case BatchKind::Homsum:
+        if (b.uniform)
+            hipLaunchKernelGGL(k_homsum_batch, grid, block, lds, s, b);
+        else
+            hipLaunchKernelGGL(k_homsum_mixed, grid, block, lds, s, b);
+        break;
+    case BatchKind::Decompress:
+        if (b.uniform)
+            hipLaunchKernelGGL(k_decompress_batch, grid, block, lds, s, b);
+        else
+            hipLaunchKernelGGL(k_decompress_mixed, grid, block, lds, s, b);
+        break;
+    }
     return hipGetLastError();
 }
 
@@ -948,7 +1343,7 @@ hipError_t launch_case_count(const uint8_t* w1, const uint8_t* w2, uint64_t nblo
 
 hipError_t launch_ws_init(WsHeader* ws, uint64_t capacity, hipStream_t s)
 {
-    uint64_t n = 2 * capacity;
+    uint64_t n = 2 * capacity * kStatusStride;
     unsigned long long* st = (unsigned long long*) ((char*) ws + sizeof(WsHeader));
     if (n) {
         uint32_t blocks = (uint32_t) (ceil_div(n, 256) < 1024 ? ceil_div(n, 256) : 1024);
