@@ -44,19 +44,24 @@ constexpr uint32_t kGroupUnits = 576;  // units per index group (at most)
 constexpr uint32_t kMaxTileBlocks = 256;
 constexpr uint32_t kStatusStride = 16; // u64 words between lookback status entries (one 128 B line each)
 
+// Element, block, group and tile counts are 32-bit: streams hold fewer than
+// 2^31 values (the reference indexes them with int, ZCCL/src/ZCCL_float.c:553-554).
+// Byte offsets inside streams stay 64-bit.
+constexpr uint64_t kMaxN = (1ull << 31) - 1;
+
 struct Geom {
-    uint64_t n;
+    uint32_t n;
     uint32_t L, T;
-    uint64_t tb, R;
+    uint32_t tb, R;
     uint32_t S;          // segments: T (+1 when R > 0)
-    uint64_t nbT, nbR;   // blocks per regular / remainder segment
-    uint64_t nblocks;
+    uint32_t nbT, nbR;   // blocks per regular / remainder segment
+    uint32_t nblocks;
     uint32_t U;          // 4-element units per block = ceil(L/4)
     uint32_t IG;         // blocks per index group
-    uint64_t gT, gR;     // groups per regular / remainder segment
-    uint64_t ngroups;
+    uint32_t gT, gR;     // groups per regular / remainder segment
+    uint32_t ngroups;
     // index layout (byte offsets inside an index buffer)
-    uint64_t ix_group, ix_seed, ix_meta, ix_bytes;
+    uint32_t ix_group, ix_seed, ix_meta, ix_bytes;
 };
 
 ZK_HD uint64_t ceil_div(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
@@ -75,26 +80,26 @@ ZK_HD uint32_t block_bytes(uint32_t b, uint32_t n)
 // ZCCL/src/ZCCL_float.c:543-544 underflows) or L is out of range.
 ZK_HD bool make_geom(uint64_t n, uint32_t L, uint32_t T, Geom* g)
 {
-    if (L < 1 || L > kMaxL || T < 1 || n < T)
+    if (L < 1 || L > kMaxL || T < 1 || n < T || n > kMaxN)
         return false;
-    g->n = n;
+    g->n = (uint32_t) n;
     g->L = L;
     g->T = T;
-    g->tb = n / T;
-    g->R = n % T;
+    g->tb = g->n / T;
+    g->R = g->n % T;
     g->S = T + (g->R ? 1u : 0u);
-    g->nbT = ceil_div(g->tb - 1, L);
-    g->nbR = g->R ? ceil_div(g->R - 1, L) : 0;
-    g->nblocks = (uint64_t) T * g->nbT + g->nbR;
+    g->nbT = (uint32_t) ceil_div(g->tb - 1, L);
+    g->nbR = g->R ? (uint32_t) ceil_div(g->R - 1, L) : 0u;
+    g->nblocks = T * g->nbT + g->nbR;
     g->U = (L + 3) / 4;
     uint32_t ig = kGroupUnits / g->U;
     g->IG = ig > 64 ? 64 : (ig < 1 ? 1 : ig);
-    g->gT = g->nbT ? ceil_div(g->nbT, g->IG) : 1;
-    g->gR = g->R ? (g->nbR ? ceil_div(g->nbR, g->IG) : 1) : 0;
-    g->ngroups = (uint64_t) T * g->gT + g->gR;
-    g->ix_group = align_up(g->nblocks, 16);
+    g->gT = g->nbT ? (uint32_t) ceil_div(g->nbT, g->IG) : 1u;
+    g->gR = g->R ? (g->nbR ? (uint32_t) ceil_div(g->nbR, g->IG) : 1u) : 0u;
+    g->ngroups = T * g->gT + g->gR;
+    g->ix_group = (uint32_t) align_up(g->nblocks, 16);
     g->ix_seed = g->ix_group + 8 * g->ngroups;
-    g->ix_meta = align_up(g->ix_seed + 8ull * g->S, 16);
+    g->ix_meta = (uint32_t) align_up(g->ix_seed + 8ull * g->S, 16);
     g->ix_bytes = g->ix_meta + 32;
     return true;
 }
@@ -121,7 +126,7 @@ struct Tiling {
     uint32_t TG;         // groups per tile
     uint32_t G;          // blocks per tile (TG * IG)
     uint32_t threads;    // workgroup size
-    uint64_t tilesT, tilesR, ntiles;
+    uint32_t tilesT, tilesR, ntiles;
     uint32_t tile_bytes_max;  // worst-case encoded bytes of one tile
 };
 
@@ -140,9 +145,9 @@ ZK_HD Tiling make_tiling(const Geom& g, uint32_t max_groups = 4)
     uint32_t units = t.G * g.U;
     uint32_t th = (uint32_t) ceil_div(units, kUnitsPerThread);
     t.threads = (uint32_t) align_up(th < 64 ? 64 : th, 64);
-    t.tilesT = ceil_div(g.gT, tg);
-    t.tilesR = g.gR ? ceil_div(g.gR, tg) : 0;
-    t.ntiles = (uint64_t) g.T * t.tilesT + t.tilesR;
+    t.tilesT = (uint32_t) ceil_div(g.gT, tg);
+    t.tilesR = g.gR ? (uint32_t) ceil_div(g.gR, tg) : 0u;
+    t.ntiles = g.T * t.tilesT + t.tilesR;
     t.tile_bytes_max = t.G * block_bytes(32, g.L);
     return t;
 }
@@ -151,49 +156,49 @@ struct Tile {
     uint32_t s;              // segment
     uint32_t nblk;           // blocks in this tile
     uint32_t ngrp;           // index groups in this tile
-    uint64_t lt;             // tile index inside the segment
-    uint64_t g0;             // global index of the tile's first block
-    uint64_t grp0;           // global index of the tile's first group
-    uint64_t seg_a, seg_end; // segment element range [seg_a, seg_end)
-    uint64_t e0;             // first element of the tile's first block
-    uint64_t seg_tile0;      // index of the segment's first tile
-    uint64_t seg_tiles;      // tiles in the segment
+    uint32_t lt;             // tile index inside the segment
+    uint32_t g0;             // global index of the tile's first block
+    uint32_t grp0;           // global index of the tile's first group
+    uint32_t seg_a, seg_end; // segment element range [seg_a, seg_end)
+    uint32_t e0;             // first element of the tile's first block
+    uint32_t seg_tile0;      // index of the segment's first tile
+    uint32_t seg_tiles;      // tiles in the segment
 };
 
-ZK_HD Tile tile_of(const Geom& g, const Tiling& tl, uint64_t t)
+ZK_HD Tile tile_of(const Geom& g, const Tiling& tl, uint32_t t)
 {
     Tile x;
-    uint64_t nb_s, ng_s;
-    uint64_t regular = (uint64_t) g.T * tl.tilesT;
+    uint32_t nb_s, ng_s;
+    const uint32_t regular = g.T * tl.tilesT;
     if (t < regular) {
-        x.s = (uint32_t) (t / tl.tilesT);
-        x.lt = t % tl.tilesT;
-        x.seg_a = (uint64_t) x.s * g.tb;
+        x.s = t / tl.tilesT;
+        x.lt = t - x.s * tl.tilesT;
+        x.seg_a = x.s * g.tb;
         x.seg_end = x.seg_a + g.tb;
         nb_s = g.nbT;
         ng_s = g.gT;
-        x.seg_tile0 = (uint64_t) x.s * tl.tilesT;
+        x.seg_tile0 = x.s * tl.tilesT;
         x.seg_tiles = tl.tilesT;
-        x.g0 = (uint64_t) x.s * g.nbT;
-        x.grp0 = (uint64_t) x.s * g.gT;
+        x.g0 = x.s * g.nbT;
+        x.grp0 = x.s * g.gT;
     } else {
         x.s = g.T;
         x.lt = t - regular;
-        x.seg_a = (uint64_t) g.T * g.tb;
+        x.seg_a = g.T * g.tb;
         x.seg_end = g.n;
         nb_s = g.nbR;
         ng_s = g.gR;
         x.seg_tile0 = regular;
         x.seg_tiles = tl.tilesR;
-        x.g0 = (uint64_t) g.T * g.nbT;
-        x.grp0 = (uint64_t) g.T * g.gT;
+        x.g0 = g.T * g.nbT;
+        x.grp0 = g.T * g.gT;
     }
-    uint64_t first = x.lt * tl.G;
-    uint64_t left = nb_s > first ? nb_s - first : 0;
-    x.nblk = (uint32_t) (left < tl.G ? left : tl.G);
-    uint64_t gfirst = x.lt * tl.TG;
-    uint64_t gleft = ng_s > gfirst ? ng_s - gfirst : 0;
-    x.ngrp = (uint32_t) (gleft < tl.TG ? gleft : tl.TG);
+    const uint32_t first = x.lt * tl.G;
+    const uint32_t left = nb_s > first ? nb_s - first : 0u;
+    x.nblk = left < tl.G ? left : tl.G;
+    const uint32_t gfirst = x.lt * tl.TG;
+    const uint32_t gleft = ng_s > gfirst ? ng_s - gfirst : 0u;
+    x.ngrp = gleft < tl.TG ? gleft : tl.TG;
     x.g0 += first;
     x.grp0 += gfirst;
     x.e0 = x.seg_a + 1 + first * g.L;
@@ -203,9 +208,9 @@ ZK_HD Tile tile_of(const Geom& g, const Tiling& tl, uint64_t t)
 // Elements in block j of tile x.
 ZK_HD uint32_t tile_block_len(const Geom& g, const Tile& x, uint32_t j)
 {
-    uint64_t e = x.e0 + (uint64_t) j * g.L;
-    uint64_t left = x.seg_end - e;
-    return (uint32_t) (left < g.L ? left : g.L);
+    const uint32_t e = x.e0 + j * g.L;
+    const uint32_t left = x.seg_end - e;
+    return left < g.L ? left : g.L;
 }
 
 // Workspace: decoupled-lookback state.  Two status arrays alternate between

@@ -158,7 +158,7 @@ __device__ __forceinline__ unsigned long long wave_sum64(unsigned long long x)
 
 // Exclusive scan of one uint32 per thread over the workgroup (blockDim a
 // multiple of 64, at most 1024).  total receives the workgroup sum.
-__device__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_wave, uint32_t& total)
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_wave, uint32_t& total)
 {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
     uint32_t x = wave_incl_scan(v);
@@ -208,7 +208,7 @@ __device__ __forceinline__ void raise_error(WsHeader* ws, unsigned long long bit
 // when it has published just its aggregate.  Every wait is bounded: past
 // kSpinLimit polls the tile records ZK_ERR_SPIN and proceeds, so the grid
 // always drains.
-__device__ unsigned long long lookback(unsigned long long* status, uint64_t t, uint64_t t0,
+__device__ __forceinline__ unsigned long long lookback(unsigned long long* status, uint64_t t, uint64_t t0,
     unsigned long long agg, unsigned long long init, WsHeader* ws)
 {
     const int lane = threadIdx.x & 63;
@@ -265,7 +265,7 @@ __device__ unsigned long long lookback(unsigned long long* status, uint64_t t, u
 // Status arrays of the current launch.  The workgroup also clears its share
 // of the array the next launch will use.  Returns false when the workspace is
 // too small (then no workgroup of the launch does anything).
-__device__ bool launch_status(WsHeader* ws, uint64_t ntiles_total, uint64_t t, unsigned long long*& status)
+__device__ __forceinline__ bool launch_status(WsHeader* ws, uint64_t ntiles_total, uint64_t t, unsigned long long*& status)
 {
     const uint64_t cap = ws->capacity;
     if (cap < ntiles_total) {
@@ -288,7 +288,7 @@ __device__ bool launch_status(WsHeader* ws, uint64_t ntiles_total, uint64_t t, u
 // has published.  A finished chain implies every tile of the chain published
 // (and so read the parity); when the last chain of the launch finishes the
 // parity flips for the next launch.
-__device__ void chain_finished(WsHeader* ws, uint32_t nchains)
+__device__ __forceinline__ void chain_finished(WsHeader* ws, uint32_t nchains)
 {
     unsigned int old = atomicAdd(&ws->chains_done, 1u);
     if (old == nchains - 1) {
@@ -299,52 +299,79 @@ __device__ void chain_finished(WsHeader* ws, uint32_t nchains)
 
 // ------------------------------------------------------------ unit codec
 
-// Decode the up-to-4 deltas of unit u of a block whose width byte sits at LDS
-// byte bo.  Layout: ZCCL/src/ZCCL_TypeManager.c:423-517 (extract) and
-// :906-950 (sign bitmap).
-__device__ __forceinline__ void decode_unit(const uint32_t* buf, uint32_t bo, uint32_t w,
-    uint32_t nj, uint32_t u, uint32_t d[4])
+// Unaligned LDS reads: gfx950 serves a byte-aligned ds_read_b32/b64/b128
+// as one instruction.
+__device__ __forceinline__ uint32_t lds_ld4(const uint8_t* p)
+{
+    uint32_t v;
+    __builtin_memcpy(&v, p, 4);
+    return v;
+}
+
+__device__ __forceinline__ uint2 lds_ld8(const uint8_t* p)
+{
+    uint2 v;
+    __builtin_memcpy(&v, p, 8);
+    return v;
+}
+
+__device__ __forceinline__ uint4 lds_ld16(const uint8_t* p)
+{
+    uint4 v;
+    __builtin_memcpy(&v, p, 16);
+    return v;
+}
+
+// Add the up-to-4 deltas of unit u of a block (width byte at bp, width w > 0,
+// nj elements) to acc[].  Magnitude = (high bytes << r) | low r bits; the
+// sign bit s (MSB-first bitmap) is applied as (m ^ -s) + s.
+// ZCCL/src/ZCCL_TypeManager.c:423-517 (extract) and :906-950 (sign bitmap).
+__device__ __forceinline__ void accumulate_unit(const uint8_t* bp, uint32_t w, uint32_t nj, uint32_t u,
+    uint32_t acc[4])
 {
     const uint32_t B = w >> 3, r = w & 7, sb = (nj + 7) >> 3;
-    uint32_t sbyte = lds_u8(buf, bo + 1 + (u >> 1));
-    uint32_t nib = (u & 1) ? (sbyte & 15u) : (sbyte >> 4);
-    uint32_t mag[4] = {0, 0, 0, 0};
-    if (B) {
-        uint32_t W[4];
-        lds_rd16(buf, bo + 1 + sb + 4 * u * B, W);
-        switch (B) {
-        case 1:
-            mag[0] = W[0] & 0xFF; mag[1] = (W[0] >> 8) & 0xFF;
-            mag[2] = (W[0] >> 16) & 0xFF; mag[3] = W[0] >> 24;
-            break;
-        case 2:
-            mag[0] = W[0] & 0xFFFF; mag[1] = W[0] >> 16;
-            mag[2] = W[1] & 0xFFFF; mag[3] = W[1] >> 16;
-            break;
-        case 3:
-            mag[0] = W[0] & 0xFFFFFF;
-            mag[1] = (W[0] >> 24) | ((W[1] & 0xFFFF) << 8);
-            mag[2] = (W[1] >> 16) | ((W[2] & 0xFF) << 16);
-            mag[3] = W[2] >> 8;
-            break;
-        default:
-            mag[0] = W[0]; mag[1] = W[1]; mag[2] = W[2]; mag[3] = W[3];
-            break;
+    const uint32_t nib = (uint32_t) (bp[1 + (u >> 1)] >> ((~u & 1u) << 2)) & 15u;
+    const uint8_t* hp = bp + 1 + sb;
+    uint32_t m[4];
+    if (B == 0) {
+        m[0] = m[1] = m[2] = m[3] = 0;
+    } else if (B == 1) {
+        uint32_t H = lds_ld4(hp + 4 * u);
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+            m[c] = __builtin_amdgcn_ubfe(H, 8 * c, 8) << r;
+    } else if (B == 2) {
+        uint2 H = lds_ld8(hp + 8 * u);
+        m[0] = (H.x & 0xFFFFu) << r;
+        m[1] = (H.x >> 16) << r;
+        m[2] = (H.y & 0xFFFFu) << r;
+        m[3] = (H.y >> 16) << r;
+    } else {
+        uint4 H = lds_ld16(hp + 4 * u * B);
+        if (B == 3) {
+            m[0] = H.x & 0xFFFFFFu;
+            m[1] = __builtin_amdgcn_alignbyte(H.y, H.x, 3) & 0xFFFFFFu;
+            m[2] = __builtin_amdgcn_alignbyte(H.z, H.y, 2) & 0xFFFFFFu;
+            m[3] = H.z >> 8;
+        } else {
+            m[0] = H.x; m[1] = H.y; m[2] = H.z; m[3] = H.w;
         }
 #pragma unroll
         for (int c = 0; c < 4; c++)
-            mag[c] <<= r;
+            m[c] <<= r;
     }
     if (r) {
-        uint32_t bitpos = 4 * u * r;
-        uint32_t X = __builtin_bswap32(lds_rd4(buf, bo + 1 + sb + B * nj + (bitpos >> 3))) << (bitpos & 7);
+        const uint32_t bitpos = 4 * u * r;
+        const uint32_t X = __builtin_bswap32(lds_ld4(hp + B * nj + (bitpos >> 3))) << (bitpos & 7);
 #pragma unroll
         for (int c = 0; c < 4; c++)
-            mag[c] |= (X << (c * r)) >> (32 - r);
+            m[c] |= __builtin_amdgcn_ubfe(X, 32 - r * (c + 1), r);
     }
 #pragma unroll
-    for (int c = 0; c < 4; c++)
-        d[c] = ((nib >> (3 - c)) & 1u) ? 0u - mag[c] : mag[c];
+    for (int c = 0; c < 4; c++) {
+        const uint32_t s = (uint32_t) __builtin_amdgcn_sbfe((int32_t) nib, 3 - c, 1);  // 0 or ~0
+        acc[c] += (m[c] ^ s) - s;
+    }
 }
 
 // OR the encoding of unit u (cnt deltas d[], two's complement) of a block of
@@ -436,7 +463,7 @@ __device__ __forceinline__ void store_bytes(uint8_t* dst, const void* src, int n
 // Write nbytes of LDS (from LDS byte 0) to dst.  16-byte aligned interior
 // chunks go out as whole stores; the partial first and last chunk are written
 // byte by byte so neighbouring tiles' bytes are never touched.
-__device__ void store_tile(const uint32_t* buf, uint8_t* dst, uint32_t nbytes)
+__device__ __forceinline__ void store_tile(const uint32_t* buf, uint8_t* dst, uint32_t nbytes)
 {
     uintptr_t d0 = (uintptr_t) dst, d1 = d0 + nbytes;
     uintptr_t c0 = d0 & ~(uintptr_t) 15;
@@ -516,7 +543,7 @@ __device__ __forceinline__ void fold_block_max(uint32_t* bmax, const Units& un, 
 // Output side shared by compress and the homomorphic sum: per-block widths
 // from sh.bmax, block offsets, the lookback for the tile's stream offset,
 // encoding into LDS, the tile store, seed / header entry and output index.
-__device__ void encode_tile(const Geom& g, const Tiling& tl, const Tile& x, uint64_t t,
+__device__ __forceinline__ void encode_tile(const Geom& g, const Tiling& tl, const Tile& x, uint64_t t,
     TileShared& sh, uint32_t* buf, unsigned long long* status, WsHeader* ws, const Units& un,
     const uint32_t (&d)[kUnitsPerThread][4], uint32_t seed,
     uint8_t* out, uint64_t out_cap, uint8_t* index, unsigned long long* size_out, bool last_tile,
@@ -602,7 +629,7 @@ __device__ void encode_tile(const Geom& g, const Tiling& tl, const Tile& x, uint
 
 // ------------------------------------------------------------ compress
 
-__device__ void compress_tile(const Geom& g, const Tiling& tl, const OpParams& p, const CompressIO& io,
+__device__ __forceinline__ void compress_tile(const Geom& g, const Tiling& tl, const OpParams& p, const CompressIO& io,
     uint64_t t, unsigned long long* status, WsHeader* ws, uint32_t nchains)
 {
     __shared__ TileShared sh;
@@ -656,14 +683,13 @@ struct OperandShared {
     unsigned long long seedoff[kMaxOperands];
     uint32_t tot[kMaxOperands];             // tile bytes
     uint32_t lds[kMaxOperands];             // LDS position of the operand's bytes
-    uint32_t cstart[kMaxOperands + 1];      // chunk prefix within a load group
     uint32_t bad[kMaxOperands];
 };
 
 // Read the k operands' widths and offsets for tile x; per-operand scans give
 // each block's byte offset inside its operand's tile bytes.  Tables: widths
 // s_w[k][G] (u8) and offsets s_off[k][G] (u16).
-__device__ void operand_tables(const Geom& g, const Tiling& tl, const Tile& x, int k, const Operand* in,
+__device__ __forceinline__ void operand_tables(const Geom& g, const Tiling& tl, const Tile& x, int k, const Operand* in,
     OperandShared& os, uint8_t* s_w, uint16_t* s_off, WsHeader* ws)
 {
     const uint32_t tid = threadIdx.x, G = tl.G;
@@ -715,14 +741,13 @@ __device__ void operand_tables(const Geom& g, const Tiling& tl, const Tile& x, i
     __syncthreads();
 }
 
-// Group operands [r0, r) so their tile bytes fit the stage area; fills
-// os.lds / os.cstart for the group and returns its end.  Executed by one
-// thread; the caller synchronises.
-__device__ int plan_group(OperandShared& os, int r0, int k, uint32_t stage_bytes)
+// Group operands [r0, r) so their tile bytes (plus 16 bytes of read slack
+// each) fit the stage area; fills os.lds for the group and returns its end.
+// Executed by one thread; the caller synchronises.
+__device__ __forceinline__ int plan_group(OperandShared& os, int r0, int k, uint32_t stage_bytes)
 {
-    uint32_t pos = 0, ch = 0;
+    uint32_t pos = 0;
     int r = r0;
-    os.cstart[0] = 0;
     while (r < k) {
         uint32_t n = os.bad[r] ? 0u : os.tot[r];
         uint32_t need = (uint32_t) align_up(n + 16, 16);
@@ -730,58 +755,56 @@ __device__ int plan_group(OperandShared& os, int r0, int k, uint32_t stage_bytes
             break;
         os.lds[r] = pos;
         pos += need;
-        ch += (n + 15) / 16;
-        os.cstart[r + 1 - r0] = ch;
         r++;
     }
     return r;
 }
 
 // Load operands [r0, r1) of a group into LDS: LDS byte os.lds[r] + i holds
-// stream byte os.base[r] + i.  Every thread issues a batch of loads before
-// writing any of them, so a group costs about one memory round trip; only
-// bytes inside each stream are read.
+// stream byte os.base[r] + i.  Wave w loads operands r0+w, r0+w+nwaves, ...,
+// issuing a batch of 16-byte loads before writing any of them, so a group
+// costs about one memory round trip; only bytes inside each stream are read.
 constexpr int kLoadBatch = 8;
 
-__device__ void load_group(const Operand* in, const OperandShared& os, int r0, int r1, uint32_t* stage)
+__device__ __forceinline__ void load_group(const Operand* in, const OperandShared& os, int r0, int r1, uint32_t* stage)
 {
-    const uint32_t nch = os.cstart[r1 - r0];
-    for (uint32_t c0 = threadIdx.x; c0 < nch; c0 += kLoadBatch * blockDim.x) {
-        uint4 v[kLoadBatch];
-        uint32_t dst[kLoadBatch];
-        int r = r0;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    for (int r = r0 + wid; r < r1; r += nw) {
+        if (os.bad[r])
+            continue;
+        const uint32_t nch = (os.tot[r] + 15) / 16;
+        const uint64_t base = os.base[r], size = os.size[r];
+        const uint8_t* src = in[r].bytes + base;
+        uint4* dst = (uint4*) ((uint8_t*) stage + os.lds[r]);
+        for (uint32_t c0 = lane; c0 < nch; c0 += kLoadBatch * 64) {
+            uint4 v[kLoadBatch];
 #pragma unroll
-        for (int b = 0; b < kLoadBatch; b++) {
-            uint32_t c = c0 + b * blockDim.x;
-            dst[b] = ~0u;
-            v[b] = make_uint4(0, 0, 0, 0);
-            if (c < nch) {
-                while (os.cstart[r + 1 - r0] <= c)
-                    r++;
-                uint32_t cl = c - os.cstart[r - r0];
-                uint64_t off = os.base[r] + 16ull * cl;
-                const uint8_t* src = in[r].bytes + off;
-                if (off + 16 <= os.size[r]) {
-                    v[b] = ld16(src);
-                } else {
-                    uint32_t w[4] = {0, 0, 0, 0};
-                    for (uint32_t p = 0; p < 16 && off + p < os.size[r]; p++)
-                        w[p >> 2] |= (uint32_t) src[p] << (8 * (p & 3));
-                    v[b] = make_uint4(w[0], w[1], w[2], w[3]);
+            for (int b = 0; b < kLoadBatch; b++) {
+                const uint32_t c = c0 + 64 * b;
+                v[b] = make_uint4(0, 0, 0, 0);
+                if (c < nch) {
+                    const uint64_t off = base + 16ull * c;
+                    if (off + 16 <= size) {
+                        v[b] = ld16(src + 16ull * c);
+                    } else {
+                        uint32_t t[4] = {0, 0, 0, 0};
+                        for (uint32_t q = 0; q < 16 && off + q < size; q++)
+                            t[q >> 2] |= (uint32_t) src[16ull * c + q] << (8 * (q & 3));
+                        v[b] = make_uint4(t[0], t[1], t[2], t[3]);
+                    }
                 }
-                dst[b] = (os.lds[r] >> 2) + 4 * cl;
             }
-        }
 #pragma unroll
-        for (int b = 0; b < kLoadBatch; b++)
-            if (dst[b] != ~0u)
-                *(uint4*) &stage[dst[b]] = v[b];
+            for (int b = 0; b < kLoadBatch; b++)
+                if (c0 + 64 * b < nch)
+                    dst[c0 + 64 * b] = v[b];
+        }
     }
 }
 
 // ------------------------------------------------------------ homomorphic sum
 
-__device__ void homsum_tile(const Geom& g, const Tiling& tl, const OpParams& p, const HomsumIO& io,
+__device__ __forceinline__ void homsum_tile(const Geom& g, const Tiling& tl, const OpParams& p, const HomsumIO& io,
     uint64_t t, unsigned long long* status, WsHeader* ws, uint32_t nchains)
 {
     __shared__ TileShared sh;
@@ -793,6 +816,8 @@ __device__ void homsum_tile(const Geom& g, const Tiling& tl, const OpParams& p, 
     uint8_t* s_w = (uint8_t*) (zk_lds + p.stage_bytes / 4);
     uint16_t* s_off = (uint16_t*) (s_w + align_up((uint64_t) p.k * tl.G, 16));
     const Units un = tile_units(g, x);
+    const uint32_t last_j = x.nblk ? x.nblk - 1 : 0u;
+    const uint32_t last_len = x.nblk ? tile_block_len(g, x, last_j) : 0u;
     operand_tables(g, tl, x, p.k, io.in, os, s_w, s_off, ws);
     uint32_t acc[kUnitsPerThread][4];
 #pragma unroll
@@ -807,30 +832,25 @@ __device__ void homsum_tile(const Geom& g, const Tiling& tl, const OpParams& p, 
         const int r1 = s_grp_end;
         load_group(io.in, os, r0, r1, stage);
         __syncthreads();
-        for (int r = r0; r < r1; r++) {
-            if (os.bad[r])
+        const uint8_t* sbytes = (const uint8_t*) stage;
+#pragma unroll
+        for (uint32_t i = 0; i < kUnitsPerThread; i++) {
+            const uint32_t pk = un.pk[i];
+            if (!unit_cnt(pk))
                 continue;
-            const uint8_t* wr = s_w + r * tl.G;
-            const uint16_t* orr = s_off + r * tl.G;
-            const uint32_t lbase = os.lds[r];
-#pragma unroll
-            for (uint32_t i = 0; i < kUnitsPerThread; i++) {
-                const uint32_t pk = un.pk[i];
-                if (!unit_cnt(pk))
+            const uint32_t j = unit_j(pk), u = unit_u(pk);
+            const uint32_t nj = j == last_j ? last_len : g.L;
+            bool bad_width = false;
+            for (int r = r0; r < r1; r++) {
+                const uint32_t w = s_w[r * tl.G + j];
+                if (!w || os.bad[r])
                     continue;
-                const uint32_t j = unit_j(pk), u = unit_u(pk);
-                uint32_t w = wr[j];
-                if (!w)
-                    continue;
-                uint32_t dd[4];
-                uint32_t bo = lbase + orr[j];
-                decode_unit(stage, bo, w, tile_block_len(g, x, j), u, dd);
-#pragma unroll
-                for (int c = 0; c < 4; c++)
-                    acc[i][c] += dd[c];
-                if (u == 0 && lds_u8(stage, bo) != w)
-                    raise_error(ws, ZK_ERR_MISMATCH);
+                const uint8_t* bp = sbytes + os.lds[r] + s_off[r * tl.G + j];
+                accumulate_unit(bp, w, nj, u, acc[i]);
+                bad_width |= (u == 0 && bp[0] != w);
             }
+            if (bad_width)
+                raise_error(ws, ZK_ERR_MISMATCH);
         }
         __syncthreads();
         r0 = r1;
@@ -862,7 +882,7 @@ __device__ void homsum_tile(const Geom& g, const Tiling& tl, const OpParams& p, 
 
 // ------------------------------------------------------------ decompress
 
-__device__ void decompress_tile(const Geom& g, const Tiling& tl, const OpParams& p, const DecompressIO& io,
+__device__ __forceinline__ void decompress_tile(const Geom& g, const Tiling& tl, const OpParams& p, const DecompressIO& io,
     uint64_t t, unsigned long long* status, WsHeader* ws, uint32_t nchains)
 {
     __shared__ TileShared sh;
@@ -891,19 +911,21 @@ __device__ void decompress_tile(const Geom& g, const Tiling& tl, const OpParams&
     }
     __syncthreads();
     uint32_t pre[kUnitsPerThread][4], us[kUnitsPerThread];
+    const uint32_t last_j = x.nblk ? x.nblk - 1 : 0u;
+    const uint32_t last_len = x.nblk ? tile_block_len(g, x, last_j) : 0u;
+    const uint8_t* base = (const uint8_t*) stage + os.lds[0];
 #pragma unroll
     for (uint32_t i = 0; i < kUnitsPerThread; i++) {
         uint32_t d[4] = {0, 0, 0, 0};
         const uint32_t pk = un.pk[i], cnt = unit_cnt(pk);
-        if (cnt && !os.bad[0]) {
-            const uint32_t j = unit_j(pk), u = unit_u(pk);
-            uint32_t w = s_w[j];
-            if (w) {
-                uint32_t bo = os.lds[0] + s_off[j];
-                decode_unit(stage, bo, w, tile_block_len(g, x, j), u, d);
-                if (u == 0 && lds_u8(stage, bo) != w)
-                    raise_error(ws, ZK_ERR_MISMATCH);
-            }
+        const uint32_t j = unit_j(pk);
+        const uint32_t w = s_w[j];
+        if (cnt && w && !os.bad[0]) {
+            const uint32_t u = unit_u(pk);
+            const uint8_t* bp = base + s_off[j];
+            accumulate_unit(bp, w, j == last_j ? last_len : g.L, u, d);
+            if (u == 0 && bp[0] != w)
+                raise_error(ws, ZK_ERR_MISMATCH);
         }
         uint32_t run = 0;
 #pragma unroll
@@ -991,7 +1013,7 @@ __global__ void __launch_bounds__(kMaxTileThreads, 4) k_homsum(HomsumArgs a)
     homsum_tile(a.job.g, a.job.tl, a.job.p, a.job.io, t, status, a.ws, 1);
 }
 
-__global__ void __launch_bounds__(kMaxTileThreads, 7) k_decompress(DecompressArgs a)
+__global__ void __launch_bounds__(kMaxTileThreads, 6) k_decompress(DecompressArgs a)
 {
     const uint64_t t = blockIdx.x;
     unsigned long long* status;
@@ -1065,7 +1087,7 @@ __global__ void __launch_bounds__(kMaxTileThreads, 6) k_compress_batch(BatchArgs
     compress_tile(b.g, b.tl, b.p, sio, t, status + b.job_tile0[jidx] * kStatusStride, b.ws, b.total_chains);
 }
 
-__global__ void __launch_bounds__(kMaxTileThreads, 7) k_decompress_batch(BatchArgs b)
+__global__ void __launch_bounds__(kMaxTileThreads, 6) k_decompress_batch(BatchArgs b)
 {
     __shared__ DecompressIO sio;
     unsigned long long* status;
@@ -1105,7 +1127,7 @@ __global__ void __launch_bounds__(kMaxTileThreads, 6) k_compress_mixed(BatchArgs
     compress_tile(sj.g, sj.tl, sj.p, sj.io, t, status + b.job_tile0[jidx] * kStatusStride, b.ws, b.total_chains);
 }
 
-__global__ void __launch_bounds__(kMaxTileThreads, 7) k_decompress_mixed(BatchArgs b)
+__global__ void __launch_bounds__(kMaxTileThreads, 6) k_decompress_mixed(BatchArgs b)
 {
     __shared__ DecompressJob sj;
     unsigned long long* status;
