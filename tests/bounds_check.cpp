@@ -28,16 +28,16 @@ static long g_cases = 0;
 
 // LDS bytes the launchers request (mirrors zccl_kernels.hip).
 static uint64_t compress_lds(const Tiling& tl) { return align_up(tl.tile_bytes_max + 64, 16); }
-static uint64_t stage_bytes(const Tiling& tl)
+static uint64_t stage_bytes(const Tiling& tl, int op)
 {
     uint64_t need = align_up(tl.tile_bytes_max + 64, 16);
-    return need > 44 * 1024 ? need : 44 * 1024;
+    return op == 1 && need < 40 * 1024 ? 40 * 1024 : need;
 }
 
-static void check_tiling(const Geom& g, uint32_t groups, int op)
+static void check_tiling(const Geom& g, uint32_t threads, int op)
 {
-    const Tiling tl = make_tiling(g, groups);
-    REQUIRE(tl.threads >= 64 && tl.threads <= kTileUnits / kUnitsPerThread && tl.threads % 64 == 0,
+    const Tiling tl = make_tiling(g, threads);
+    REQUIRE(tl.threads >= 64 && tl.threads <= threads && tl.threads % 64 == 0,
         "threads %u", tl.threads);
     REQUIRE(tl.G <= kMaxTileBlocks && tl.G == tl.TG * g.IG, "G %u", tl.G);
     REQUIRE((uint64_t) tl.G * g.U <= (uint64_t) kUnitsPerThread * tl.threads, "units %u", tl.G * g.U);
@@ -96,12 +96,27 @@ static void check_tiling(const Geom& g, uint32_t groups, int op)
                 uint64_t e = x.e0 + (uint64_t) j * g.L + 4 * u;
                 REQUIRE(e + cnt <= x.seg_end && e + cnt <= g.n, "unit values");
                 REQUIRE(e >= 1 && e - 1 >= x.seg_a, "previous value");
-                // LDS bytes an encoder or decoder touches for this unit at width 32
-                uint32_t B = 4, sb = (nj + 7) >> 3;
-                uint64_t bo_max = tile_bytes - block_bytes(32, nj);  // last block start bound
-                uint64_t touch = bo_max + 1 + sb + 4ull * u * B + 20;
-                uint64_t lds = op == 0 ? compress_lds(tl) : stage_bytes(tl);
-                REQUIRE(touch + 16 <= lds, "LDS reach %lu of %lu", (unsigned long) touch, (unsigned long) lds);
+                // LDS bytes an encoder or decoder touches for this unit, every
+                // width: the decoder reads 4 bytes at each of the unit's 4
+                // high-byte fields (masked to B bytes) and 4 bytes of low bits
+                uint64_t lds = op == 0 ? compress_lds(tl) : stage_bytes(tl, op);
+                for (uint32_t w = 0; w <= 32; w++) {
+                    uint32_t B = w >> 3, r = w & 7, sb = w ? (nj + 7) >> 3 : 0;
+                    uint64_t bo = tile_bytes - block_bytes(w, nj);  // a block of width w ending the tile
+                    uint64_t hp = bo + 1 + sb;
+                    uint64_t hi_end = hp + 4ull * u * B + 3ull * B + 4;
+                    uint64_t lo_end = hp + (uint64_t) B * nj + ((4ull * u * r) >> 3) + 4;
+                    uint64_t nib_end = bo + (w ? 1 + (u >> 1) : 0) + 1;
+                    uint64_t reach = hi_end > lo_end ? hi_end : lo_end;
+                    reach = reach > nib_end ? reach : nib_end;
+                    // decoder: within the 16 bytes of slack after the tile's bytes
+                    REQUIRE(reach <= tile_bytes + 16, "decoder reach %lu tile %lu w %u", (unsigned long) reach,
+                        (unsigned long) tile_bytes, w);
+                    // encoder (16-byte OR of the high bytes, 4-byte OR of the low bits)
+                    uint64_t enc = w ? hp + 4ull * u * B + 16 : 0;
+                    REQUIRE(enc + 16 <= lds && reach + 16 <= lds + 16, "LDS reach %lu of %lu", (unsigned long) enc,
+                        (unsigned long) lds);
+                }
             }
         }
         if (op == 2) {
@@ -111,7 +126,7 @@ static void check_tiling(const Geom& g, uint32_t groups, int op)
         // homomorphic-sum operand tables for k = 32: per-operand need fits the stage
         if (op == 1) {
             uint64_t need = align_up(tile_bytes + 16, 16);
-            REQUIRE(need <= stage_bytes(tl), "stage");
+            REQUIRE(need <= stage_bytes(tl, op), "stage");
             REQUIRE(tl.G <= 256, "per-operand scan covers 4 blocks per lane");
         }
     }
@@ -141,9 +156,9 @@ int main()
                     "index layout");
                 // the parse kernel's group offsets: segment s group q = s*gT + jb/IG
                 REQUIRE((uint64_t) g.T * g.gT + g.gR == g.ngroups, "group count");
-                check_tiling(g, 4, 0);  // compress
-                check_tiling(g, 2, 1);  // homomorphic sum
-                check_tiling(g, 4, 2);  // decompress
+                check_tiling(g, 512, 0);  // compress
+                check_tiling(g, 256, 1);  // homomorphic sum
+                check_tiling(g, 512, 2);  // decompress
             }
     std::printf("ok %ld\n", g_cases);
     return 0;

@@ -127,8 +127,8 @@ extern "C" size_t zccl_num_tiles(size_t n, int L, int T)
     Geom g;
     if (!geom_of(n, L, T, &g))
         return 0;
-    uint64_t a = make_tiling(g, kCompressGroups).ntiles, b = make_tiling(g, kHomsumGroups).ntiles,
-             c = make_tiling(g, kDecompressGroups).ntiles;
+    uint64_t a = make_tiling(g, kCompressThreads).ntiles, b = make_tiling(g, kHomsumThreads).ntiles,
+             c = make_tiling(g, kDecompressThreads).ntiles;
     return (size_t) std::max(a, std::max(b, c));
 }
 
@@ -181,7 +181,7 @@ static OpParams op_params(const Tiling& tl, double inv, float eb, int k)
     p.inv = inv;
     p.eb = eb;
     p.k = k;
-    p.stage_bytes = stage_bytes_for(tl);
+    p.stage_bytes = stage_bytes_for(tl, k);
     return p;
 }
 
@@ -213,7 +213,7 @@ extern "C" int zccl_compress_dev(const float* x, size_t n, float eb, int L, int 
         return ZCCL_EINVAL;
     CompressArgs a;
     a.job.g = g;
-    a.job.tl = make_tiling(g, kCompressGroups);
+    a.job.tl = make_tiling(g, kCompressThreads);
     int rc = check_ws(ws, ws_bytes, a.job.tl.ntiles);
     if (rc)
         return rc;
@@ -236,7 +236,7 @@ extern "C" int zccl_decompress_dev(const unsigned char* in, const void* index, s
         return ZCCL_EINVAL;
     DecompressArgs a;
     a.job.g = g;
-    a.job.tl = make_tiling(g, kDecompressGroups);
+    a.job.tl = make_tiling(g, kDecompressThreads);
     int rc = check_ws(ws, ws_bytes, a.job.tl.ntiles);
     if (rc)
         return rc;
@@ -259,7 +259,7 @@ extern "C" int zccl_homsum_dev(const unsigned char* const* ins, const void* cons
     HomsumArgs a;
     memset(&a, 0, sizeof(a));
     a.job.g = g;
-    a.job.tl = make_tiling(g, kHomsumGroups);
+    a.job.tl = make_tiling(g, kHomsumThreads);
     int rc = check_ws(ws, ws_bytes, a.job.tl.ntiles);
     if (rc)
         return rc;
@@ -384,7 +384,7 @@ extern "C" zccl_batch* zccl_batch_homsum_create(int njobs, int k, const size_t* 
             *st = ZCCL_EINVAL;
             return nullptr;
         }
-        h.tl = make_tiling(h.g, kHomsumGroups);
+        h.tl = make_tiling(h.g, kHomsumThreads);
         h.p = op_params(h.tl, 0.0, 0.0f, k);
         for (int r = 0; r < k; r++) {
             const unsigned char* in = ins[(size_t) j * k + r];
@@ -421,7 +421,7 @@ extern "C" zccl_batch* zccl_batch_compress_create(int njobs, const size_t* nbEle
             *st = ZCCL_EINVAL;
             return nullptr;
         }
-        c.tl = make_tiling(c.g, kCompressGroups);
+        c.tl = make_tiling(c.g, kCompressThreads);
         c.p = op_params(c.tl, inverse_bound(ebs[j]), ebs[j], 1);
         c.io.x = xs[j];
         c.io.out = outs[j];
@@ -451,7 +451,7 @@ extern "C" zccl_batch* zccl_batch_decompress_create(int njobs, const size_t* nbE
             *st = ZCCL_EINVAL;
             return nullptr;
         }
-        d.tl = make_tiling(d.g, kDecompressGroups);
+        d.tl = make_tiling(d.g, kDecompressThreads);
         d.p = op_params(d.tl, 0.0, ebs[j], 1);
         d.io.in[0].bytes = ins[j];
         d.io.in[0].index = (const uint8_t*) indices[j];

@@ -13,8 +13,8 @@
 // Device layout chosen here:
 //   * a "segment" is one seeded run of the Lorenzo chain (T regular chunks plus
 //     the optional remainder);
-//   * a segment's blocks are cut into index groups of IG blocks (IG = 64 for
-//     L <= 36); every segment owns at least one group (an empty one when it
+//   * a segment's blocks are cut into index groups of IG blocks (IG = 16 for
+//     L <= 144); every segment owns at least one group (an empty one when it
 //     has no blocks);
 //   * a kernel tile is up to TG consecutive groups of one segment (a tile never
 //     straddles two segments);
@@ -39,8 +39,10 @@ namespace zk {
 constexpr int kMaxOperands = 32;     // operands of one fused homomorphic-sum launch
 constexpr uint32_t kMaxL = 4096;     // block length supported by the tile kernels
 constexpr uint32_t kUnitsPerThread = 4;
-constexpr uint32_t kTileUnits = 2304; // 4-element units per tile (at most)
+constexpr uint32_t kMaxTileThreads = 512;                            // 8 waves
+constexpr uint32_t kTileUnits = kMaxTileThreads * kUnitsPerThread;  // 4-element units per tile (at most)
 constexpr uint32_t kGroupUnits = 576;  // units per index group (at most)
+constexpr uint32_t kMaxGroupBlocks = 16;
 constexpr uint32_t kMaxTileBlocks = 256;
 constexpr uint32_t kStatusStride = 16; // u64 words between lookback status entries (one 128 B line each)
 
@@ -93,7 +95,7 @@ ZK_HD bool make_geom(uint64_t n, uint32_t L, uint32_t T, Geom* g)
     g->nblocks = T * g->nbT + g->nbR;
     g->U = (L + 3) / 4;
     uint32_t ig = kGroupUnits / g->U;
-    g->IG = ig > 64 ? 64 : (ig < 1 ? 1 : ig);
+    g->IG = ig > kMaxGroupBlocks ? kMaxGroupBlocks : (ig < 1 ? 1 : ig);
     g->gT = g->nbT ? (uint32_t) ceil_div(g->nbT, g->IG) : 1u;
     g->gR = g->R ? (g->nbR ? (uint32_t) ceil_div(g->nbR, g->IG) : 1u) : 0u;
     g->ngroups = T * g->gT + g->gR;
@@ -121,7 +123,9 @@ ZK_HD uint64_t worst_case_bytes(const Geom& g)
     return tot;
 }
 
-// Per-launch tiling: TG index groups per tile, up to kTileUnits units.
+// Per-launch tiling: TG index groups per tile, as many as fill a workgroup of
+// (at most) target_threads threads at kUnitsPerThread units each.  Targets
+// are multiples of 256 so a workgroup is a whole number of waves per SIMD.
 struct Tiling {
     uint32_t TG;         // groups per tile
     uint32_t G;          // blocks per tile (TG * IG)
@@ -130,12 +134,10 @@ struct Tiling {
     uint32_t tile_bytes_max;  // worst-case encoded bytes of one tile
 };
 
-ZK_HD Tiling make_tiling(const Geom& g, uint32_t max_groups = 4)
+ZK_HD Tiling make_tiling(const Geom& g, uint32_t target_threads)
 {
     Tiling t;
-    uint32_t tg = kTileUnits / (g.IG * g.U);
-    if (tg > max_groups)
-        tg = max_groups;
+    uint32_t tg = target_threads * kUnitsPerThread / (g.IG * g.U);
     if (tg < 1)
         tg = 1;
     while (tg > 1 && tg * g.IG > kMaxTileBlocks)

@@ -9,18 +9,16 @@ namespace zk {
 
 // Workgroup size bound of every tile kernel: a tile holds at most kTileUnits
 // 4-value units and a thread owns four of them.
-constexpr uint32_t kMaxTileThreads = kTileUnits / kUnitsPerThread;
+// Workgroup size targets (threads): compress and decompress tiles fill 8
+// waves, the homomorphic sum (more state per thread, k operands' bytes in LDS)
+// fills 4 so three of its workgroups share a CU.
+constexpr uint32_t kCompressThreads = 512;
+constexpr uint32_t kDecompressThreads = 512;
+constexpr uint32_t kHomsumThreads = 256;
 
-// Index groups per tile: compress and decompress use 4 (256 blocks at
-// L = 36); the homomorphic sum keeps more state per thread and uses 2 so
-// three of its workgroups fit one CU.
-constexpr uint32_t kCompressGroups = 4;
-constexpr uint32_t kDecompressGroups = 4;
-constexpr uint32_t kHomsumGroups = 2;
-
-// LDS stage area of the operand-reading kernels (bytes).  Sized so three
-// 576-thread workgroups fit one CU's 160 KB of LDS with 8 operands' tables.
-constexpr uint32_t kStageMin = 44 * 1024;
+// LDS stage area of the homomorphic sum (bytes): 8 operands' tile bytes at
+// typical compression ratios fit in one pass.
+constexpr uint32_t kStageMin = 40 * 1024;
 
 struct Operand {
     const uint8_t* bytes;
@@ -126,7 +124,7 @@ struct ParseArgs {
 };
 
 size_t compress_lds_bytes(const Tiling& tl);
-uint32_t stage_bytes_for(const Tiling& tl);
+uint32_t stage_bytes_for(const Tiling& tl, int k);
 size_t operand_lds_bytes(const Tiling& tl, int k, uint32_t stage);
 
 hipError_t launch_compress(const CompressArgs& a, hipStream_t s);

@@ -323,29 +323,30 @@ __device__ __forceinline__ uint4 lds_ld16(const uint8_t* p)
 }
 
 // Add the up-to-4 deltas of unit u of a block (width byte at bp, width w > 0,
-// nj elements) to acc[].  Magnitude = (high bytes << r) | low r bits; the
-// sign bit s (MSB-first bitmap) is applied as (m ^ -s) + s.
+// nj elements, sb = ceil(nj/8) sign bytes) to acc[].  Magnitude = (high
+// bytes << r) | low r bits; the sign bit s (MSB-first bitmap) is applied as
+// (m ^ -s) + s.  Reads reach at most 16 bytes past the block's last byte.
 // ZCCL/src/ZCCL_TypeManager.c:423-517 (extract) and :906-950 (sign bitmap).
-__device__ __forceinline__ void accumulate_unit(const uint8_t* bp, uint32_t w, uint32_t nj, uint32_t u,
-    uint32_t acc[4])
+__device__ __forceinline__ void accumulate_unit(const uint8_t* bp, uint32_t w, uint32_t nj, uint32_t sb,
+    uint32_t u, uint32_t acc[4])
 {
-    const uint32_t B = w >> 3, r = w & 7, sb = (nj + 7) >> 3;
-    const uint32_t nib = (uint32_t) (bp[1 + (u >> 1)] >> ((~u & 1u) << 2)) & 15u;
+    const uint32_t B = w >> 3, r = w & 7;
+    const uint32_t nib = __builtin_amdgcn_ubfe((uint32_t) bp[1 + (u >> 1)], (~u & 1u) << 2, 4);
     const uint8_t* hp = bp + 1 + sb;
     uint32_t m[4];
-    if (B == 0) {
-        m[0] = m[1] = m[2] = m[3] = 0;
-    } else if (B == 1) {
-        uint32_t H = lds_ld4(hp + 4 * u);
+    if (B <= 1) {
+        // widths below 16, the common case: one 4-byte read; B == 0 reads
+        // the low-bit bytes and masks them to nothing
+        const uint32_t H = lds_ld4(hp + 4 * u * B);
 #pragma unroll
         for (int c = 0; c < 4; c++)
-            m[c] = __builtin_amdgcn_ubfe(H, 8 * c, 8) << r;
+            m[c] = __builtin_amdgcn_ubfe(H, 8 * c, 8 * B);
     } else if (B == 2) {
         uint2 H = lds_ld8(hp + 8 * u);
-        m[0] = (H.x & 0xFFFFu) << r;
-        m[1] = (H.x >> 16) << r;
-        m[2] = (H.y & 0xFFFFu) << r;
-        m[3] = (H.y >> 16) << r;
+        m[0] = H.x & 0xFFFFu;
+        m[1] = H.x >> 16;
+        m[2] = H.y & 0xFFFFu;
+        m[3] = H.y >> 16;
     } else {
         uint4 H = lds_ld16(hp + 4 * u * B);
         if (B == 3) {
@@ -356,21 +357,15 @@ __device__ __forceinline__ void accumulate_unit(const uint8_t* bp, uint32_t w, u
         } else {
             m[0] = H.x; m[1] = H.y; m[2] = H.z; m[3] = H.w;
         }
-#pragma unroll
-        for (int c = 0; c < 4; c++)
-            m[c] <<= r;
     }
-    if (r) {
-        const uint32_t bitpos = 4 * u * r;
-        const uint32_t X = __builtin_bswap32(lds_ld4(hp + B * nj + (bitpos >> 3))) << (bitpos & 7);
-#pragma unroll
-        for (int c = 0; c < 4; c++)
-            m[c] |= __builtin_amdgcn_ubfe(X, 32 - r * (c + 1), r);
-    }
+    // low bits for every width (r == 0 extracts zero-width fields)
+    const uint32_t bitpos = 4 * u * r;
+    const uint32_t X = __builtin_bswap32(lds_ld4(hp + B * nj + (bitpos >> 3))) << (bitpos & 7);
 #pragma unroll
     for (int c = 0; c < 4; c++) {
+        const uint32_t mc = (m[c] << r) | __builtin_amdgcn_ubfe(X, 32 - r * (c + 1), r);
         const uint32_t s = (uint32_t) __builtin_amdgcn_sbfe((int32_t) nib, 3 - c, 1);  // 0 or ~0
-        acc[c] += (m[c] ^ s) - s;
+        acc[c] += (mc ^ s) - s;
     }
 }
 
@@ -570,16 +565,17 @@ __device__ __forceinline__ void encode_tile(const Geom& g, const Tiling& tl, con
         if (j < x.nblk)
             sh.boff[j] = base + loc[c];
     }
+    uint32_t nwords = (total + 3) / 4 + 8;
+    for (uint32_t i = tid; i < nwords; i += blockDim.x)
+        buf[i] = 0;
+    __syncthreads();
+    // wave 0 resolves the tile's stream offset while the others encode; the
+    // encoding needs only offsets inside the tile
     if (tid < 64) {
         unsigned long long ex = lookback(status, t, 0, total, 0, ws);
         if (tid == 0)
             sh.bc[2] = ex;
     }
-    uint32_t nwords = (total + 3) / 4 + 8;
-    for (uint32_t i = tid; i < nwords; i += blockDim.x)
-        buf[i] = 0;
-    __syncthreads();
-    const uint64_t P = 8ull * g.T + 4ull * (x.s + 1) + sh.bc[2];
 #pragma unroll
     for (uint32_t i = 0; i < kUnitsPerThread; i++) {
         const uint32_t pk = un.pk[i], cnt = unit_cnt(pk);
@@ -593,6 +589,7 @@ __device__ __forceinline__ void encode_tile(const Geom& g, const Tiling& tl, con
     for (uint32_t j = tid; j < x.nblk; j += blockDim.x)
         lds_or4(buf, sh.boff[j], sh.width[j]);
     __syncthreads();
+    const uint64_t P = 8ull * g.T + 4ull * (x.s + 1) + sh.bc[2];
     const bool fits = P + total <= out_cap;
     if (fits)
         store_tile(buf, out + P, total);
@@ -687,10 +684,11 @@ struct OperandShared {
 };
 
 // Read the k operands' widths and offsets for tile x; per-operand scans give
-// each block's byte offset inside its operand's tile bytes.  Tables: widths
-// s_w[k][G] (u8) and offsets s_off[k][G] (u16).
+// each block's byte offset inside its operand's tile bytes.  Table s_wo[k][G]
+// holds offset | width << 16 (a width above 32 raises ZK_ERR_WIDTH and reads
+// as 0).
 __device__ __forceinline__ void operand_tables(const Geom& g, const Tiling& tl, const Tile& x, int k, const Operand* in,
-    OperandShared& os, uint8_t* s_w, uint16_t* s_off, WsHeader* ws)
+    OperandShared& os, uint32_t* s_wo, WsHeader* ws)
 {
     const uint32_t tid = threadIdx.x, G = tl.G;
     if (tid < (uint32_t) k) {
@@ -709,16 +707,17 @@ __device__ __forceinline__ void operand_tables(const Geom& g, const Tiling& tl, 
                 w = 0;
             }
         }
-        s_w[idx] = (uint8_t) w;
+        s_wo[idx] = w;
     }
     __syncthreads();
     const int lane = tid & 63, wid = tid >> 6, nw = blockDim.x >> 6;
     for (int r = wid; r < k; r += nw) {
-        uint32_t sz[4], run = 0;
+        uint32_t sz[4], wj[4], run = 0;
 #pragma unroll
         for (int c = 0; c < 4; c++) {
             uint32_t j = 4 * lane + c;
-            sz[c] = j < x.nblk ? block_bytes(s_w[r * G + j], tile_block_len(g, x, j)) : 0u;
+            wj[c] = j < G ? s_wo[r * G + j] : 0u;
+            sz[c] = j < x.nblk ? block_bytes(wj[c], tile_block_len(g, x, j)) : 0u;
             run += sz[c];
         }
         uint32_t incl = wave_incl_scan(run);
@@ -727,7 +726,7 @@ __device__ __forceinline__ void operand_tables(const Geom& g, const Tiling& tl, 
         for (int c = 0; c < 4; c++) {
             uint32_t j = 4 * lane + c;
             if (j < G)
-                s_off[r * G + j] = (uint16_t) off;
+                s_wo[r * G + j] = off | (wj[c] << 16);
             off += sz[c];
         }
         if (lane == 63) {
@@ -739,6 +738,22 @@ __device__ __forceinline__ void operand_tables(const Geom& g, const Tiling& tl, 
         }
     }
     __syncthreads();
+}
+
+// Every staged block's width byte must equal the index width.
+__device__ __forceinline__ void check_width_bytes(const uint8_t* sbytes, const OperandShared& os, const uint32_t* s_wo,
+    uint32_t G, uint32_t nblk, int r0, int r1, WsHeader* ws)
+{
+    bool bad = false;
+    for (uint32_t idx = threadIdx.x; idx < (uint32_t) (r1 - r0) * G; idx += blockDim.x) {
+        const uint32_t r = r0 + idx / G, j = idx % G;
+        if (j < nblk && !os.bad[r]) {
+            const uint32_t e = s_wo[r * G + j];
+            bad |= sbytes[os.lds[r] + (e & 0xFFFFu)] != (e >> 16);
+        }
+    }
+    if (bad)
+        raise_error(ws, ZK_ERR_MISMATCH);
 }
 
 // Group operands [r0, r) so their tile bytes (plus 16 bytes of read slack
@@ -761,44 +776,55 @@ __device__ __forceinline__ int plan_group(OperandShared& os, int r0, int k, uint
 }
 
 // Load operands [r0, r1) of a group into LDS: LDS byte os.lds[r] + i holds
-// stream byte os.base[r] + i.  Wave w loads operands r0+w, r0+w+nwaves, ...,
-// issuing a batch of 16-byte loads before writing any of them, so a group
-// costs about one memory round trip; only bytes inside each stream are read.
+// stream byte os.base[r] + i.  The workgroup's threads are split evenly over
+// the group's operands; each thread issues its operand's 16-byte chunks in
+// batches before writing any of them, so a group costs about one memory round
+// trip.  Only the operand's last chunk can cross the stream's end; it is read
+// byte by byte when it does, so no byte past the stream is read.
 constexpr int kLoadBatch = 8;
 
 __device__ __forceinline__ void load_group(const Operand* in, const OperandShared& os, int r0, int r1, uint32_t* stage)
 {
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    for (int r = r0 + wid; r < r1; r += nw) {
-        if (os.bad[r])
-            continue;
-        const uint32_t nch = (os.tot[r] + 15) / 16;
-        const uint64_t base = os.base[r], size = os.size[r];
-        const uint8_t* src = in[r].bytes + base;
-        uint4* dst = (uint4*) ((uint8_t*) stage + os.lds[r]);
-        for (uint32_t c0 = lane; c0 < nch; c0 += kLoadBatch * 64) {
-            uint4 v[kLoadBatch];
+    const uint32_t ng = (uint32_t) (r1 - r0);
+    const uint32_t tpo = blockDim.x / ng;  // threads per operand (blockDim >= 64 >= ng)
+    const uint32_t r = r0 + threadIdx.x / tpo;
+    const uint32_t q = threadIdx.x % tpo;
+    if (r >= (uint32_t) r1 || os.bad[r])
+        return;
+    const uint32_t nch = (os.tot[r] + 15) / 16;
+    if (!nch)
+        return;
+    const uint64_t base = os.base[r], size = os.size[r];
+    const uint8_t* src = in[r].bytes + base;
+    uint4* dst = (uint4*) ((uint8_t*) stage + os.lds[r]);
+    const uint32_t nfast = nch - 1;  // chunks that end inside the stream's tile bytes
+    for (uint32_t c0 = q; c0 < nfast; c0 += kLoadBatch * tpo) {
+        uint4 v[kLoadBatch];
 #pragma unroll
-            for (int b = 0; b < kLoadBatch; b++) {
-                const uint32_t c = c0 + 64 * b;
-                v[b] = make_uint4(0, 0, 0, 0);
-                if (c < nch) {
-                    const uint64_t off = base + 16ull * c;
-                    if (off + 16 <= size) {
-                        v[b] = ld16(src + 16ull * c);
-                    } else {
-                        uint32_t t[4] = {0, 0, 0, 0};
-                        for (uint32_t q = 0; q < 16 && off + q < size; q++)
-                            t[q >> 2] |= (uint32_t) src[16ull * c + q] << (8 * (q & 3));
-                        v[b] = make_uint4(t[0], t[1], t[2], t[3]);
-                    }
-                }
-            }
-#pragma unroll
-            for (int b = 0; b < kLoadBatch; b++)
-                if (c0 + 64 * b < nch)
-                    dst[c0 + 64 * b] = v[b];
+        for (int b = 0; b < kLoadBatch; b++) {
+            const uint32_t c = c0 + b * tpo;
+            if (c < nfast)
+                v[b] = ld16(src + 16ull * c);
         }
+#pragma unroll
+        for (int b = 0; b < kLoadBatch; b++) {
+            const uint32_t c = c0 + b * tpo;
+            if (c < nfast)
+                dst[c] = v[b];
+        }
+    }
+    if (q == (nfast % tpo)) {
+        const uint64_t off = base + 16ull * nfast;
+        uint4 v;
+        if (off + 16 <= size) {
+            v = ld16(src + 16ull * nfast);
+        } else {
+            uint32_t t[4] = {0, 0, 0, 0};
+            for (uint32_t i = 0; i < 16 && off + i < size; i++)
+                t[i >> 2] |= (uint32_t) src[16ull * nfast + i] << (8 * (i & 3));
+            v = make_uint4(t[0], t[1], t[2], t[3]);
+        }
+        dst[nfast] = v;
     }
 }
 
@@ -813,12 +839,17 @@ __device__ __forceinline__ void homsum_tile(const Geom& g, const Tiling& tl, con
     const Tile x = tile_of(g, tl, t);
     const uint32_t tid = threadIdx.x;
     uint32_t* stage = zk_lds;
-    uint8_t* s_w = (uint8_t*) (zk_lds + p.stage_bytes / 4);
-    uint16_t* s_off = (uint16_t*) (s_w + align_up((uint64_t) p.k * tl.G, 16));
+    uint32_t* s_wo = zk_lds + p.stage_bytes / 4;
     const Units un = tile_units(g, x);
     const uint32_t last_j = x.nblk ? x.nblk - 1 : 0u;
     const uint32_t last_len = x.nblk ? tile_block_len(g, x, last_j) : 0u;
-    operand_tables(g, tl, x, p.k, io.in, os, s_w, s_off, ws);
+    uint32_t nj[kUnitsPerThread], sbj[kUnitsPerThread];
+#pragma unroll
+    for (uint32_t i = 0; i < kUnitsPerThread; i++) {
+        nj[i] = unit_j(un.pk[i]) == last_j ? last_len : g.L;
+        sbj[i] = (nj[i] + 7) >> 3;
+    }
+    operand_tables(g, tl, x, p.k, io.in, os, s_wo, ws);
     uint32_t acc[kUnitsPerThread][4];
 #pragma unroll
     for (uint32_t i = 0; i < kUnitsPerThread; i++)
@@ -833,24 +864,18 @@ __device__ __forceinline__ void homsum_tile(const Geom& g, const Tiling& tl, con
         load_group(io.in, os, r0, r1, stage);
         __syncthreads();
         const uint8_t* sbytes = (const uint8_t*) stage;
-#pragma unroll
-        for (uint32_t i = 0; i < kUnitsPerThread; i++) {
-            const uint32_t pk = un.pk[i];
-            if (!unit_cnt(pk))
+        check_width_bytes(sbytes, os, s_wo, tl.G, x.nblk, r0, r1, ws);
+        for (int r = r0; r < r1; r++) {
+            if (__builtin_amdgcn_readfirstlane(os.bad[r]))
                 continue;
-            const uint32_t j = unit_j(pk), u = unit_u(pk);
-            const uint32_t nj = j == last_j ? last_len : g.L;
-            bool bad_width = false;
-            for (int r = r0; r < r1; r++) {
-                const uint32_t w = s_w[r * tl.G + j];
-                if (!w || os.bad[r])
-                    continue;
-                const uint8_t* bp = sbytes + os.lds[r] + s_off[r * tl.G + j];
-                accumulate_unit(bp, w, nj, u, acc[i]);
-                bad_width |= (u == 0 && bp[0] != w);
+            const uint8_t* ob = sbytes + __builtin_amdgcn_readfirstlane(os.lds[r]);
+            const uint32_t* tab = s_wo + r * tl.G;
+#pragma unroll
+            for (uint32_t i = 0; i < kUnitsPerThread; i++) {
+                const uint32_t e = tab[unit_j(un.pk[i])];
+                if (e >> 16)
+                    accumulate_unit(ob + (e & 0xFFFFu), e >> 16, nj[i], sbj[i], unit_u(un.pk[i]), acc[i]);
             }
-            if (bad_width)
-                raise_error(ws, ZK_ERR_MISMATCH);
         }
         __syncthreads();
         r0 = r1;
@@ -890,10 +915,9 @@ __device__ __forceinline__ void decompress_tile(const Geom& g, const Tiling& tl,
     const Tile x = tile_of(g, tl, t);
     const uint32_t tid = threadIdx.x;
     uint32_t* stage = zk_lds;
-    uint8_t* s_w = (uint8_t*) (zk_lds + p.stage_bytes / 4);
-    uint16_t* s_off = (uint16_t*) (s_w + align_up(tl.G, 16));
+    uint32_t* s_wo = zk_lds + p.stage_bytes / 4;
     const Units un = tile_units(g, x);
-    operand_tables(g, tl, x, 1, io.in, os, s_w, s_off, ws);
+    operand_tables(g, tl, x, 1, io.in, os, s_wo, ws);
     if (tid == 0)
         plan_group(os, 0, 1, p.stage_bytes);
     __syncthreads();
@@ -913,19 +937,19 @@ __device__ __forceinline__ void decompress_tile(const Geom& g, const Tiling& tl,
     uint32_t pre[kUnitsPerThread][4], us[kUnitsPerThread];
     const uint32_t last_j = x.nblk ? x.nblk - 1 : 0u;
     const uint32_t last_len = x.nblk ? tile_block_len(g, x, last_j) : 0u;
+    check_width_bytes((const uint8_t*) stage, os, s_wo, tl.G, x.nblk, 0, 1, ws);
     const uint8_t* base = (const uint8_t*) stage + os.lds[0];
+    const bool bad0 = __builtin_amdgcn_readfirstlane(os.bad[0]);
 #pragma unroll
     for (uint32_t i = 0; i < kUnitsPerThread; i++) {
         uint32_t d[4] = {0, 0, 0, 0};
         const uint32_t pk = un.pk[i], cnt = unit_cnt(pk);
         const uint32_t j = unit_j(pk);
-        const uint32_t w = s_w[j];
-        if (cnt && w && !os.bad[0]) {
-            const uint32_t u = unit_u(pk);
-            const uint8_t* bp = base + s_off[j];
-            accumulate_unit(bp, w, j == last_j ? last_len : g.L, u, d);
-            if (u == 0 && bp[0] != w)
-                raise_error(ws, ZK_ERR_MISMATCH);
+        if (cnt && !bad0) {
+            const uint32_t e = s_wo[j];
+            const uint32_t nj = j == last_j ? last_len : g.L;
+            if (e >> 16)
+                accumulate_unit(base + (e & 0xFFFFu), e >> 16, nj, (nj + 7) >> 3, unit_u(pk), d);
         }
         uint32_t run = 0;
 #pragma unroll
@@ -995,7 +1019,7 @@ __device__ __forceinline__ void decompress_tile(const Geom& g, const Tiling& tl,
 
 // ------------------------------------------------------------ kernels
 
-__global__ void __launch_bounds__(kMaxTileThreads, 6) k_compress(CompressArgs a)
+__global__ void __launch_bounds__(kCompressThreads) k_compress(CompressArgs a)
 {
     const uint64_t t = blockIdx.x;
     unsigned long long* status;
@@ -1004,7 +1028,7 @@ __global__ void __launch_bounds__(kMaxTileThreads, 6) k_compress(CompressArgs a)
     compress_tile(a.job.g, a.job.tl, a.job.p, a.job.io, t, status, a.ws, 1);
 }
 
-__global__ void __launch_bounds__(kMaxTileThreads, 4) k_homsum(HomsumArgs a)
+__global__ void __launch_bounds__(kHomsumThreads, 4) k_homsum(HomsumArgs a)
 {
     const uint64_t t = blockIdx.x;
     unsigned long long* status;
@@ -1013,7 +1037,7 @@ __global__ void __launch_bounds__(kMaxTileThreads, 4) k_homsum(HomsumArgs a)
     homsum_tile(a.job.g, a.job.tl, a.job.p, a.job.io, t, status, a.ws, 1);
 }
 
-__global__ void __launch_bounds__(kMaxTileThreads, 6) k_decompress(DecompressArgs a)
+__global__ void __launch_bounds__(kDecompressThreads) k_decompress(DecompressArgs a)
 {
     const uint64_t t = blockIdx.x;
     unsigned long long* status;
@@ -1061,7 +1085,7 @@ __device__ __forceinline__ void stage_desc(const void* base, uint32_t jidx, D* d
 }
 
 // Uniform batches: geometry in kernel arguments, IO blocks per job.
-__global__ void __launch_bounds__(kMaxTileThreads, 4) k_homsum_batch(BatchArgs b)
+__global__ void __launch_bounds__(kHomsumThreads, 4) k_homsum_batch(BatchArgs b)
 {
     __shared__ HomsumIO sio;
     unsigned long long* status;
@@ -1074,7 +1098,7 @@ __global__ void __launch_bounds__(kMaxTileThreads, 4) k_homsum_batch(BatchArgs b
     homsum_tile(b.g, b.tl, b.p, sio, t, status + b.job_tile0[jidx] * kStatusStride, b.ws, b.total_chains);
 }
 
-__global__ void __launch_bounds__(kMaxTileThreads, 6) k_compress_batch(BatchArgs b)
+__global__ void __launch_bounds__(kCompressThreads) k_compress_batch(BatchArgs b)
 {
     __shared__ CompressIO sio;
     unsigned long long* status;
@@ -1087,7 +1111,7 @@ __global__ void __launch_bounds__(kMaxTileThreads, 6) k_compress_batch(BatchArgs
     compress_tile(b.g, b.tl, b.p, sio, t, status + b.job_tile0[jidx] * kStatusStride, b.ws, b.total_chains);
 }
 
-__global__ void __launch_bounds__(kMaxTileThreads, 6) k_decompress_batch(BatchArgs b)
+__global__ void __launch_bounds__(kDecompressThreads) k_decompress_batch(BatchArgs b)
 {
     __shared__ DecompressIO sio;
     unsigned long long* status;
@@ -1101,7 +1125,7 @@ __global__ void __launch_bounds__(kMaxTileThreads, 6) k_decompress_batch(BatchAr
 }
 
 // Mixed-geometry batches: whole descriptors per job.
-__global__ void __launch_bounds__(kMaxTileThreads, 4) k_homsum_mixed(BatchArgs b)
+__global__ void __launch_bounds__(kHomsumThreads, 4) k_homsum_mixed(BatchArgs b)
 {
     __shared__ HomsumJob sj;
     unsigned long long* status;
@@ -1114,7 +1138,7 @@ __global__ void __launch_bounds__(kMaxTileThreads, 4) k_homsum_mixed(BatchArgs b
     homsum_tile(sj.g, sj.tl, sj.p, sj.io, t, status + b.job_tile0[jidx] * kStatusStride, b.ws, b.total_chains);
 }
 
-__global__ void __launch_bounds__(kMaxTileThreads, 6) k_compress_mixed(BatchArgs b)
+__global__ void __launch_bounds__(kCompressThreads) k_compress_mixed(BatchArgs b)
 {
     __shared__ CompressJob sj;
     unsigned long long* status;
@@ -1127,7 +1151,7 @@ __global__ void __launch_bounds__(kMaxTileThreads, 6) k_compress_mixed(BatchArgs
     compress_tile(sj.g, sj.tl, sj.p, sj.io, t, status + b.job_tile0[jidx] * kStatusStride, b.ws, b.total_chains);
 }
 
-__global__ void __launch_bounds__(kMaxTileThreads, 6) k_decompress_mixed(BatchArgs b)
+__global__ void __launch_bounds__(kDecompressThreads) k_decompress_mixed(BatchArgs b)
 {
     __shared__ DecompressJob sj;
     unsigned long long* status;
@@ -1289,15 +1313,17 @@ __global__ void k_ws_init(WsHeader* ws, uint64_t capacity)
 
 size_t compress_lds_bytes(const Tiling& tl) { return (size_t) align_up(tl.tile_bytes_max + 64, 16); }
 
-uint32_t stage_bytes_for(const Tiling& tl)
+// One operand always fits (worst case); k > 1 operands share at least
+// kStageMin bytes and are staged in groups when their tiles do not fit.
+uint32_t stage_bytes_for(const Tiling& tl, int k)
 {
     uint64_t need = align_up(tl.tile_bytes_max + 64, 16);
-    return (uint32_t) (need > kStageMin ? need : kStageMin);
+    return (uint32_t) (k > 1 && need < kStageMin ? kStageMin : need);
 }
 
 size_t operand_lds_bytes(const Tiling& tl, int k, uint32_t stage)
 {
-    return (size_t) (stage + align_up((uint64_t) k * tl.G, 16) + align_up(2ull * k * tl.G, 16));
+    return (size_t) (stage + 4ull * k * tl.G);
 }
 
 hipError_t launch_compress(const CompressArgs& a, hipStream_t s)
