@@ -53,7 +53,7 @@ def parse_args():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="bound on the CPU baseline sample")
     ap.add_argument("--copy-calibration", action="store_true",
-                    help="only run a plain 256 MiB device copy (rocprof counter calibration)")
+                    help="only run a 256 MiB elementwise copy kernel (rocprof counter calibration)")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "homsum_traffic.json"))
     return ap.parse_args()
 
@@ -132,11 +132,14 @@ def main():
 
     torch.cuda.set_device(0)
     if args.copy_calibration:
+        # elementwise kernel reading 256 MiB and writing 256 MiB per launch
         a = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
         for _ in range(3):
-            a.clone()
+            torch.mul(a, 2.0, out=b)
         torch.cuda.synchronize()
-        print(json.dumps({"copy_bytes_per_launch": a.numel() * 4}))
+        print(json.dumps({"copy_read_bytes_per_launch": a.numel() * 4,
+                          "copy_write_bytes_per_launch": b.numel() * 4}))
         return
 
     P, side, L, T = args.ranks, args.side, args.L, args.T

@@ -44,7 +44,7 @@ constexpr uint32_t kTileUnits = kMaxTileThreads * kUnitsPerThread;  // 4-element
 constexpr uint32_t kGroupUnits = 576;  // units per index group (at most)
 constexpr uint32_t kMaxGroupBlocks = 16;
 constexpr uint32_t kMaxTileBlocks = 256;
-constexpr uint32_t kStatusStride = 16; // u64 words between lookback status entries (one 128 B line each)
+constexpr uint32_t kStatusStride = 1;  // u64 words between lookback status entries (packed: a 64-wide lookback window reads 512 B)
 
 // Element, block, group and tile counts are 32-bit: streams hold fewer than
 // 2^31 values (the reference indexes them with int, ZCCL/src/ZCCL_float.c:553-554).
