@@ -183,6 +183,34 @@ size_t zccl_batch_tiles(const zccl_batch *batch);
 int zccl_batch_launch(const zccl_batch *batch, void *workspace, size_t workspace_bytes, void *stream);
 void zccl_batch_destroy(zccl_batch *batch);
 
+/* Host-only plans: validate and pack the descriptors without any HIP call.
+ * The caller copies zccl_batch_desc_bytes() bytes from zccl_batch_desc_host()
+ * to 16-byte aligned device memory (any stream-ordered copy) and binds that
+ * address before launching; destroy() then frees host memory only.  The
+ * homomorphic-sum plan takes a per-job operand count ks[j]; ins / indices
+ * hold job 0's ks[0] operands, then job 1's, and so on. */
+zccl_batch *zccl_batch_homsum_plan(int njobs, const int *ks, const size_t *nbEle, int blockSize,
+    int nchunks, const unsigned char *const *ins, const void *const *indices,
+    unsigned char *const *outs, const size_t *out_caps, void *const *out_indices, int *status);
+zccl_batch *zccl_batch_compress_plan(int njobs, const size_t *nbEle, const float *absErrBounds,
+    int blockSize, int nchunks, const float *const *xs, unsigned char *const *outs,
+    const size_t *out_caps, void *const *out_indices, int *status);
+zccl_batch *zccl_batch_decompress_plan(int njobs, const size_t *nbEle, const float *absErrBounds,
+    int blockSize, int nchunks, const unsigned char *const *ins, const void *const *indices,
+    float *const *outs, int *status);
+size_t zccl_batch_desc_bytes(const zccl_batch *batch);
+const void *zccl_batch_desc_host(const zccl_batch *batch);
+int zccl_batch_bind(zccl_batch *batch, void *device_desc);
+
+/* ------------------------------------------------------------ tuning
+ *
+ * Workgroup size target (threads, a multiple of 64) of operation op: 0
+ * compress (at most 512), 1 homomorphic sum (at most 256), 2 decompress (at
+ * most 512); 0 restores the default (the maximum).  Affects launches and
+ * plans created afterwards, and zccl_num_tiles(); never the stream bytes. */
+int zccl_tune_threads(int op, int threads);
+int zccl_tune_get_threads(int op);
+
 /* Block-case histogram of two indexed streams (see the _record add). */
 int zccl_case_count_dev(const void *index1, const void *index2, size_t nbEle, int blockSize,
     int nchunks, unsigned long long *d_counts4, void *stream);

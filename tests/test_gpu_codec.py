@@ -306,3 +306,135 @@ def test_batch_plans_uniform_and_mixed(codec):
             ch = O.homsum([cx, cy], n, L, T)
             assert np.array_equal(sums[i].to_host(), ch), (sizes, i)
             assert np.array_equal(bits(decs[i].cpu().numpy()), bits(O.decompress(ch, n, ebs[i], L, T)))
+
+
+def test_batch_homsum_ragged_operand_counts(codec):
+    """One launch with jobs of 1 to 32 operands and different sizes."""
+    from zccl_amd import batch
+
+    L, T, eb = 36, 8, 1e-3
+    shapes = [(50_000, 1), (123_457, 3), (8, 8), (77_777, 32), (1000, 5), (262_144, 17)]
+    say("ragged", shapes)
+    jobs, outs, want = [], [], []
+    for j, (n, k) in enumerate(shapes):
+        xs = [gen.make("grf", n, 300 + 40 * j + r) for r in range(k)]
+        cs = [O.compress(x, eb, L, T) for x in xs]
+        jobs.append([codec.upload(c, n, L, T) for c in cs])
+        outs.append(codec.empty_stream(n, L, T))
+        want.append(O.homsum(cs, n, L, T))
+    plan = batch.homsum_plan(jobs, outs)
+    ws = codec.Workspace(plan.tiles)
+    plan.launch(ws)
+    plan.launch(ws)
+    torch.cuda.synchronize()
+    codec.check_errors(ws)
+    for j, (n, k) in enumerate(shapes):
+        assert np.array_equal(outs[j].to_host(), want[j]), shapes[j]
+
+
+def test_batch_owning_create_entry_points(codec):
+    """The allocating create() forms give the same bytes as host-built plans."""
+    from zccl_amd._lib import lib, check
+
+    L, T, eb = 36, 8, 2e-3
+    ns = [100_003, 65_536, 4_097]
+    xs = [gen.make("sines", n, 500 + i) for i, n in enumerate(ns)]
+    dxs = [dev(x) for x in xs]
+    outs = [codec.empty_stream(n, L, T) for n in ns]
+    P = lambda ts: (ctypes.c_void_p * len(ts))(*[t.data_ptr() for t in ts])  # noqa: E731
+    nn = (ctypes.c_size_t * 3)(*ns)
+    ebs = (ctypes.c_float * 3)(eb, eb, eb)
+    st = ctypes.c_int(0)
+    h = lib().zccl_batch_compress_create(3, nn, ebs, L, T, P(dxs), P([o.data for o in outs]),
+                                         (ctypes.c_size_t * 3)(*[o.data.numel() for o in outs]),
+                                         P([o.index for o in outs]), ctypes.byref(st))
+    check(st.value, "compress create")
+    sums = [codec.empty_stream(n, L, T) for n in ns]
+    h2 = lib().zccl_batch_homsum_create(3, 2, nn, L, T, P([t for o in outs for t in (o.data, o.data)]),
+                                        P([t for o in outs for t in (o.index, o.index)]),
+                                        P([o.data for o in sums]),
+                                        (ctypes.c_size_t * 3)(*[o.data.numel() for o in sums]),
+                                        P([o.index for o in sums]), ctypes.byref(st))
+    check(st.value, "homsum create")
+    ws = codec.Workspace(max(int(lib().zccl_batch_tiles(h)), int(lib().zccl_batch_tiles(h2))))
+    check(lib().zccl_batch_launch(h, ws.ptr, ws.nbytes, codec._stream_handle()), "launch")
+    check(lib().zccl_batch_launch(h2, ws.ptr, ws.nbytes, codec._stream_handle()), "launch")
+    torch.cuda.synchronize()
+    codec.check_errors(ws)
+    lib().zccl_batch_destroy(h)
+    lib().zccl_batch_destroy(h2)
+    for i, n in enumerate(ns):
+        c = O.compress(xs[i], eb, L, T)
+        assert np.array_equal(outs[i].to_host(), c)
+        assert np.array_equal(sums[i].to_host(), O.homsum([c, c], n, L, T))
+
+
+def test_graph_capture_replays_match_eager(codec):
+    """Batch launches captured in a HIP graph: every replay, with eager work
+    and synchronisation between replays, equals the eager result."""
+    from zccl_amd import batch
+
+    L, T = 36, 16
+    ns = [300_000, 1_000, 65_537, 2_000_000]
+    xs = [dev(gen.make("grf", n, 700 + i)) for i, n in enumerate(ns)]
+    ebs = [1e-3] * len(ns)
+    outs = [codec.empty_stream(n, L, T) for n in ns]
+    sums = [codec.empty_stream(n, L, T) for n in ns]
+    cp = batch.compress_plan(xs, ebs, outs)
+    hp = batch.homsum_plan([[o, o, o] for o in outs], sums)
+    ws = codec.Workspace(max(cp.tiles, hp.tiles))
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        cp.launch(ws)
+        hp.launch(ws)
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    codec.check_errors(ws)
+    eager = [(o.data[: o.size].clone(), m.data[: m.size].clone()) for o, m in zip(outs, sums)]
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        cp.launch(ws)
+        hp.launch(ws)
+    for rep in range(4):
+        for o, m in zip(outs, sums):
+            o.data.zero_()
+            m.data.zero_()
+        g.replay()
+        torch.cuda.synchronize()
+        for o, m in zip(outs, sums):
+            o.invalidate()
+            m.invalidate()
+        codec.check_errors(ws)
+        for (e1, e2), o, m in zip(eager, outs, sums):
+            assert torch.equal(o.data[: o.size], e1), rep
+            assert torch.equal(m.data[: m.size], e2), rep
+        _ = (xs[0] * 2).sum().item()  # eager work between replays
+
+
+def test_tuned_workgroup_sizes_give_same_bytes(codec):
+    from zccl_amd._lib import lib
+
+    L, T, n, eb = 36, 8, 333_333, 1e-3
+    x = gen.make("grf", n, 900)
+    y = gen.make("grf", n, 901)
+    want_c = O.compress(x, eb, L, T)
+    want_h = O.homsum([want_c, O.compress(y, eb, L, T)], n, L, T)
+    try:
+        for th in ((64, 64, 64), (256, 128, 256), (512, 256, 512)):
+            say("threads", th)
+            for op, t in enumerate(th):
+                assert lib().zccl_tune_threads(op, t) == 0
+            a = codec.compress(dev(x), eb, L, T)
+            b = codec.compress(dev(y), eb, L, T)
+            h = codec.homsum([a, b])
+            d = codec.decompress(h, eb)
+            torch.cuda.synchronize()
+            codec.check_errors()
+            assert np.array_equal(a.to_host(), want_c)
+            assert np.array_equal(h.to_host(), want_h)
+            assert np.array_equal(bits(d.cpu().numpy()), bits(O.decompress(want_h, n, eb, L, T)))
+    finally:
+        for op in range(3):
+            lib().zccl_tune_threads(op, 0)
+    assert lib().zccl_tune_threads(1, 512) != 0  # above the sum kernel's launch bound

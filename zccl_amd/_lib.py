@@ -38,6 +38,9 @@ EXPORTS = (
     "zccl_case_count_dev",
     "zccl_batch_homsum_create", "zccl_batch_compress_create", "zccl_batch_decompress_create",
     "zccl_batch_tiles", "zccl_batch_launch", "zccl_batch_destroy",
+    "zccl_batch_homsum_plan", "zccl_batch_compress_plan", "zccl_batch_decompress_plan",
+    "zccl_batch_desc_bytes", "zccl_batch_desc_host", "zccl_batch_bind",
+    "zccl_tune_threads", "zccl_tune_get_threads",
 )
 
 
@@ -102,6 +105,14 @@ def lib() -> ctypes.CDLL:
         "zccl_batch_tiles": ([vp], sz),
         "zccl_batch_launch": ([vp, vp, sz, vp], i32),
         "zccl_batch_destroy": ([vp], None),
+        "zccl_batch_homsum_plan": ([i32, i32_p, vp, i32, i32, vp, vp, vp, vp, vp, i32_p], vp),
+        "zccl_batch_compress_plan": ([i32, vp, vp, i32, i32, vp, vp, vp, vp, i32_p], vp),
+        "zccl_batch_decompress_plan": ([i32, vp, vp, i32, i32, vp, vp, vp, i32_p], vp),
+        "zccl_batch_desc_bytes": ([vp], sz),
+        "zccl_batch_desc_host": ([vp], vp),
+        "zccl_batch_bind": ([vp, vp], i32),
+        "zccl_tune_threads": ([i32, i32], i32),
+        "zccl_tune_get_threads": ([i32], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

@@ -1,10 +1,13 @@
 """Batch plans: many codec operations in one launch (zccl_batch_* in
 include/zccl_amd.h).
 
-A plan is built once (descriptors uploaded to device memory) and launched any
-number of times; each launch is a single kernel on the current stream, so it
-can be replayed inside a captured CUDA/HIP graph.  The plan keeps references to
-the tensors it names so they outlive it.
+A plan is packed on the host by the library (no HIP call), its descriptors are
+copied to a device tensor with a stream-ordered copy from pinned memory, and
+the plan is bound to that tensor.  Building a plan therefore never allocates
+with hipMalloc nor synchronises, so plans can be built while other work runs
+(the serving loop does).  Each launch is a single kernel on the current
+stream, which can be replayed inside a captured HIP graph.  The plan keeps
+references to the tensors it names so they outlive it.
 """
 from __future__ import annotations
 
@@ -21,9 +24,16 @@ from .codec import DeviceStream, Workspace, _stream_handle, default_workspace
 class BatchPlan:
     def __init__(self, handle: int, keep: list, outs: Optional[List[DeviceStream]] = None):
         if not handle:
-            raise _lib.ZcclError(-1, "batch create")
+            raise _lib.ZcclError(-1, "batch plan")
         self.handle = ctypes.c_void_p(handle)
-        self.tiles = int(lib().zccl_batch_tiles(self.handle))
+        L = lib()
+        self.tiles = int(L.zccl_batch_tiles(self.handle))
+        nbytes = int(L.zccl_batch_desc_bytes(self.handle))
+        host = torch.empty(nbytes, dtype=torch.uint8, pin_memory=True)
+        ctypes.memmove(host.data_ptr(), L.zccl_batch_desc_host(self.handle), nbytes)
+        self.desc = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+        self.desc.copy_(host, non_blocking=True)
+        check(L.zccl_batch_bind(self.handle, ctypes.c_void_p(self.desc.data_ptr())), "batch bind")
         self._keep = keep
         self.outs = outs or []
 
@@ -52,16 +62,19 @@ def _ptrs(ts) -> ctypes.Array:
 
 
 def homsum_plan(jobs: Sequence[Sequence[DeviceStream]], outs: Sequence[DeviceStream]) -> BatchPlan:
-    """jobs[j] is the list of k operand streams summed into outs[j]."""
+    """jobs[j] is the list of operand streams (1 to 32) summed into outs[j];
+    jobs may have different operand counts and sizes."""
     nj = len(jobs)
-    k = len(jobs[0])
-    if any(len(j) != k for j in jobs) or len(outs) != nj:
-        raise ValueError("every job needs k operands and one output")
-    L, T = jobs[0][0].L, jobs[0][0].T
+    if len(outs) != nj or nj == 0:
+        raise ValueError("one output per job")
+    L, T = outs[0].L, outs[0].T
     for j, ops in enumerate(jobs):
+        if not 1 <= len(ops) <= 32:
+            raise ValueError("1 to 32 operands per job")
         for s in ops:
             if (s.n, s.L, s.T) != (outs[j].n, L, T):
                 raise ValueError("operands of a job must share (n, L, T) with its output")
+    ks = (ctypes.c_int * nj)(*[len(ops) for ops in jobs])
     n = (ctypes.c_size_t * nj)(*[o.n for o in outs])
     ins = _ptrs([s.data for ops in jobs for s in ops])
     ixs = _ptrs([s.index for ops in jobs for s in ops])
@@ -69,8 +82,8 @@ def homsum_plan(jobs: Sequence[Sequence[DeviceStream]], outs: Sequence[DeviceStr
     ocap = (ctypes.c_size_t * nj)(*[o.data.numel() for o in outs])
     oix = _ptrs([o.index for o in outs])
     st = ctypes.c_int(0)
-    h = lib().zccl_batch_homsum_create(nj, k, n, L, T, ins, ixs, od, ocap, oix, ctypes.byref(st))
-    check(st.value, "batch homsum create")
+    h = lib().zccl_batch_homsum_plan(nj, ks, n, L, T, ins, ixs, od, ocap, oix, ctypes.byref(st))
+    check(st.value, "batch homsum plan")
     return BatchPlan(h, [list(jobs), list(outs)], list(outs))
 
 
@@ -83,10 +96,10 @@ def compress_plan(xs: Sequence[torch.Tensor], ebs: Sequence[float], outs: Sequen
     n = (ctypes.c_size_t * nj)(*[o.n for o in outs])
     eb = (ctypes.c_float * nj)(*ebs)
     st = ctypes.c_int(0)
-    h = lib().zccl_batch_compress_create(nj, n, eb, L, T, _ptrs(xs), _ptrs([o.data for o in outs]),
-                                          (ctypes.c_size_t * nj)(*[o.data.numel() for o in outs]),
-                                          _ptrs([o.index for o in outs]), ctypes.byref(st))
-    check(st.value, "batch compress create")
+    h = lib().zccl_batch_compress_plan(nj, n, eb, L, T, _ptrs(xs), _ptrs([o.data for o in outs]),
+                                        (ctypes.c_size_t * nj)(*[o.data.numel() for o in outs]),
+                                        _ptrs([o.index for o in outs]), ctypes.byref(st))
+    check(st.value, "batch compress plan")
     return BatchPlan(h, [list(xs), list(outs)], list(outs))
 
 
@@ -99,7 +112,7 @@ def decompress_plan(streams: Sequence[DeviceStream], ebs: Sequence[float], outs:
     n = (ctypes.c_size_t * nj)(*[s.n for s in streams])
     eb = (ctypes.c_float * nj)(*ebs)
     st = ctypes.c_int(0)
-    h = lib().zccl_batch_decompress_create(nj, n, eb, L, T, _ptrs([s.data for s in streams]),
-                                            _ptrs([s.index for s in streams]), _ptrs(outs), ctypes.byref(st))
-    check(st.value, "batch decompress create")
+    h = lib().zccl_batch_decompress_plan(nj, n, eb, L, T, _ptrs([s.data for s in streams]),
+                                          _ptrs([s.index for s in streams]), _ptrs(outs), ctypes.byref(st))
+    check(st.value, "batch decompress plan")
     return BatchPlan(h, [list(streams), list(outs)])

@@ -101,6 +101,29 @@ extern "C" int zccl_get_num_chunks(void)
     return o > 0 ? o : omp_team_size();
 }
 
+// ------------------------------------------------------------ tunables
+
+// Workgroup size targets per operation (0 compress, 1 homomorphic sum,
+// 2 decompress); each is capped by the kernel's compiled launch bound.
+static std::atomic<int> g_threads[3] = {{(int) kCompressThreads}, {(int) kHomsumThreads}, {(int) kDecompressThreads}};
+static const int kThreadsMax[3] = {(int) kCompressThreads, (int) kHomsumThreads, (int) kDecompressThreads};
+
+static uint32_t op_threads(int op) { return (uint32_t) g_threads[op].load(); }
+
+extern "C" int zccl_tune_threads(int op, int threads)
+{
+    if (op < 0 || op > 2)
+        return ZCCL_EINVAL;
+    if (threads == 0)
+        threads = kThreadsMax[op];
+    if (threads < 64 || threads > kThreadsMax[op] || threads % 64)
+        return ZCCL_EINVAL;
+    g_threads[op].store(threads);
+    return ZCCL_OK;
+}
+
+extern "C" int zccl_tune_get_threads(int op) { return op < 0 || op > 2 ? 0 : g_threads[op].load(); }
+
 // ------------------------------------------------------------ sizes
 
 static bool geom_of(size_t n, int L, int T, Geom* g)
@@ -127,8 +150,8 @@ extern "C" size_t zccl_num_tiles(size_t n, int L, int T)
     Geom g;
     if (!geom_of(n, L, T, &g))
         return 0;
-    uint64_t a = make_tiling(g, kCompressThreads).ntiles, b = make_tiling(g, kHomsumThreads).ntiles,
-             c = make_tiling(g, kDecompressThreads).ntiles;
+    uint64_t a = make_tiling(g, op_threads(0)).ntiles, b = make_tiling(g, op_threads(1)).ntiles,
+             c = make_tiling(g, op_threads(2)).ntiles;
     return (size_t) std::max(a, std::max(b, c));
 }
 
@@ -213,7 +236,7 @@ extern "C" int zccl_compress_dev(const float* x, size_t n, float eb, int L, int 
         return ZCCL_EINVAL;
     CompressArgs a;
     a.job.g = g;
-    a.job.tl = make_tiling(g, kCompressThreads);
+    a.job.tl = make_tiling(g, op_threads(0));
     int rc = check_ws(ws, ws_bytes, a.job.tl.ntiles);
     if (rc)
         return rc;
@@ -236,7 +259,7 @@ extern "C" int zccl_decompress_dev(const unsigned char* in, const void* index, s
         return ZCCL_EINVAL;
     DecompressArgs a;
     a.job.g = g;
-    a.job.tl = make_tiling(g, kDecompressThreads);
+    a.job.tl = make_tiling(g, op_threads(2));
     int rc = check_ws(ws, ws_bytes, a.job.tl.ntiles);
     if (rc)
         return rc;
@@ -259,7 +282,7 @@ extern "C" int zccl_homsum_dev(const unsigned char* const* ins, const void* cons
     HomsumArgs a;
     memset(&a, 0, sizeof(a));
     a.job.g = g;
-    a.job.tl = make_tiling(g, kHomsumThreads);
+    a.job.tl = make_tiling(g, op_threads(1));
     int rc = check_ws(ws, ws_bytes, a.job.tl.ntiles);
     if (rc)
         return rc;
@@ -280,20 +303,29 @@ extern "C" int zccl_homsum_dev(const unsigned char* const* ins, const void* cons
 }
 
 // ------------------------------------------------------------ batch plans
+//
+// A plan is built on the host: jobs are validated and their device
+// descriptors packed into plan-owned host memory ([IO or Job] x njobs, then
+// njobs+1 u64 tile starts).  The descriptors reach device memory either by the
+// owning create() calls (hipMalloc + copy) or by the caller (any async copy)
+// followed by zccl_batch_bind(); the latter makes no allocating or
+// synchronising HIP call, which a serving loop with work in flight needs.
 
 struct zccl_batch {
     BatchKind kind;
     BatchArgs args;
     uint32_t threads;
     size_t lds;
-    void* d_jobs;
-    unsigned long long* d_tile0;
+    std::vector<unsigned char> host;  // packed descriptors
+    size_t tile0_off;                 // byte offset of the tile starts in `host`
+    void* owned;                      // device copy owned by the plan (create()), else null
+    bool bound;
 };
 
 namespace {
 
 template <typename IO, typename Job>
-zccl_batch* batch_upload(BatchKind kind, const std::vector<Job>& jobs, uint32_t chains, int* status)
+zccl_batch* batch_plan(BatchKind kind, const std::vector<Job>& jobs, uint32_t chains, int* status)
 {
     const size_t nj = jobs.size();
     bool uniform = true;
@@ -304,9 +336,13 @@ zccl_batch* batch_upload(BatchKind kind, const std::vector<Job>& jobs, uint32_t 
             memcmp(&a.p, &b.p, sizeof(OpParams)))
             uniform = false;
     }
-    std::vector<unsigned long long> tile0(nj + 1, 0);
+    zccl_batch* b = new zccl_batch();
+    b->kind = kind;
+    b->owned = nullptr;
+    b->bound = false;
     uint32_t threads = 64;
     size_t lds = 0;
+    std::vector<unsigned long long> tile0(nj + 1, 0);
     for (size_t j = 0; j < nj; j++) {
         tile0[j + 1] = tile0[j] + jobs[j].tl.ntiles;
         threads = std::max(threads, jobs[j].tl.threads);
@@ -315,40 +351,20 @@ zccl_batch* batch_upload(BatchKind kind, const std::vector<Job>& jobs, uint32_t 
                                                   : operand_lds_bytes(jobs[j].tl, 1, jobs[j].p.stage_bytes);
         lds = std::max(lds, need);
     }
-    zccl_batch* b = new zccl_batch();
-    b->kind = kind;
     b->threads = threads;
     b->lds = lds;
-    b->d_jobs = nullptr;
-    b->d_tile0 = nullptr;
-    size_t desc_bytes = uniform ? nj * sizeof(IO) : nj * sizeof(Job);
-    if (hipMalloc(&b->d_jobs, desc_bytes) != hipSuccess || hipMalloc(&b->d_tile0, 8 * (nj + 1)) != hipSuccess) {
-        (void) hipGetLastError();
-        zccl_batch_destroy(b);
-        *status = ZCCL_ENOMEM;
-        return nullptr;
-    }
-    hipError_t e;
+    const size_t desc = uniform ? nj * sizeof(IO) : nj * sizeof(Job);
+    b->tile0_off = (size_t) align_up(desc, 16);
+    b->host.assign(b->tile0_off + 8 * (nj + 1), 0);
     if (uniform) {
-        std::vector<IO> ios(nj);
         for (size_t j = 0; j < nj; j++)
-            ios[j] = jobs[j].io;
-        e = hipMemcpy(b->d_jobs, ios.data(), desc_bytes, hipMemcpyHostToDevice);
+            memcpy(b->host.data() + j * sizeof(IO), &jobs[j].io, sizeof(IO));
     } else {
-        e = hipMemcpy(b->d_jobs, jobs.data(), desc_bytes, hipMemcpyHostToDevice);
+        memcpy(b->host.data(), jobs.data(), desc);
     }
-    if (e == hipSuccess)
-        e = hipMemcpy(b->d_tile0, tile0.data(), 8 * (nj + 1), hipMemcpyHostToDevice);
-    if (e != hipSuccess) {
-        (void) hipGetLastError();
-        zccl_batch_destroy(b);
-        *status = ZCCL_EDEVICE;
-        return nullptr;
-    }
+    memcpy(b->host.data() + b->tile0_off, tile0.data(), 8 * (nj + 1));
     BatchArgs& a = b->args;
     memset(&a, 0, sizeof(a));
-    a.jobs = b->d_jobs;
-    a.job_tile0 = b->d_tile0;
     a.njobs = (uint32_t) nj;
     a.uniform = uniform;
     a.interleave = uniform && nj > 1;
@@ -363,52 +379,91 @@ zccl_batch* batch_upload(BatchKind kind, const std::vector<Job>& jobs, uint32_t 
     return b;
 }
 
+// Owning create(): device copy of the descriptors (allocates, synchronises).
+zccl_batch* batch_own(zccl_batch* b, int* status)
+{
+    if (!b)
+        return nullptr;
+    if (hipMalloc(&b->owned, b->host.size()) != hipSuccess) {
+        (void) hipGetLastError();
+        b->owned = nullptr;
+        zccl_batch_destroy(b);
+        *status = ZCCL_ENOMEM;
+        return nullptr;
+    }
+    if (hipMemcpy(b->owned, b->host.data(), b->host.size(), hipMemcpyHostToDevice) != hipSuccess) {
+        (void) hipGetLastError();
+        zccl_batch_destroy(b);
+        *status = ZCCL_EDEVICE;
+        return nullptr;
+    }
+    zccl_batch_bind(b, b->owned);
+    return b;
+}
+
+int* status_or(int* status, int* dummy) { return status ? status : dummy; }
+
 }  // namespace
 
-extern "C" zccl_batch* zccl_batch_homsum_create(int njobs, int k, const size_t* nbEle, int L, int T,
+extern "C" zccl_batch* zccl_batch_homsum_plan(int njobs, const int* ks, const size_t* nbEle, int L, int T,
     const unsigned char* const* ins, const void* const* indices, unsigned char* const* outs,
     const size_t* out_caps, void* const* out_indices, int* status)
 {
     int dummy;
-    int* st = status ? status : &dummy;
-    if (njobs < 1 || k < 1 || k > kMaxOperands || !nbEle || !ins || !indices || !outs || !out_caps ||
-        !out_indices) {
+    int* st = status_or(status, &dummy);
+    if (njobs < 1 || !ks || !nbEle || !ins || !indices || !outs || !out_caps || !out_indices) {
         *st = ZCCL_EINVAL;
         return nullptr;
     }
     std::vector<HomsumJob> jobs(njobs);
+    size_t pos = 0;
     for (int j = 0; j < njobs; j++) {
         HomsumJob& h = jobs[j];
         memset(&h, 0, sizeof(h));
-        if (!geom_of(nbEle[j], L, T, &h.g) || !outs[j] || !out_indices[j]) {
+        const int k = ks[j];
+        if (k < 1 || k > kMaxOperands || !geom_of(nbEle[j], L, T, &h.g) || !outs[j] || !out_indices[j]) {
             *st = ZCCL_EINVAL;
             return nullptr;
         }
-        h.tl = make_tiling(h.g, kHomsumThreads);
+        h.tl = make_tiling(h.g, op_threads(1));
         h.p = op_params(h.tl, 0.0, 0.0f, k);
-        for (int r = 0; r < k; r++) {
-            const unsigned char* in = ins[(size_t) j * k + r];
-            if (!in || !indices[(size_t) j * k + r] || in == outs[j]) {
+        for (int r = 0; r < k; r++, pos++) {
+            if (!ins[pos] || !indices[pos] || ins[pos] == outs[j]) {
                 *st = ZCCL_EINVAL;
                 return nullptr;
             }
-            h.io.in[r].bytes = in;
-            h.io.in[r].index = (const uint8_t*) indices[(size_t) j * k + r];
+            h.io.in[r].bytes = ins[pos];
+            h.io.in[r].index = (const uint8_t*) indices[pos];
         }
         h.io.out = outs[j];
         h.io.out_cap = out_caps[j];
         h.io.index = (uint8_t*) out_indices[j];
         h.io.size_out = nullptr;
     }
-    return batch_upload<HomsumIO>(BatchKind::Homsum, jobs, (uint32_t) njobs, st);
+    return batch_plan<HomsumIO>(BatchKind::Homsum, jobs, (uint32_t) njobs, st);
 }
 
-extern "C" zccl_batch* zccl_batch_compress_create(int njobs, const size_t* nbEle, const float* ebs, int L,
-    int T, const float* const* xs, unsigned char* const* outs, const size_t* out_caps,
-    void* const* out_indices, int* status)
+extern "C" zccl_batch* zccl_batch_homsum_create(int njobs, int k, const size_t* nbEle, int L, int T,
+    const unsigned char* const* ins, const void* const* indices, unsigned char* const* outs,
+    const size_t* out_caps, void* const* out_indices, int* status)
 {
     int dummy;
-    int* st = status ? status : &dummy;
+    int* st = status_or(status, &dummy);
+    if (njobs < 1) {
+        *st = ZCCL_EINVAL;
+        return nullptr;
+    }
+    std::vector<int> ks((size_t) njobs, k);
+    return batch_own(zccl_batch_homsum_plan(njobs, ks.data(), nbEle, L, T, ins, indices, outs, out_caps,
+                         out_indices, st), st);
+}
+
+extern "C" zccl_batch* zccl_batch_compress_plan(int njobs, const size_t* nbEle, const float* ebs, int L, int T,
+    const float* const* xs, unsigned char* const* outs, const size_t* out_caps, void* const* out_indices,
+    int* status)
+{
+    int dummy;
+    int* st = status_or(status, &dummy);
     if (njobs < 1 || !nbEle || !ebs || !xs || !outs || !out_caps || !out_indices) {
         *st = ZCCL_EINVAL;
         return nullptr;
@@ -421,7 +476,7 @@ extern "C" zccl_batch* zccl_batch_compress_create(int njobs, const size_t* nbEle
             *st = ZCCL_EINVAL;
             return nullptr;
         }
-        c.tl = make_tiling(c.g, kCompressThreads);
+        c.tl = make_tiling(c.g, op_threads(0));
         c.p = op_params(c.tl, inverse_bound(ebs[j]), ebs[j], 1);
         c.io.x = xs[j];
         c.io.out = outs[j];
@@ -429,15 +484,23 @@ extern "C" zccl_batch* zccl_batch_compress_create(int njobs, const size_t* nbEle
         c.io.index = (uint8_t*) out_indices[j];
         c.io.size_out = nullptr;
     }
-    return batch_upload<CompressIO>(BatchKind::Compress, jobs, (uint32_t) njobs, st);
+    return batch_plan<CompressIO>(BatchKind::Compress, jobs, (uint32_t) njobs, st);
 }
 
-extern "C" zccl_batch* zccl_batch_decompress_create(int njobs, const size_t* nbEle, const float* ebs,
-    int L, int T, const unsigned char* const* ins, const void* const* indices, float* const* outs,
-    int* status)
+extern "C" zccl_batch* zccl_batch_compress_create(int njobs, const size_t* nbEle, const float* ebs, int L,
+    int T, const float* const* xs, unsigned char* const* outs, const size_t* out_caps,
+    void* const* out_indices, int* status)
 {
     int dummy;
-    int* st = status ? status : &dummy;
+    int* st = status_or(status, &dummy);
+    return batch_own(zccl_batch_compress_plan(njobs, nbEle, ebs, L, T, xs, outs, out_caps, out_indices, st), st);
+}
+
+extern "C" zccl_batch* zccl_batch_decompress_plan(int njobs, const size_t* nbEle, const float* ebs, int L,
+    int T, const unsigned char* const* ins, const void* const* indices, float* const* outs, int* status)
+{
+    int dummy;
+    int* st = status_or(status, &dummy);
     if (njobs < 1 || !nbEle || !ebs || !ins || !indices || !outs) {
         *st = ZCCL_EINVAL;
         return nullptr;
@@ -451,21 +514,44 @@ extern "C" zccl_batch* zccl_batch_decompress_create(int njobs, const size_t* nbE
             *st = ZCCL_EINVAL;
             return nullptr;
         }
-        d.tl = make_tiling(d.g, kDecompressThreads);
+        d.tl = make_tiling(d.g, op_threads(2));
         d.p = op_params(d.tl, 0.0, ebs[j], 1);
         d.io.in[0].bytes = ins[j];
         d.io.in[0].index = (const uint8_t*) indices[j];
         d.io.out = outs[j];
         chains += d.g.S;
     }
-    return batch_upload<DecompressIO>(BatchKind::Decompress, jobs, chains, st);
+    return batch_plan<DecompressIO>(BatchKind::Decompress, jobs, chains, st);
+}
+
+extern "C" zccl_batch* zccl_batch_decompress_create(int njobs, const size_t* nbEle, const float* ebs,
+    int L, int T, const unsigned char* const* ins, const void* const* indices, float* const* outs,
+    int* status)
+{
+    int dummy;
+    int* st = status_or(status, &dummy);
+    return batch_own(zccl_batch_decompress_plan(njobs, nbEle, ebs, L, T, ins, indices, outs, st), st);
+}
+
+extern "C" size_t zccl_batch_desc_bytes(const zccl_batch* b) { return b ? b->host.size() : 0; }
+
+extern "C" const void* zccl_batch_desc_host(const zccl_batch* b) { return b ? b->host.data() : nullptr; }
+
+extern "C" int zccl_batch_bind(zccl_batch* b, void* dev_desc)
+{
+    if (!b || !dev_desc || ((uintptr_t) dev_desc & 15))
+        return ZCCL_EINVAL;
+    b->args.jobs = dev_desc;
+    b->args.job_tile0 = (const unsigned long long*) ((unsigned char*) dev_desc + b->tile0_off);
+    b->bound = true;
+    return ZCCL_OK;
 }
 
 extern "C" size_t zccl_batch_tiles(const zccl_batch* b) { return b ? (size_t) b->args.total_tiles : 0; }
 
 extern "C" int zccl_batch_launch(const zccl_batch* b, void* ws, size_t ws_bytes, void* stream)
 {
-    if (!b)
+    if (!b || !b->bound)
         return ZCCL_EINVAL;
     int rc = check_ws(ws, ws_bytes, b->args.total_tiles);
     if (rc)
@@ -480,10 +566,8 @@ extern "C" void zccl_batch_destroy(zccl_batch* b)
 {
     if (!b)
         return;
-    if (b->d_jobs)
-        (void) hipFree(b->d_jobs);
-    if (b->d_tile0)
-        (void) hipFree(b->d_tile0);
+    if (b->owned)
+        (void) hipFree(b->owned);
     delete b;
 }
 
