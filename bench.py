@@ -55,6 +55,11 @@ def parse_args():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 256 MiB elementwise copy kernel (rocprof counter calibration)")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "homsum_traffic.json"))
+    ap.add_argument("--no-batched", action="store_true", help="skip the uneven-size batched workload")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving workload")
+    ap.add_argument("--batch-buffers", type=int, default=4096)
+    ap.add_argument("--batch-ranks", type=int, default=16)
+    ap.add_argument("--serve-requests", type=int, default=400)
     return ap.parse_args()
 
 
@@ -250,6 +255,19 @@ def main():
         roof["traffic"] = tr.get("hbm_bytes_per_launch")
         roof["traffic_source"] = os.path.relpath(args.traffic, ROOT)
 
+    # uneven-size batched form and the serving loop (extra workloads, reported
+    # beside the headline line)
+    from zccl_amd import workloads
+
+    batched = None
+    if not args.no_batched:
+        log("batched uneven-size workload")
+        batched = workloads.uneven_batch(nbuf=args.batch_buffers, ranks=args.batch_ranks, L=L, T=T)
+    served = None
+    if not args.no_serve:
+        log("serving workload")
+        served = workloads.serving(nreq=args.serve_requests, L=L, T=T)
+
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline")
@@ -289,7 +307,12 @@ def main():
         "max_abs_err": max_err,
         "max_err_over_P_eb": round(max_err / (P * eb), 4),
         "n_err_over_P_eb": n_over,
+        "batched_uneven": batched,
+        "serve": served,
     }
+    if served:
+        res["serve_p50_ms"] = served["loads"][-1]["p50_ms"]
+        res["serve_p99_ms"] = served["loads"][-1]["p99_ms"]
     print(json.dumps(res), flush=True)
 
 

@@ -216,15 +216,20 @@ ZK_HD uint32_t tile_block_len(const Geom& g, const Tile& x, uint32_t j)
 }
 
 // Workspace: decoupled-lookback state.  Two status arrays alternate between
-// consecutive launches; the parity flips when the last lookback chain of a
-// launch completes, and every launch clears the array the next one uses, so
+// consecutive launches.  `seq` holds the parity of the array in use (bit 32)
+// and the next tile ticket (low 32 bits): every workgroup takes its tile
+// index from it, so tiles are numbered in the order they start and a started
+// tile's predecessors have all started, whatever else runs on the device.
+// When the last lookback chain of a launch completes, seq is reset to ticket
+// 0 of the other parity; every launch clears the array the next one uses, so
 // nothing is reset from the host between launches (graph replays included).
 struct WsHeader {
-    unsigned int parity;         // status array in use
+    unsigned long long seq;      // parity << 32 | next tile ticket
     unsigned int chains_done;    // chains of the current launch that completed
+    unsigned int pad0;
     unsigned long long capacity; // status entries per array
     unsigned long long errors;   // sticky error bits (ZK_ERR_*)
-    unsigned long long pad[29];
+    unsigned long long pad[28];
 };
 static_assert(sizeof(WsHeader) == 256, "workspace header is 256 bytes");
 

@@ -262,18 +262,26 @@ __device__ __forceinline__ unsigned long long lookback(unsigned long long* statu
     return excl;
 }
 
-// Status arrays of the current launch.  The workgroup also clears its share
-// of the array the next launch will use.  Returns false when the workspace is
-// too small (then no workgroup of the launch does anything).
-__device__ __forceinline__ bool launch_status(WsHeader* ws, uint64_t ntiles_total, uint64_t t, unsigned long long*& status)
+// Tile ticket and status arrays of the current launch.  The workgroup takes
+// the next ticket (its tile index t, in start order) and clears its share of
+// the array the next launch will use.  Returns false when the workspace is
+// too small (then no workgroup of the launch takes a ticket or does anything).
+__device__ __forceinline__ bool launch_status(WsHeader* ws, uint64_t ntiles_total, uint64_t& t,
+    unsigned long long*& status)
 {
+    __shared__ unsigned long long s_ticket;
     const uint64_t cap = ws->capacity;
     if (cap < ntiles_total) {
-        if (threadIdx.x == 0 && t == 0)
+        if (threadIdx.x == 0 && blockIdx.x == 0)
             raise_error(ws, ZK_ERR_CAPACITY);
         return false;
     }
-    const uint32_t par = ws->parity & 1u;
+    if (threadIdx.x == 0)
+        s_ticket = atomicAdd(&ws->seq, 1ull);
+    __syncthreads();
+    const unsigned long long tk = s_ticket;
+    const uint32_t par = (uint32_t) (tk >> 32) & 1u;
+    t = (uint32_t) tk;
     unsigned long long* base = (unsigned long long*) ((char*) ws + sizeof(WsHeader));
     status = base + (uint64_t) par * cap * kStatusStride;
     unsigned long long* other = base + (uint64_t) (par ^ 1u) * cap * kStatusStride;
@@ -285,15 +293,17 @@ __device__ __forceinline__ bool launch_status(WsHeader* ws, uint64_t ntiles_tota
 }
 
 // Called by one thread of the last tile of a lookback chain once that tile
-// has published.  A finished chain implies every tile of the chain published
-// (and so read the parity); when the last chain of the launch finishes the
-// parity flips for the next launch.
+// has published.  A finished chain implies every tile of the chain took its
+// ticket; when the last chain of the launch finishes, every ticket of the
+// launch has been taken and the ticket word restarts at 0 with the other
+// parity for the next launch.
 __device__ __forceinline__ void chain_finished(WsHeader* ws, uint32_t nchains)
 {
     unsigned int old = atomicAdd(&ws->chains_done, 1u);
     if (old == nchains - 1) {
         ws->chains_done = 0;
-        __hip_atomic_store(&ws->parity, ws->parity ^ 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned long long par = (__hip_atomic_load(&ws->seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >> 32) & 1ull;
+        __hip_atomic_store(&ws->seq, (par ^ 1ull) << 32, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -1021,7 +1031,7 @@ __device__ __forceinline__ void decompress_tile(const Geom& g, const Tiling& tl,
 
 __global__ void __launch_bounds__(kCompressThreads) k_compress(CompressArgs a)
 {
-    const uint64_t t = blockIdx.x;
+    uint64_t t;
     unsigned long long* status;
     if (!launch_status(a.ws, a.job.tl.ntiles, t, status))
         return;
@@ -1030,7 +1040,7 @@ __global__ void __launch_bounds__(kCompressThreads) k_compress(CompressArgs a)
 
 __global__ void __launch_bounds__(kHomsumThreads, 4) k_homsum(HomsumArgs a)
 {
-    const uint64_t t = blockIdx.x;
+    uint64_t t;
     unsigned long long* status;
     if (!launch_status(a.ws, a.job.tl.ntiles, t, status))
         return;
@@ -1039,7 +1049,7 @@ __global__ void __launch_bounds__(kHomsumThreads, 4) k_homsum(HomsumArgs a)
 
 __global__ void __launch_bounds__(kDecompressThreads) k_decompress(DecompressArgs a)
 {
-    const uint64_t t = blockIdx.x;
+    uint64_t t;
     unsigned long long* status;
     if (!launch_status(a.ws, a.job.tl.ntiles, t, status))
         return;
@@ -1089,11 +1099,12 @@ __global__ void __launch_bounds__(kHomsumThreads, 4) k_homsum_batch(BatchArgs b)
 {
     __shared__ HomsumIO sio;
     unsigned long long* status;
-    if (!launch_status(b.ws, b.total_tiles, blockIdx.x, status))
+    uint64_t tg;
+    if (!launch_status(b.ws, b.total_tiles, tg, status))
         return;
     uint32_t jidx;
     uint64_t t;
-    job_of(b, blockIdx.x, jidx, t);
+    job_of(b, tg, jidx, t);
     stage_desc(b.jobs, jidx, &sio);
     homsum_tile(b.g, b.tl, b.p, sio, t, status + b.job_tile0[jidx] * kStatusStride, b.ws, b.total_chains);
 }
@@ -1102,11 +1113,12 @@ __global__ void __launch_bounds__(kCompressThreads) k_compress_batch(BatchArgs b
 {
     __shared__ CompressIO sio;
     unsigned long long* status;
-    if (!launch_status(b.ws, b.total_tiles, blockIdx.x, status))
+    uint64_t tg;
+    if (!launch_status(b.ws, b.total_tiles, tg, status))
         return;
     uint32_t jidx;
     uint64_t t;
-    job_of(b, blockIdx.x, jidx, t);
+    job_of(b, tg, jidx, t);
     stage_desc(b.jobs, jidx, &sio);
     compress_tile(b.g, b.tl, b.p, sio, t, status + b.job_tile0[jidx] * kStatusStride, b.ws, b.total_chains);
 }
@@ -1115,11 +1127,12 @@ __global__ void __launch_bounds__(kDecompressThreads) k_decompress_batch(BatchAr
 {
     __shared__ DecompressIO sio;
     unsigned long long* status;
-    if (!launch_status(b.ws, b.total_tiles, blockIdx.x, status))
+    uint64_t tg;
+    if (!launch_status(b.ws, b.total_tiles, tg, status))
         return;
     uint32_t jidx;
     uint64_t t;
-    job_of(b, blockIdx.x, jidx, t);
+    job_of(b, tg, jidx, t);
     stage_desc(b.jobs, jidx, &sio);
     decompress_tile(b.g, b.tl, b.p, sio, t, status + b.job_tile0[jidx] * kStatusStride, b.ws, b.total_chains);
 }
@@ -1129,11 +1142,12 @@ __global__ void __launch_bounds__(kHomsumThreads, 4) k_homsum_mixed(BatchArgs b)
 {
     __shared__ HomsumJob sj;
     unsigned long long* status;
-    if (!launch_status(b.ws, b.total_tiles, blockIdx.x, status))
+    uint64_t tg;
+    if (!launch_status(b.ws, b.total_tiles, tg, status))
         return;
     uint32_t jidx;
     uint64_t t;
-    job_of(b, blockIdx.x, jidx, t);
+    job_of(b, tg, jidx, t);
     stage_desc(b.jobs, jidx, &sj);
     homsum_tile(sj.g, sj.tl, sj.p, sj.io, t, status + b.job_tile0[jidx] * kStatusStride, b.ws, b.total_chains);
 }
@@ -1142,11 +1156,12 @@ __global__ void __launch_bounds__(kCompressThreads) k_compress_mixed(BatchArgs b
 {
     __shared__ CompressJob sj;
     unsigned long long* status;
-    if (!launch_status(b.ws, b.total_tiles, blockIdx.x, status))
+    uint64_t tg;
+    if (!launch_status(b.ws, b.total_tiles, tg, status))
         return;
     uint32_t jidx;
     uint64_t t;
-    job_of(b, blockIdx.x, jidx, t);
+    job_of(b, tg, jidx, t);
     stage_desc(b.jobs, jidx, &sj);
     compress_tile(sj.g, sj.tl, sj.p, sj.io, t, status + b.job_tile0[jidx] * kStatusStride, b.ws, b.total_chains);
 }
@@ -1155,11 +1170,12 @@ __global__ void __launch_bounds__(kDecompressThreads) k_decompress_mixed(BatchAr
 {
     __shared__ DecompressJob sj;
     unsigned long long* status;
-    if (!launch_status(b.ws, b.total_tiles, blockIdx.x, status))
+    uint64_t tg;
+    if (!launch_status(b.ws, b.total_tiles, tg, status))
         return;
     uint32_t jidx;
     uint64_t t;
-    job_of(b, blockIdx.x, jidx, t);
+    job_of(b, tg, jidx, t);
     stage_desc(b.jobs, jidx, &sj);
     decompress_tile(sj.g, sj.tl, sj.p, sj.io, t, status + b.job_tile0[jidx] * kStatusStride, b.ws, b.total_chains);
 }
@@ -1302,7 +1318,7 @@ __global__ void k_fill_u64(unsigned long long* p, uint64_t n, unsigned long long
 __global__ void k_ws_init(WsHeader* ws, uint64_t capacity)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
-        ws->parity = 0;
+        ws->seq = 0;
         ws->chains_done = 0;
         ws->capacity = capacity;
         ws->errors = 0;
