@@ -31,7 +31,7 @@ static uint64_t compress_lds(const Tiling& tl) { return align_up(tl.tile_bytes_m
 static uint64_t stage_bytes(const Tiling& tl, int op)
 {
     uint64_t need = align_up(tl.tile_bytes_max + 64, 16);
-    return op == 1 && need < 40 * 1024 ? 40 * 1024 : need;
+    return op == 1 && need < 32 * 1024 ? 32 * 1024 : need;
 }
 
 static void check_tiling(const Geom& g, uint32_t threads, int op)

@@ -16,9 +16,11 @@ constexpr uint32_t kCompressThreads = 512;
 constexpr uint32_t kDecompressThreads = 512;
 constexpr uint32_t kHomsumThreads = 256;
 
-// LDS stage area of the homomorphic sum (bytes): 8 operands' tile bytes at
-// typical compression ratios fit in one pass.
-constexpr uint32_t kStageMin = 40 * 1024;
+// LDS stage area of the homomorphic sum (bytes).  Sized so four 256-thread
+// workgroups share a CU's 160 KB; operands that do not fit are staged in a
+// further round (measured: four workgroups with a second round beat three
+// with one).
+constexpr uint32_t kStageMin = 32 * 1024;
 
 struct Operand {
     const uint8_t* bytes;
