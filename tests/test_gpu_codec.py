@@ -438,3 +438,41 @@ def test_tuned_workgroup_sizes_give_same_bytes(codec):
         for op in range(3):
             lib().zccl_tune_threads(op, 0)
     assert lib().zccl_tune_threads(1, 512) != 0  # above the sum kernel's launch bound
+
+
+def test_staged_allreduce_forms_agree(codec):
+    """hZCCL data flow on one device: batched and per-call compress agree,
+    the fused reduce-scatter equals the reference's ring of pairwise adds,
+    batched and per-call decompress agree, and both the homomorphic result
+    and the decompress-operate-compress ring stay within their error bounds
+    of the FP64 sum."""
+    from zccl_amd.allreduce import StagedAllreduce
+
+    P, count, L, T, eb = 4, 400_003, 36, 8, 1e-3
+    xs = [dev(gen.make("grf", count, 1200 + r)) for r in range(P)]
+    ar = StagedAllreduce(P=P, count=count, eb=eb, L=L, T=T)
+    ar.compress_all(xs, batched=True)
+    torch.cuda.synchronize()
+    batched = [[s.data[: s.size].clone() for s in row] for row in ar.streams]
+    ar.compress_all(xs, batched=False)
+    torch.cuda.synchronize()
+    for r in range(P):
+        for j in range(P):
+            assert torch.equal(ar.streams[r][j].data[: ar.streams[r][j].size], batched[r][j]), (r, j)
+    fused = ar.reduce_scatter_fused()
+    ring = ar.reduce_scatter_ring(ar.ring_scratch())
+    for a, b in zip(fused, ring):
+        assert torch.equal(a.data[: a.size], b.data[: b.size])
+    y1 = ar.decompress_all(fused, batched=True)
+    y2 = ar.decompress_all(fused, batched=False)
+    torch.cuda.synchronize()
+    codec.check_errors()
+    assert torch.equal(y1.view(torch.int32), y2.view(torch.int32))
+    truth = sum(x.double() for x in xs)
+    bound = P * eb + 2.0 ** -21 * sum(x.double().abs() for x in xs)
+    assert bool(((y1.double() - truth).abs() <= bound).all())
+    doc = ar.reduce_scatter_doc(xs)
+    for j, seg in enumerate(doc):
+        t = truth[ar.displs[j]: ar.displs[j] + ar.cnts[j]]
+        err = (seg.double() - t).abs().max().item()
+        assert err <= (P - 1) * (eb + 1e-4), (j, err)

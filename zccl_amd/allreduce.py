@@ -75,16 +75,27 @@ class StagedAllreduce:
         self.streams: Optional[List[List[codec.DeviceStream]]] = None
 
     # step 1
-    def compress_all(self, xs: Sequence[torch.Tensor]) -> List[List[codec.DeviceStream]]:
+    def compress_all(self, xs: Sequence[torch.Tensor], batched: bool = True) -> List[List[codec.DeviceStream]]:
+        """Compress every rank's P segments (ZCCL_ring_ho.c:247-256): one
+        batched launch for all P*P segments, or one call per segment."""
         if len(xs) != self.P:
             raise ValueError("one input per rank")
         if self.streams is None:
             self.streams = [[codec.empty_stream(self.cnts[j], self.L, self.T) for j in range(self.P)]
                             for _ in range(self.P)]
-        for r, x in enumerate(xs):
-            for j in range(self.P):
-                seg = x[self.displs[j]: self.displs[j] + self.cnts[j]]
-                codec.compress(seg, self.eb, self.L, self.T, out=self.streams[r][j])
+        segs = [x[self.displs[j]: self.displs[j] + self.cnts[j]] for x in xs for j in range(self.P)]
+        if batched:
+            from . import batch
+
+            key = tuple(s.data_ptr() for s in segs)
+            if getattr(self, "_cplan_key", None) != key:
+                outs = [self.streams[r][j] for r in range(self.P) for j in range(self.P)]
+                self._cplan = batch.compress_plan(segs, [self.eb] * len(segs), outs)
+                self._cplan_key = key
+            self._cplan.launch()
+        else:
+            for i, seg in enumerate(segs):
+                codec.compress(seg, self.eb, self.L, self.T, out=self.streams[i // self.P][i % self.P])
         return self.streams
 
     # step 2, fused
@@ -111,8 +122,42 @@ class StagedAllreduce:
         return [[codec.empty_stream(self.cnts[j], self.L, self.T) for _ in range(2)] for j in range(self.P)]
 
     # step 3
-    def decompress_all(self, sums: Sequence[codec.DeviceStream], out: Optional[torch.Tensor] = None) -> torch.Tensor:
+    def decompress_all(self, sums: Sequence[codec.DeviceStream], out: Optional[torch.Tensor] = None,
+                       batched: bool = True) -> torch.Tensor:
+        """Decompress the P reduced segments into the full output
+        (ZCCL_ring_ho.c:163-169); one batched launch or one call each."""
         y = out if out is not None else torch.empty(self.count, dtype=torch.float32, device="cuda")
-        for j in range(self.P):
-            codec.decompress(sums[j], self.eb, out=y[self.displs[j]: self.displs[j] + self.cnts[j]])
+        views = [y[self.displs[j]: self.displs[j] + self.cnts[j]] for j in range(self.P)]
+        if batched:
+            from . import batch
+
+            key = tuple(v.data_ptr() for v in views) + tuple(s.data.data_ptr() for s in sums)
+            if getattr(self, "_dplan_key", None) != key:
+                self._dplan = batch.decompress_plan(list(sums), [self.eb] * self.P, views)
+                self._dplan_key = key
+            self._dplan.launch()
+        else:
+            for j in range(self.P):
+                codec.decompress(sums[j], self.eb, out=views[j])
         return y
+
+    # the reference's decompress-operate-compress ring (DOC), for comparison
+    def reduce_scatter_doc(self, xs: Sequence[torch.Tensor]) -> List[torch.Tensor]:
+        """MPI_Allreduce_ZCCL_RI2_mt_oa_record's reduce-scatter
+        (ZCCL/src/ZCCL_ring.c:979-1108) with all P ranks on one device: at
+        each of the P-1 steps a rank compresses its partial FP32 sum of the
+        segment it sends (:1049-1055), the receiver decompresses it (:1066-1067)
+        and adds its own FP32 segment (MPI_Reduce_local, :1068-1069).  Returns
+        the FP32 segment each rank ends with (rank j owns segment j)."""
+        P = self.P
+        part = [[xs[r][self.displs[j]: self.displs[j] + self.cnts[j]].clone() for j in range(P)] for r in range(P)]
+        tmp = [codec.empty_stream(self.cnts[j], self.L, self.T) for j in range(P)]
+        for i in range(P - 1):
+            # every rank r sends segment (r-1-i) mod P to rank r+1
+            for r in range(P):
+                send = (P + r - 1 - i) % P
+                dst = (r + 1) % P
+                codec.compress(part[r][send], self.eb, self.L, self.T, out=tmp[send])
+                y = codec.decompress(tmp[send], self.eb)
+                part[dst][send] += y
+        return [part[(j) % P][j] for j in range(P)]
