@@ -156,28 +156,6 @@ __device__ __forceinline__ unsigned long long wave_sum64(unsigned long long x)
     return x;
 }
 
-// Exclusive scan of one uint32 per thread over the workgroup (blockDim a
-// multiple of 64, at most 1024).  total receives the workgroup sum.
-__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_wave, uint32_t& total)
-{
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    uint32_t x = wave_incl_scan(v);
-    if (lane == 63)
-        s_wave[wid] = x;
-    __syncthreads();
-    if (wid == 0) {
-        uint32_t t = lane < nw ? s_wave[lane] : 0u;
-        t = wave_incl_scan(t);
-        if (lane < nw)
-            s_wave[lane] = t;
-    }
-    __syncthreads();
-    uint32_t base = wid ? s_wave[wid - 1] : 0u;
-    total = s_wave[nw - 1];
-    __syncthreads();
-    return base + x - v;
-}
-
 // ------------------------------------------------------------ lookback
 
 constexpr unsigned long long kFlagAgg = 1ull << 62;
@@ -555,30 +533,41 @@ __device__ __forceinline__ void encode_tile(const Geom& g, const Tiling& tl, con
     uint32_t nchains)
 {
     const uint32_t tid = threadIdx.x;
-    // widths and sizes, blocks assigned to threads in contiguous runs
-    const uint32_t cpt = (uint32_t) ceil_div(tl.G, blockDim.x);
-    uint32_t loc[4] = {0, 0, 0, 0};
-    uint32_t run = 0;
-    for (uint32_t c = 0; c < cpt; c++) {
-        uint32_t j = tid * cpt + c;
-        if (j < x.nblk) {
-            uint32_t w = dwidth(sh.bmax[j]);
-            sh.width[j] = (uint8_t) w;
-            loc[c] = run;
-            run += block_bytes(w, tile_block_len(g, x, j));
+    if (tid < 64) {
+        // wave 0: widths, sizes and offsets of the tile's blocks (up to 4 per
+        // lane, contiguous runs) with one wave scan
+        const uint32_t cpt = (uint32_t) ceil_div(tl.G, 64);
+        uint32_t loc[4] = {0, 0, 0, 0};
+        uint32_t run = 0;
+        for (uint32_t c = 0; c < cpt; c++) {
+            uint32_t j = tid * cpt + c;
+            if (j < x.nblk) {
+                uint32_t w = dwidth(sh.bmax[j]);
+                sh.width[j] = (uint8_t) w;
+                loc[c] = run;
+                run += block_bytes(w, tile_block_len(g, x, j));
+            }
         }
+        const uint32_t incl = wave_incl_scan(run);
+        for (uint32_t c = 0; c < cpt; c++) {
+            uint32_t j = tid * cpt + c;
+            if (j < x.nblk)
+                sh.boff[j] = incl - run + loc[c];
+        }
+        if (tid == 63)
+            sh.bc[0] = incl;
     }
-    uint32_t total;
-    uint32_t base = block_excl_scan(run, sh.wave, total);
-    for (uint32_t c = 0; c < cpt; c++) {
-        uint32_t j = tid * cpt + c;
-        if (j < x.nblk)
-            sh.boff[j] = base + loc[c];
+    {
+        // the other waves (wave 0 too when it is the only one) clear the
+        // encode buffer for the largest tile
+        const uint32_t nwords = (tl.tile_bytes_max + 3) / 4 + 8;
+        const uint32_t first = blockDim.x > 64 ? 64u : 0u;
+        if (tid >= first)
+            for (uint32_t i = tid - first; i < nwords; i += blockDim.x - first)
+                buf[i] = 0;
     }
-    uint32_t nwords = (total + 3) / 4 + 8;
-    for (uint32_t i = tid; i < nwords; i += blockDim.x)
-        buf[i] = 0;
     __syncthreads();
+    const uint32_t total = (uint32_t) sh.bc[0];
     // wave 0 resolves the tile's stream offset while the others encode; the
     // encoding needs only offsets inside the tile
     if (tid < 64) {
@@ -851,6 +840,8 @@ __device__ __forceinline__ void homsum_tile(const Geom& g, const Tiling& tl, con
     uint32_t* stage = zk_lds;
     uint32_t* s_wo = zk_lds + p.stage_bytes / 4;
     const Units un = tile_units(g, x);
+    for (uint32_t j = tid; j < tl.G; j += blockDim.x)
+        sh.bmax[j] = 0;  // visible to the block-max atomics after the staging barriers
     const uint32_t last_j = x.nblk ? x.nblk - 1 : 0u;
     const uint32_t last_len = x.nblk ? tile_block_len(g, x, last_j) : 0u;
     uint32_t nj[kUnitsPerThread], sbj[kUnitsPerThread];
@@ -900,15 +891,12 @@ __device__ __forceinline__ void homsum_tile(const Geom& g, const Tiling& tl, con
                 raise_error(ws, ZK_ERR_OVERRUN);
         }
     }
-    for (uint32_t j = tid; j < tl.G; j += blockDim.x)
-        sh.bmax[j] = 0;
 #pragma unroll
     for (uint32_t i = 0; i < kUnitsPerThread; i++)
 #pragma unroll
         for (int c = 0; c < 4; c++)
             if ((uint32_t) c >= unit_cnt(un.pk[i]))
                 acc[i][c] = 0;
-    __syncthreads();
     fold_block_max(sh.bmax, un, acc);
     __syncthreads();
     encode_tile(g, tl, x, t, sh, stage, status, ws, un, acc, seed, io.out, io.out_cap, io.index, io.size_out,
