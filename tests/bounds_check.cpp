@@ -30,8 +30,53 @@ static long g_cases = 0;
 static uint64_t compress_lds(const Tiling& tl) { return align_up(tl.tile_bytes_max + 64, 16); }
 static uint64_t stage_bytes(const Tiling& tl, int op)
 {
-    uint64_t need = align_up(tl.tile_bytes_max + 64, 16);
-    return op == 1 && need < 32 * 1024 ? 32 * 1024 : need;
+    if (op != 1)
+        return align_up(tl.tile_bytes_max + 64, 16);
+    uint64_t need = align_up(tl.tile_bytes_max + 48, 32);
+    return need < 18 * 1024 ? 18 * 1024 : need;
+}
+
+// Staging plans of the sum: for k operands of any tile size up to the worst
+// case and any alignment, every operand's chunks plus slack lie inside its
+// round's region (a half, or the whole stage), operands of a round do not
+// overlap, and rounds cover the operands in order.
+static void check_stage_plans(const Tiling& tl, int k, uint32_t seed)
+{
+    const uint32_t stage = (uint32_t) stage_bytes(tl, 1), half = stage / 2;
+    REQUIRE(stage % 32 == 0, "stage %u", stage);
+    uint32_t tot[kMaxOperands], shift[kMaxOperands];
+    uint64_t st = seed * 2654435761ull + 1;
+    auto rnd = [&](uint32_t m) { st = st * 6364136223846793005ull + 1442695040888963407ull; return (uint32_t) ((st >> 33) % m); };
+    for (int r = 0; r < k; r++) {
+        const uint32_t mode = rnd(4);
+        tot[r] = mode == 0 ? tl.tile_bytes_max : mode == 1 ? 0u : rnd(tl.tile_bytes_max + 1);
+        shift[r] = rnd(16);
+    }
+    StagePlan sp;
+    plan_stage(tot, shift, k, half, sp);
+    REQUIRE(sp.nrounds >= 1 && sp.nrounds <= k, "rounds %d", sp.nrounds);
+    int r = 0;
+    for (int g = 0; g < sp.nrounds; g++) {
+        REQUIRE(sp.gend[g] > r && sp.gend[g] <= k, "round end");
+        const uint32_t lo = sp.whole[g] ? 0u : (uint32_t) (g & 1) * half;
+        const uint32_t hi = sp.whole[g] ? stage : lo + half;
+        uint32_t prev_end = lo;
+        for (; r < sp.gend[g]; r++) {
+            const uint32_t need = stage_need(shift[r], tot[r]);
+            REQUIRE(sp.nch[r] == stage_chunks(shift[r], tot[r]), "chunks");
+            REQUIRE(tot[r] == 0 || 16 * sp.nch[r] >= shift[r] + tot[r], "chunks cover the bytes");
+            REQUIRE(sp.lds[r] >= prev_end && sp.lds[r] + need <= hi, "operand %d in round %d: [%u,%u) of [%u,%u)", r,
+                g, sp.lds[r], sp.lds[r] + need, lo, hi);
+            REQUIRE(sp.lds[r] % 16 == 0, "aligned");
+            // the decoder reads up to 16 bytes past the operand's bytes
+            REQUIRE(tot[r] == 0 || sp.lds[r] + shift[r] + tot[r] + 16 <= sp.lds[r] + need, "slack");
+            prev_end = sp.lds[r] + need;
+        }
+        if (sp.whole[g])
+            REQUIRE(sp.gend[g] == (g ? sp.gend[g - 1] : 0) + 1, "a whole-stage round holds one operand");
+    }
+    REQUIRE(r == k, "every operand staged");
+    g_cases++;
 }
 
 static void check_tiling(const Geom& g, uint32_t threads, int op)
@@ -159,6 +204,9 @@ int main()
                 check_tiling(g, 512, 0);  // compress
                 check_tiling(g, 256, 1);  // homomorphic sum
                 check_tiling(g, 512, 2);  // decompress
+                const Tiling th = make_tiling(g, 256);
+                for (int k : {1, 2, 3, 8, 17, 32})
+                    check_stage_plans(th, k, (uint32_t) (n * 131 + L * 7 + T + k));
             }
     std::printf("ok %ld\n", g_cases);
     return 0;

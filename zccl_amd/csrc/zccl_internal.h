@@ -215,6 +215,55 @@ ZK_HD uint32_t tile_block_len(const Geom& g, const Tile& x, uint32_t j)
     return left < g.L ? left : g.L;
 }
 
+// Staging plan of the homomorphic sum's operands.  Operand r's tile bytes
+// [base, base + tot) are fetched as nch 16-byte chunks starting at the
+// 16-byte aligned stream address at or below base (shift = address mod 16),
+// so the operand needs 16*nch + 16 LDS bytes (16 of decoder read slack).
+// Operands are packed greedily into rounds; consecutive rounds alternate
+// between the two halves of the stage, so the next round can be fetched while
+// the current one is decoded.  An operand larger than a half forms a round of
+// its own over the whole stage (fetched without overlap).
+struct StagePlan {
+    uint32_t lds[kMaxOperands];   // stage byte offset of the operand's first chunk
+    uint32_t nch[kMaxOperands];   // chunks of the operand
+    int gend[kMaxOperands];       // round g holds operands [gend[g-1], gend[g])
+    uint32_t whole[kMaxOperands]; // round g uses the whole stage
+    int nrounds;
+};
+
+ZK_HD uint32_t stage_chunks(uint32_t shift, uint32_t tot) { return tot ? (shift + tot + 15) / 16 : 0u; }
+ZK_HD uint32_t stage_need(uint32_t shift, uint32_t tot) { return 16 * stage_chunks(shift, tot) + 16; }
+
+// stage = 2 * half bytes; an operand's tot is 0 when it is skipped.
+ZK_HD void plan_stage(const uint32_t* tot, const uint32_t* shift, int k, uint32_t half, StagePlan& sp)
+{
+    int g = 0, r = 0;
+    while (r < k) {
+        const uint32_t need0 = stage_need(shift[r], tot[r]);
+        if (need0 > half) {
+            sp.lds[r] = 0;
+            sp.nch[r] = stage_chunks(shift[r], tot[r]);
+            sp.whole[g] = 1;
+            sp.gend[g++] = ++r;
+            continue;
+        }
+        const uint32_t base = (uint32_t) (g & 1) * half;
+        uint32_t pos = 0;
+        while (r < k) {
+            const uint32_t need = stage_need(shift[r], tot[r]);
+            if (need > half || pos + need > half)
+                break;
+            sp.lds[r] = base + pos;
+            sp.nch[r] = stage_chunks(shift[r], tot[r]);
+            pos += need;
+            r++;
+        }
+        sp.whole[g] = 0;
+        sp.gend[g++] = r;
+    }
+    sp.nrounds = g;
+}
+
 // Workspace: decoupled-lookback state.  Two status arrays alternate between
 // consecutive launches.  `seq` holds the parity of the array in use (bit 32)
 // and the next tile ticket (low 32 bits): every workgroup takes its tile

@@ -16,11 +16,12 @@ constexpr uint32_t kCompressThreads = 512;
 constexpr uint32_t kDecompressThreads = 512;
 constexpr uint32_t kHomsumThreads = 256;
 
-// LDS stage area of the homomorphic sum (bytes).  Sized so four 256-thread
-// workgroups share a CU's 160 KB; operands that do not fit are staged in a
-// further round (measured: four workgroups with a second round beat three
-// with one).
-constexpr uint32_t kStageMin = 32 * 1024;
+// LDS stage of the homomorphic sum (bytes): two halves, one being decoded
+// while the next round of operand bytes lands in the other.  With the tables
+// and static state a workgroup needs about 27 KB, so six 256-thread
+// workgroups share a CU's 160 KB (measured: 18 KB beats 24 KB with five
+// workgroups and 32 KB with four).
+constexpr uint32_t kStageMin = 18 * 1024;
 
 struct Operand {
     const uint8_t* bytes;

@@ -680,6 +680,7 @@ struct OperandShared {
     uint32_t tot[kMaxOperands];             // tile bytes
     uint32_t lds[kMaxOperands];             // LDS position of the operand's bytes
     uint32_t bad[kMaxOperands];
+    uint32_t shift[kMaxOperands];           // (stream address of the tile bytes) mod 16
 };
 
 // Read the k operands' widths and offsets for tile x; per-operand scans give
@@ -692,7 +693,9 @@ __device__ __forceinline__ void operand_tables(const Geom& g, const Tiling& tl, 
     const uint32_t tid = threadIdx.x, G = tl.G;
     if (tid < (uint32_t) k) {
         IndexView ix = index_view(g, in[tid].index);
-        os.base[tid] = ix.group_off[x.grp0];
+        const unsigned long long base = ix.group_off[x.grp0];
+        os.base[tid] = base;
+        os.shift[tid] = (uint32_t) ((uintptr_t) (in[tid].bytes + base) & 15u);
         os.size[tid] = ix.meta[0];
         os.seedoff[tid] = x.lt == 0 ? ix.seed_off[x.s] : 0ull;
     }
@@ -827,6 +830,47 @@ __device__ __forceinline__ void load_group(const Operand* in, const OperandShare
     }
 }
 
+// Fetch round g of a staging plan into the stage: every wave takes runs of 64
+// consecutive 16-byte chunks of one operand and moves them from HBM to LDS
+// with direct-to-LDS loads (no registers, completion tracked by the wave's
+// vector memory counter).  Chunks not wholly inside the stream (the first
+// chunk when the aligned start precedes the stream, the last one past the
+// stream's end) are read byte by byte instead, so no byte outside the stream
+// is read.  The caller waits for the counter and synchronises before use.
+__device__ __forceinline__ void issue_round(const Operand* in, const OperandShared& os, const StagePlan& sp, int g,
+    uint8_t* stage)
+{
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int r0 = g ? sp.gend[g - 1] : 0, r1 = sp.gend[g];
+    uint32_t run = 0;
+    for (int r = r0; r < r1; r++) {
+        const uint32_t nch = sp.nch[r];
+        const uint32_t nruns = (nch + 63) / 64;
+        const int64_t start = (int64_t) os.base[r] - (int64_t) os.shift[r];  // stream offset of chunk 0
+        const int64_t size = (int64_t) os.size[r];
+        const uint8_t* bytes = in[r].bytes;
+        for (uint32_t q = 0; q < nruns; q++, run++) {
+            if ((int) (run % (uint32_t) nw) != wid)
+                continue;
+            const uint32_t c = q * 64 + lane;
+            uint8_t* lbase = stage + sp.lds[r] + q * 1024u;
+            if (c >= nch)
+                continue;
+            const int64_t off = start + 16ll * c;
+            if (off >= 0 && off + 16 <= size) {
+                __builtin_amdgcn_global_load_lds((const void*) (bytes + off),
+                    (__attribute__((address_space(3))) void*) lbase, 16, 0, 0);
+            } else {
+                uint32_t t[4] = {0, 0, 0, 0};
+                for (int i = 0; i < 16; i++)
+                    if (off + i >= 0 && off + i < size)
+                        t[i >> 2] |= (uint32_t) bytes[off + i] << (8 * (i & 3));
+                __builtin_memcpy(lbase + 16u * lane, t, 16);
+            }
+        }
+    }
+}
+
 // ------------------------------------------------------------ homomorphic sum
 
 __device__ __forceinline__ void homsum_tile(const Geom& g, const Tiling& tl, const OpParams& p, const HomsumIO& io,
@@ -834,7 +878,6 @@ __device__ __forceinline__ void homsum_tile(const Geom& g, const Tiling& tl, con
 {
     __shared__ TileShared sh;
     __shared__ OperandShared os;
-    __shared__ int s_grp_end;
     const Tile x = tile_of(g, tl, t);
     const uint32_t tid = threadIdx.x;
     uint32_t* stage = zk_lds;
@@ -857,19 +900,30 @@ __device__ __forceinline__ void homsum_tile(const Geom& g, const Tiling& tl, con
 #pragma unroll
         for (int c = 0; c < 4; c++)
             acc[i][c] = 0;
-    for (int r0 = 0; r0 < p.k;) {
-        if (tid == 0)
-            s_grp_end = plan_group(os, r0, p.k, p.stage_bytes);
-        __syncthreads();
-        const int r1 = s_grp_end;
-        load_group(io.in, os, r0, r1, stage);
-        __syncthreads();
-        const uint8_t* sbytes = (const uint8_t*) stage;
-        check_width_bytes(sbytes, os, s_wo, tl.G, x.nblk, r0, r1, ws);
+    __shared__ StagePlan sp;
+    if (tid == 0) {
+        uint32_t tot[kMaxOperands];
+        for (int r = 0; r < p.k; r++)
+            tot[r] = os.bad[r] ? 0u : os.tot[r];
+        plan_stage(tot, os.shift, p.k, p.stage_bytes / 2, sp);
+        for (int r = 0; r < p.k; r++)
+            os.lds[r] = sp.lds[r] + os.shift[r];
+    }
+    __syncthreads();
+    uint8_t* stage8 = (uint8_t*) stage;
+    issue_round(io.in, os, sp, 0, stage8);
+    for (int round = 0; round < sp.nrounds; round++) {
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();  // round `round` is in LDS
+        const bool overlap = round + 1 < sp.nrounds && !sp.whole[round] && !sp.whole[round + 1];
+        if (overlap)
+            issue_round(io.in, os, sp, round + 1, stage8);  // lands in the other half during the decode
+        const int r0 = round ? sp.gend[round - 1] : 0, r1 = sp.gend[round];
+        check_width_bytes(stage8, os, s_wo, tl.G, x.nblk, r0, r1, ws);
         for (int r = r0; r < r1; r++) {
             if (__builtin_amdgcn_readfirstlane(os.bad[r]))
                 continue;
-            const uint8_t* ob = sbytes + __builtin_amdgcn_readfirstlane(os.lds[r]);
+            const uint8_t* ob = stage8 + __builtin_amdgcn_readfirstlane(os.lds[r]);
             const uint32_t* tab = s_wo + r * tl.G;
 #pragma unroll
             for (uint32_t i = 0; i < kUnitsPerThread; i++) {
@@ -878,8 +932,9 @@ __device__ __forceinline__ void homsum_tile(const Geom& g, const Tiling& tl, con
                     accumulate_unit(ob + (e & 0xFFFFu), e >> 16, nj[i], sbj[i], unit_u(un.pk[i]), acc[i]);
             }
         }
-        __syncthreads();
-        r0 = r1;
+        __syncthreads();  // this round's half is free
+        if (round + 1 < sp.nrounds && !overlap)
+            issue_round(io.in, os, sp, round + 1, stage8);
     }
     uint32_t seed = 0;
     if (x.lt == 0 && tid == 0) {
@@ -1317,12 +1372,15 @@ __global__ void k_ws_init(WsHeader* ws, uint64_t capacity)
 
 size_t compress_lds_bytes(const Tiling& tl) { return (size_t) align_up(tl.tile_bytes_max + 64, 16); }
 
-// One operand always fits (worst case); k > 1 operands share at least
-// kStageMin bytes and are staged in groups when their tiles do not fit.
+// Decompress (k = 1): one worst-case tile.  Homomorphic sum: two halves of
+// at least kStageMin / 2 bytes each; the whole stage holds one worst-case
+// operand (aligned chunks plus slack).
 uint32_t stage_bytes_for(const Tiling& tl, int k)
 {
-    uint64_t need = align_up(tl.tile_bytes_max + 64, 16);
-    return (uint32_t) (k > 1 && need < kStageMin ? kStageMin : need);
+    if (k <= 1)
+        return (uint32_t) align_up(tl.tile_bytes_max + 64, 16);
+    uint64_t need = align_up(tl.tile_bytes_max + 48, 32);
+    return (uint32_t) (need < kStageMin ? kStageMin : need);
 }
 
 size_t operand_lds_bytes(const Tiling& tl, int k, uint32_t stage)
