@@ -33,7 +33,7 @@ static uint64_t stage_bytes(const Tiling& tl, int op)
     if (op != 1)
         return align_up(tl.tile_bytes_max + 64, 16);
     uint64_t need = align_up(tl.tile_bytes_max + 48, 32);
-    return need < 18 * 1024 ? 18 * 1024 : need;
+    return need < 17 * 1024 + 512 ? 17 * 1024 + 512 : need;
 }
 
 // Staging plans of the sum: for k operands of any tile size up to the worst

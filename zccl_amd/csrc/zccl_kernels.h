@@ -18,10 +18,11 @@ constexpr uint32_t kHomsumThreads = 256;
 
 // LDS stage of the homomorphic sum (bytes): two halves, one being decoded
 // while the next round of operand bytes lands in the other.  With the tables
-// and static state a workgroup needs about 27 KB, so six 256-thread
-// workgroups share a CU's 160 KB (measured: 18 KB beats 24 KB with five
-// workgroups and 32 KB with four).
-constexpr uint32_t kStageMin = 18 * 1024;
+// and static state a workgroup needs just under 26 KB, so six 256-thread
+// workgroups share a CU's 160 KB at the allocation granularity (measured:
+// 17.5 KB beats 18 KB, which drops to five workgroups, 24 KB with five and
+// 32 KB with four).
+constexpr uint32_t kStageMin = 17 * 1024 + 512;
 
 struct Operand {
     const uint8_t* bytes;
