@@ -97,6 +97,14 @@ static void check_tiling(const Geom& g, uint32_t threads, int op)
         uint64_t hi_other = (par ^ 1u) * cap * kStatusStride + (cap - 1) * kStatusStride;
         REQUIRE(hi_status < status_words && hi_other < status_words, "status words");
     }
+    if (op == 0) {
+        // compress stages the tile's values and the one before them in its
+        // encode buffer (up to 15 bytes of alignment shift in front); a
+        // partial unit reads up to 12 bytes past them
+        const uint64_t reach = 15 + 4ull * ((uint64_t) tl.G * g.L + 1) + 12;
+        REQUIRE(reach <= compress_lds(tl), "compress staging %lu of %lu", (unsigned long) reach,
+            (unsigned long) compress_lds(tl));
+    }
     uint64_t blocks_seen = 0, groups_seen = 0;
     std::vector<uint8_t> seg_seen(g.S, 0);
     for (uint64_t t = 0; t < tl.ntiles; t++) {

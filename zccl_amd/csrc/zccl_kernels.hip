@@ -625,6 +625,37 @@ __device__ __forceinline__ void encode_tile(const Geom& g, const Tiling& tl, con
 
 // ------------------------------------------------------------ compress
 
+// Fetch `bytes` bytes from src (a global address with any 4-byte alignment)
+// into LDS at dst + (src mod 16), as aligned 16-byte chunks moved by
+// direct-to-LDS loads in runs of 64 per wave.  Chunks not wholly inside
+// [lo, hi) are read byte by byte within it.  Returns src mod 16.
+__device__ __forceinline__ uint32_t fetch_range(const uint8_t* src, uint32_t bytes, const uint8_t* lo, const uint8_t* hi,
+    uint8_t* dst)
+{
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const uint32_t shift = (uint32_t) ((uintptr_t) src & 15u);
+    const uint8_t* a = src - shift;
+    const uint32_t nch = (shift + bytes + 15) / 16;
+    const uint32_t nruns = (nch + 63) / 64;
+    for (uint32_t q = wid; q < nruns; q += nw) {
+        const uint32_t c = q * 64 + lane;
+        uint8_t* lbase = dst + q * 1024u;
+        if (c >= nch)
+            continue;
+        const uint8_t* g = a + 16ull * c;
+        if (g >= lo && g + 16 <= hi) {
+            __builtin_amdgcn_global_load_lds((const void*) g, (__attribute__((address_space(3))) void*) lbase, 16, 0, 0);
+        } else {
+            uint32_t t[4] = {0, 0, 0, 0};
+            for (int i = 0; i < 16; i++)
+                if (g + i >= lo && g + i < hi)
+                    t[i >> 2] |= (uint32_t) g[i] << (8 * (i & 3));
+            __builtin_memcpy(lbase + 16u * lane, t, 16);
+        }
+    }
+    return shift;
+}
+
 __device__ __forceinline__ void compress_tile(const Geom& g, const Tiling& tl, const OpParams& p, const CompressIO& io,
     uint64_t t, unsigned long long* status, WsHeader* ws, uint32_t nchains)
 {
@@ -635,26 +666,25 @@ __device__ __forceinline__ void compress_tile(const Geom& g, const Tiling& tl, c
     for (uint32_t j = tid; j < tl.G; j += blockDim.x)
         sh.bmax[j] = 0;
     const Units un = tile_units(g, x);
+    // the tile's values and the one before them, staged in the encode buffer
+    const uint32_t tile_end = x.nblk ? x.e0 + (x.nblk - 1) * g.L + tile_block_len(g, x, x.nblk - 1) : x.e0;
+    const uint32_t shift = fetch_range((const uint8_t*) (io.x + x.e0 - 1), 4u * (tile_end - x.e0 + 1),
+        (const uint8_t*) io.x, (const uint8_t*) (io.x + g.n), (uint8_t*) buf);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    const uint8_t* xs = (const uint8_t*) buf + shift;  // float i of the tile at xs + 4 * (e - (e0 - 1))
     uint32_t d[kUnitsPerThread][4];
 #pragma unroll
     for (uint32_t i = 0; i < kUnitsPerThread; i++) {
         const uint32_t pk = un.pk[i], cnt = unit_cnt(pk);
-        const float* xp = io.x + x.e0 + (uint64_t) unit_j(pk) * g.L + 4 * unit_u(pk);
-        if (cnt == 4) {
-            float4 v = ldf4(xp);
-            d[i][0] = quant(v.x, p.inv); d[i][1] = quant(v.y, p.inv);
-            d[i][2] = quant(v.z, p.inv); d[i][3] = quant(v.w, p.inv);
-        } else {
+        const uint32_t rel = 1 + unit_j(pk) * g.L + 4 * unit_u(pk);  // index of the unit's first value
+        float v[4], pv;
+        __builtin_memcpy(v, xs + 4u * rel, 16);
+        __builtin_memcpy(&pv, xs + 4u * (rel - 1), 4);
+        const uint32_t prev = cnt ? (uint32_t) quant(pv, p.inv) : 0u;
 #pragma unroll
-            for (int c = 0; c < 4; c++)
-                d[i][c] = (uint32_t) c < cnt ? (uint32_t) quant(xp[c], p.inv) : 0u;
-        }
-        // previous value: last value of unit v-1, held by lane-1 (units of a
-        // tile are contiguous in v); lane 0 of each wave reads it
-        uint32_t last = cnt == 4 ? d[i][3] : cnt == 3 ? d[i][2] : cnt == 2 ? d[i][1] : d[i][0];
-        uint32_t prev = __shfl_up(last, 1, 64);
-        if ((tid & 63) == 0 && cnt)
-            prev = (uint32_t) quant(xp[-1], p.inv);
+        for (int c = 0; c < 4; c++)
+            d[i][c] = (uint32_t) c < cnt ? (uint32_t) quant(v[c], p.inv) : 0u;
         // deltas in place (int32 wrap)
 #pragma unroll
         for (int c = 3; c >= 1; c--)
