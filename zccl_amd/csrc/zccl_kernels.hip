@@ -186,17 +186,23 @@ __device__ __forceinline__ void raise_error(WsHeader* ws, unsigned long long bit
 // when it has published just its aggregate.  Every wait is bounded: past
 // kSpinLimit polls the tile records ZK_ERR_SPIN and proceeds, so the grid
 // always drains.
+// First half of the lookback, callable early by any one thread: the tile's
+// aggregate (the chain's first tile: its inclusive value) becomes visible.
+__device__ __forceinline__ void lookback_publish(unsigned long long* status, uint64_t t, uint64_t t0,
+    unsigned long long agg, unsigned long long init)
+{
+    st_publish(&status[t * kStatusStride], t == t0 ? kFlagInc | ((init + agg) & kValMask) : kFlagAgg | (agg & kValMask));
+}
+
+// `published`: lookback_publish already ran for this tile.
 __device__ __forceinline__ unsigned long long lookback(unsigned long long* status, uint64_t t, uint64_t t0,
-    unsigned long long agg, unsigned long long init, WsHeader* ws)
+    unsigned long long agg, unsigned long long init, WsHeader* ws, bool published = false)
 {
     const int lane = threadIdx.x & 63;
-    if (t == t0) {
-        if (lane == 0)
-            st_publish(&status[t * kStatusStride], kFlagInc | ((init + agg) & kValMask));
+    if (!published && lane == 0)
+        lookback_publish(status, t, t0, agg, init);
+    if (t == t0)
         return init;
-    }
-    if (lane == 0)
-        st_publish(&status[t * kStatusStride], kFlagAgg | (agg & kValMask));
     unsigned long long excl = 0;
     int64_t end = (int64_t) t;
     uint32_t spins = 0;
@@ -554,27 +560,22 @@ __device__ __forceinline__ void encode_tile(const Geom& g, const Tiling& tl, con
             if (j < x.nblk)
                 sh.boff[j] = incl - run + loc[c];
         }
-        if (tid == 63)
+        if (tid == 63) {
             sh.bc[0] = incl;
+            lookback_publish(status, t, 0, incl, 0);  // successors see the aggregate now
+        }
     }
     {
         // the other waves (wave 0 too when it is the only one) clear the
         // encode buffer for the largest tile
-        const uint32_t nwords = (tl.tile_bytes_max + 3) / 4 + 8;
+        const uint32_t nvec = (tl.tile_bytes_max + 32 + 15) / 16;
         const uint32_t first = blockDim.x > 64 ? 64u : 0u;
         if (tid >= first)
-            for (uint32_t i = tid - first; i < nwords; i += blockDim.x - first)
-                buf[i] = 0;
+            for (uint32_t i = tid - first; i < nvec; i += blockDim.x - first)
+                reinterpret_cast<uint4*>(buf)[i] = make_uint4(0, 0, 0, 0);
     }
     __syncthreads();
     const uint32_t total = (uint32_t) sh.bc[0];
-    // wave 0 resolves the tile's stream offset while the others encode; the
-    // encoding needs only offsets inside the tile
-    if (tid < 64) {
-        unsigned long long ex = lookback(status, t, 0, total, 0, ws);
-        if (tid == 0)
-            sh.bc[2] = ex;
-    }
 #pragma unroll
     for (uint32_t i = 0; i < kUnitsPerThread; i++) {
         const uint32_t pk = un.pk[i], cnt = unit_cnt(pk);
@@ -587,6 +588,14 @@ __device__ __forceinline__ void encode_tile(const Geom& g, const Tiling& tl, con
     }
     for (uint32_t j = tid; j < x.nblk; j += blockDim.x)
         lds_or4(buf, sh.boff[j], sh.width[j]);
+    // the encoding needs only offsets inside the tile; wave 0 resolves the
+    // tile's stream offset after it, when predecessors have had longer to
+    // publish
+    if (tid < 64) {
+        unsigned long long ex = lookback(status, t, 0, total, 0, ws, true);
+        if (tid == 0)
+            sh.bc[2] = ex;
+    }
     __syncthreads();
     const uint64_t P = 8ull * g.T + 4ull * (x.s + 1) + sh.bc[2];
     const bool fits = P + total <= out_cap;
