@@ -113,6 +113,8 @@ static void check_tiling(const Geom& g, uint32_t threads, int op)
         REQUIRE(x.ngrp >= 1 && x.ngrp <= tl.TG, "ngrp %u", x.ngrp);
         REQUIRE(x.grp0 + x.ngrp <= g.ngroups, "groups");
         REQUIRE(x.g0 + x.nblk <= g.nblocks, "blocks");
+        // widths are read as dwords of 4 blocks: stays inside the index
+        REQUIRE(x.g0 + 4ull * ((x.nblk + 3) / 4) <= g.ix_bytes, "width dwords");
         REQUIRE(x.lt < x.seg_tiles && x.seg_tile0 + x.lt == t, "tile in segment");
         REQUIRE(x.nblk == 0 || (uint64_t) (x.ngrp - 1) * g.IG < x.nblk, "group/block counts");
         if (x.lt == 0)

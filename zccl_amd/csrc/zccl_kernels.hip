@@ -729,50 +729,47 @@ struct OperandShared {
 __device__ __forceinline__ void operand_tables(const Geom& g, const Tiling& tl, const Tile& x, int k, const Operand* in,
     OperandShared& os, uint32_t* s_wo, WsHeader* ws)
 {
-    const uint32_t tid = threadIdx.x, G = tl.G;
-    if (tid < (uint32_t) k) {
-        IndexView ix = index_view(g, in[tid].index);
+    const uint32_t G = tl.G;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    // one wave per operand: its offsets, and the widths of blocks 4*lane ..
+    // 4*lane+3 as one (unaligned) dword; bytes past the tile's blocks are
+    // masked, and stay inside the index (the group offsets follow the widths)
+    for (int r = wid; r < k; r += nw) {
+        const IndexView ix = index_view(g, in[r].index);
         const unsigned long long base = ix.group_off[x.grp0];
-        os.base[tid] = base;
-        os.shift[tid] = (uint32_t) ((uintptr_t) (in[tid].bytes + base) & 15u);
-        os.size[tid] = ix.meta[0];
-        os.seedoff[tid] = x.lt == 0 ? ix.seed_off[x.s] : 0ull;
-    }
-    for (uint32_t idx = tid; idx < (uint32_t) k * G; idx += blockDim.x) {
-        uint32_t r = idx / G, j = idx - r * G;
-        uint32_t w = 0;
-        if (j < x.nblk) {
-            w = in[r].index[x.g0 + j];
+        const unsigned long long size = ix.meta[0];
+        uint32_t wv = 0;
+        if (4u * lane < x.nblk)
+            __builtin_memcpy(&wv, ix.widths + x.g0 + 4u * lane, 4);
+        uint32_t sz[4], wj[4], run = 0;
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const uint32_t j = 4 * lane + c;
+            uint32_t w = j < x.nblk ? (wv >> (8 * c)) & 255u : 0u;
             if (w > 32) {
                 raise_error(ws, ZK_ERR_WIDTH);
                 w = 0;
             }
-        }
-        s_wo[idx] = w;
-    }
-    __syncthreads();
-    const int lane = tid & 63, wid = tid >> 6, nw = blockDim.x >> 6;
-    for (int r = wid; r < k; r += nw) {
-        uint32_t sz[4], wj[4], run = 0;
-#pragma unroll
-        for (int c = 0; c < 4; c++) {
-            uint32_t j = 4 * lane + c;
-            wj[c] = j < G ? s_wo[r * G + j] : 0u;
-            sz[c] = j < x.nblk ? block_bytes(wj[c], tile_block_len(g, x, j)) : 0u;
+            wj[c] = w;
+            sz[c] = j < x.nblk ? block_bytes(w, tile_block_len(g, x, j)) : 0u;
             run += sz[c];
         }
-        uint32_t incl = wave_incl_scan(run);
+        const uint32_t incl = wave_incl_scan(run);
         uint32_t off = incl - run;
 #pragma unroll
         for (int c = 0; c < 4; c++) {
-            uint32_t j = 4 * lane + c;
+            const uint32_t j = 4 * lane + c;
             if (j < G)
                 s_wo[r * G + j] = off | (wj[c] << 16);
             off += sz[c];
         }
         if (lane == 63) {
+            os.base[r] = base;
+            os.shift[r] = (uint32_t) ((uintptr_t) (in[r].bytes + base) & 15u);
+            os.size[r] = size;
+            os.seedoff[r] = x.lt == 0 ? ix.seed_off[x.s] : 0ull;
             os.tot[r] = incl;
-            bool bad = os.base[r] + incl > os.size[r];
+            const bool bad = base + incl > size;
             os.bad[r] = bad;
             if (bad)
                 raise_error(ws, ZK_ERR_OVERRUN);
