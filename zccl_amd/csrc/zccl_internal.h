@@ -234,15 +234,20 @@ struct StagePlan {
 ZK_HD uint32_t stage_chunks(uint32_t shift, uint32_t tot) { return tot ? (shift + tot + 15) / 16 : 0u; }
 ZK_HD uint32_t stage_need(uint32_t shift, uint32_t tot) { return 16 * stage_chunks(shift, tot) + 16; }
 
-// stage = 2 * half bytes; an operand's tot is 0 when it is skipped.
-ZK_HD void plan_stage(const uint32_t* tot, const uint32_t* shift, int k, uint32_t half, StagePlan& sp)
+// stage = 2 * half bytes; an operand's tot is 0 when it is skipped.  op(r)
+// yields operand r's tile bytes and shift (so a device caller can read them
+// from registers); sp.lds / sp.nch / sp.gend / sp.whole are only written.
+template <typename Op>
+ZK_HD void plan_stage_by(Op op, int k, uint32_t half, StagePlan& sp)
 {
     int g = 0, r = 0;
     while (r < k) {
-        const uint32_t need0 = stage_need(shift[r], tot[r]);
+        uint32_t tot, shift;
+        op(r, tot, shift);
+        const uint32_t need0 = stage_need(shift, tot);
         if (need0 > half) {
             sp.lds[r] = 0;
-            sp.nch[r] = stage_chunks(shift[r], tot[r]);
+            sp.nch[r] = stage_chunks(shift, tot);
             sp.whole[g] = 1;
             sp.gend[g++] = ++r;
             continue;
@@ -250,11 +255,12 @@ ZK_HD void plan_stage(const uint32_t* tot, const uint32_t* shift, int k, uint32_
         const uint32_t base = (uint32_t) (g & 1) * half;
         uint32_t pos = 0;
         while (r < k) {
-            const uint32_t need = stage_need(shift[r], tot[r]);
+            op(r, tot, shift);
+            const uint32_t need = stage_need(shift, tot);
             if (need > half || pos + need > half)
                 break;
             sp.lds[r] = base + pos;
-            sp.nch[r] = stage_chunks(shift[r], tot[r]);
+            sp.nch[r] = stage_chunks(shift, tot);
             pos += need;
             r++;
         }
@@ -262,6 +268,98 @@ ZK_HD void plan_stage(const uint32_t* tot, const uint32_t* shift, int k, uint32_
         sp.gend[g++] = r;
     }
     sp.nrounds = g;
+}
+
+ZK_HD void plan_stage(const uint32_t* tot, const uint32_t* shift, int k, uint32_t half, StagePlan& sp)
+{
+    plan_stage_by([&](int r, uint32_t& t, uint32_t& s) { t = tot[r]; s = shift[r]; }, k, half, sp);
+}
+
+// Encoder of one 4-value unit in the slot layout (units of a block in
+// neighbouring lanes).  unit_fields computes, for deltas d[] (two's
+// complement, zero past the block's end) at width w: the magnitudes, the
+// MSB-first sign nibble and the unit's low bits as one 4r-bit field with value
+// 0 on top (r = w & 7).  emit_unit writes the bytes unit u of a block owns
+// (width byte at buf[bo], nj values, cnt of them in this unit, U units in a
+// full block); pnib / plv are the fields of unit u+1 of the same block
+// (ignored when u+1 == U).  Ownership: unit 0 writes the width byte; an even
+// unit writes sign byte u/2 and the r bytes of low bits it shares with unit
+// u+1 (ceil of the bits actually present, zero padded); every unit writes its
+// own high bytes.  Every byte of the block then has exactly one writer, so the
+// buffer needs neither clearing nor atomics.
+// ZCCL/src/ZCCL_TypeManager.c:12-91 (fixed-length bits), :824-848 (signs).
+struct UnitFields {
+    uint32_t m[4];
+    uint32_t nib;
+    uint32_t lv;
+};
+
+ZK_HD uint32_t mag32(uint32_t d) { return (int32_t) d < 0 ? 0u - d : d; }
+
+ZK_HD UnitFields unit_fields(uint32_t w, const uint32_t d[4])
+{
+    UnitFields f;
+    const uint32_t r = w & 7, mask = (1u << r) - 1u;
+    f.nib = 0;
+    f.lv = 0;
+    for (int c = 0; c < 4; c++) {
+        f.m[c] = mag32(d[c]);
+        f.nib |= (d[c] >> 31) << (3 - c);
+        f.lv |= (f.m[c] & mask) << (r * (3 - c));
+    }
+    return f;
+}
+
+// Store the n (< 8) low bytes of v at p, any alignment.
+ZK_HD void put_bytes(uint8_t* p, uint64_t v, uint32_t n)
+{
+    if (n & 4) {
+        const uint32_t lo = (uint32_t) v;
+        __builtin_memcpy(p, &lo, 4);
+        p += 4;
+        v >>= 32;
+    }
+    if (n & 2) {
+        const uint16_t h = (uint16_t) v;
+        __builtin_memcpy(p, &h, 2);
+        p += 2;
+        v >>= 16;
+    }
+    if (n & 1)
+        *p = (uint8_t) v;
+}
+
+ZK_HD void emit_unit(uint8_t* buf, uint32_t bo, uint32_t w, uint32_t nj, uint32_t u, uint32_t U, uint32_t cnt,
+    const UnitFields& f, uint32_t pnib, uint32_t plv)
+{
+    if (!cnt)
+        return;
+    if (u == 0)
+        buf[bo] = (uint8_t) w;
+    if (!w)
+        return;
+    const uint32_t B = w >> 3, r = w & 7;
+    const uint32_t hi0 = bo + 1 + ((nj + 7) >> 3);
+    if ((u & 1) == 0) {
+        const bool has_p = u + 1 < U;
+        const uint32_t up = 4 * (u + 1);
+        const uint32_t pc = has_p && up < nj ? (nj - up < 4 ? nj - up : 4u) : 0u;
+        buf[bo + 1 + (u >> 1)] = (uint8_t) ((f.nib << 4) | (has_p ? pnib : 0u));
+        if (r) {
+            const uint64_t cv = ((uint64_t) f.lv << (4 * r)) | (has_p ? plv : 0u);
+            put_bytes(buf + hi0 + B * nj + (u >> 1) * r, __builtin_bswap64(cv << (64 - 8 * r)), ((cnt + pc) * r + 7) >> 3);
+        }
+    }
+    if (B == 1 && cnt == 4) {
+        const uint32_t h = (f.m[0] >> r) | ((f.m[1] >> r) << 8) | ((f.m[2] >> r) << 16) | ((f.m[3] >> r) << 24);
+        __builtin_memcpy(buf + hi0 + 4 * u, &h, 4);
+    } else if (B) {
+        for (uint32_t c = 0; c < cnt; c++) {
+            const uint32_t h = f.m[c] >> r;
+            for (uint32_t b = 0; b < B; b++)
+                buf[hi0 + (4 * u + c) * B + b] = (uint8_t) (h >> (8 * b));
+        }
+    }
 }
 
 // Workspace: decoupled-lookback state.  Two status arrays alternate between

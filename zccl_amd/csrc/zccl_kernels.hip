@@ -529,6 +529,59 @@ __device__ __forceinline__ void fold_block_max(uint32_t* bmax, const Units& un, 
     }
 }
 
+// Tail shared by every encoder, run by the whole workgroup once the tile's
+// bytes sit in LDS at buf[0, total) and its aggregate is published: wave 0
+// resolves the tile's stream offset (lookback), then the tile is stored, the
+// seed and chunk header written by a segment's first tile, and the output
+// index (widths, group offsets, seed offsets, size) filled in.  SH provides
+// bc[], width[] and boff[] of the tile's blocks.
+template <typename SH>
+__device__ __forceinline__ void finish_tile(const Geom& g, const Tile& x, uint64_t t, SH& sh, const uint32_t* buf,
+    uint32_t total, unsigned long long* status, WsHeader* ws, uint32_t seed, uint8_t* out, uint64_t out_cap,
+    uint8_t* index, unsigned long long* size_out, bool last_tile, uint32_t nchains)
+{
+    const uint32_t tid = threadIdx.x;
+    if (tid < 64) {
+        unsigned long long ex = lookback(status, t, 0, total, 0, ws, true);
+        if (tid == 0)
+            sh.bc[2] = ex;
+    }
+    __syncthreads();
+    const uint64_t P = 8ull * g.T + 4ull * (x.s + 1) + sh.bc[2];
+    const bool fits = P + total <= out_cap;
+    if (fits)
+        store_tile(buf, out + P, total);
+    else if (tid == 0)
+        raise_error(ws, ZK_ERR_NOSPACE);
+    IndexView ix = index_view(g, index);
+    for (uint32_t j = tid; j < x.nblk; j += blockDim.x)
+        ix.widths[x.g0 + j] = sh.width[j];
+    if (tid < x.ngrp)
+        ix.group_off[x.grp0 + tid] = P + (x.nblk ? sh.boff[tid * g.IG] : 0u);
+    if (tid == 0) {
+        if (x.lt == 0) {
+            uint64_t so = P - 4;
+            if (so + 4 <= out_cap) {
+                store_bytes(out + so, &seed, 4);
+                if (x.s < g.T) {
+                    unsigned long long h = so - 8ull * g.T;
+                    store_bytes(out + 8ull * x.s, &h, 8);
+                }
+            }
+            ix.seed_off[x.s] = so;
+        }
+        if (last_tile) {
+            // the last tile ends the stream: if it fits, every tile fits.  A
+            // stream that did not fit records size 0 so no consumer reads it.
+            unsigned long long sz = fits ? P + total : 0ull;
+            ix.meta[0] = sz;
+            if (size_out)
+                *size_out = sz;
+            chain_finished(ws, nchains);
+        }
+    }
+}
+
 // Output side shared by compress and the homomorphic sum: per-block widths
 // from sh.bmax, block offsets, the lookback for the tile's stream offset,
 // encoding into LDS, the tile store, seed / header entry and output index.
@@ -591,45 +644,7 @@ __device__ __forceinline__ void encode_tile(const Geom& g, const Tiling& tl, con
     // the encoding needs only offsets inside the tile; wave 0 resolves the
     // tile's stream offset after it, when predecessors have had longer to
     // publish
-    if (tid < 64) {
-        unsigned long long ex = lookback(status, t, 0, total, 0, ws, true);
-        if (tid == 0)
-            sh.bc[2] = ex;
-    }
-    __syncthreads();
-    const uint64_t P = 8ull * g.T + 4ull * (x.s + 1) + sh.bc[2];
-    const bool fits = P + total <= out_cap;
-    if (fits)
-        store_tile(buf, out + P, total);
-    else if (tid == 0)
-        raise_error(ws, ZK_ERR_NOSPACE);
-    IndexView ix = index_view(g, index);
-    for (uint32_t j = tid; j < x.nblk; j += blockDim.x)
-        ix.widths[x.g0 + j] = sh.width[j];
-    if (tid < x.ngrp)
-        ix.group_off[x.grp0 + tid] = P + (x.nblk ? sh.boff[tid * g.IG] : 0u);
-    if (tid == 0) {
-        if (x.lt == 0) {
-            uint64_t so = P - 4;
-            if (so + 4 <= out_cap) {
-                store_bytes(out + so, &seed, 4);
-                if (x.s < g.T) {
-                    unsigned long long h = so - 8ull * g.T;
-                    store_bytes(out + 8ull * x.s, &h, 8);
-                }
-            }
-            ix.seed_off[x.s] = so;
-        }
-        if (last_tile) {
-            // the last tile ends the stream: if it fits, every tile fits.  A
-            // stream that did not fit records size 0 so no consumer reads it.
-            unsigned long long sz = fits ? P + total : 0ull;
-            ix.meta[0] = sz;
-            if (size_out)
-                *size_out = sz;
-            chain_finished(ws, nchains);
-        }
-    }
+    finish_tile(g, x, t, sh, buf, total, status, ws, seed, out, out_cap, index, size_out, last_tile, nchains);
 }
 
 // ------------------------------------------------------------ compress
@@ -866,6 +881,30 @@ __device__ __forceinline__ void load_group(const Operand* in, const OperandShare
     }
 }
 
+// 32-bit LDS address of a pointer into the workgroup's shared memory.
+__device__ __forceinline__ uint32_t lds_addr(const void* p)
+{
+    return (uint32_t) (uintptr_t) (const __attribute__((address_space(3))) void*) p;
+}
+
+// Direct-to-LDS load of 16 bytes per lane: lane i's bytes from g land at LDS
+// byte lds_base + 16 * i (lds_base is the same for every lane).  Issued from
+// inline assembly on purpose: the compiler models the builtin form as an
+// asynchronous write to all of LDS and puts a full vector-memory wait in
+// front of every later LDS read, which would make each round's decode wait
+// for the next round's prefetch.  The instruction writes no register; the
+// caller waits for the vector-memory counter (s_waitcnt) before it reads
+// the bytes.
+__device__ __forceinline__ void dma16(const void* g, uint32_t lds_base)
+{
+    asm volatile(
+        "s_nop 0\n\t"
+        "global_load_lds_dwordx4 %0, off"
+        :
+        : "v"(g), "{m0}"(__builtin_amdgcn_readfirstlane(lds_base))
+        : "memory");
+}
+
 // Fetch round g of a staging plan into the stage: every wave takes runs of 64
 // consecutive 16-byte chunks of one operand and moves them from HBM to LDS
 // with direct-to-LDS loads (no registers, completion tracked by the wave's
@@ -894,8 +933,7 @@ __device__ __forceinline__ void issue_round(const Operand* in, const OperandShar
                 continue;
             const int64_t off = start + 16ll * c;
             if (off >= 0 && off + 16 <= size) {
-                __builtin_amdgcn_global_load_lds((const void*) (bytes + off),
-                    (__attribute__((address_space(3))) void*) lbase, 16, 0, 0);
+                dma16(bytes + off, lds_addr(lbase));
             } else {
                 uint32_t t[4] = {0, 0, 0, 0};
                 for (int i = 0; i < 16; i++)
@@ -991,6 +1029,326 @@ __device__ __forceinline__ void homsum_tile(const Geom& g, const Tiling& tl, con
     fold_block_max(sh.bmax, un, acc);
     __syncthreads();
     encode_tile(g, tl, x, t, sh, stage, status, ws, un, acc, seed, io.out, io.out_cap, io.index, io.size_out,
+        t == tl.ntiles - 1, nchains);
+}
+
+// ------------------------------------------------------------ homomorphic sum, block length fixed at compile time
+
+// Slot layout: a wave slot is BPS whole blocks with one 4-value unit per
+// lane (lane = U * block-in-slot + unit), so no block straddles two waves
+// and the two units that share a sign byte or a byte of low bits sit in
+// neighbouring lanes.  Each of the 4 waves owns 4 consecutive slots, so a
+// tile holds G = 16 * BPS blocks; with IG = 16 this is the tile that
+// make_tiling(g, 256) cuts (TG = 64 / U groups), so the layout in HBM and the
+// lookback chains are those of the generic sum.
+template <uint32_t L>
+struct SlotMap {
+    static constexpr uint32_t U = (L + 3) / 4;
+    static constexpr uint32_t BPS = 64 / U;
+    static constexpr uint32_t G = 16 * BPS;
+    static constexpr uint32_t SB = (L + 7) / 8;
+    static constexpr uint32_t ACTIVE = BPS * U;
+    static_assert(U >= 4 && U <= 36 && G <= kMaxTileBlocks, "slot layout needs 13 <= L <= 144");
+};
+
+template <uint32_t L>
+struct SlotShared {
+    unsigned long long bc[4];
+    uint32_t bmax[SlotMap<L>::G];
+    uint32_t boff[SlotMap<L>::G];
+    uint8_t width[SlotMap<L>::G];
+};
+
+// Decoder of the slot-layout sum: add the up-to-4 values of unit u of one
+// operand block (width byte at bp, width w > 0, nj values, sb sign bytes) to
+// acc[] as magnitude XOR sign mask; the +1 that completes the two's
+// complement of each negative value is counted in negs instead, one byte per
+// value (byte 3 - c counts value c; at most 32 operands, so no byte
+// overflows), and added once after all operands.  u4 = 4u, uh = u >> 1 and
+// nsh = (u & 1) ? 0 : 4 are the lane's unit constants.  Same fields and reads
+// as accumulate_unit: ZCCL/src/ZCCL_TypeManager.c:423-517 and :906-950.
+__device__ __forceinline__ void accumulate_slot(const uint8_t* bp, uint32_t w, uint32_t nj, uint32_t sb,
+    uint32_t u4, uint32_t uh, uint32_t nsh, uint32_t acc[4], uint32_t& negs)
+{
+    const uint32_t B = w >> 3, r = w & 7;
+    const uint32_t nib = __builtin_amdgcn_ubfe((uint32_t) bp[1 + uh], nsh, 4);
+    negs += __umul24(nib, 0x00204081u) & 0x01010101u;  // bit 3 - c of nib -> byte 3 - c
+    const uint8_t* hp = bp + 1 + sb;
+    uint32_t hs[4];
+    if (B <= 1) {
+        const uint32_t H = lds_ld4(hp + __umul24(u4, B));
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+            hs[c] = __builtin_amdgcn_ubfe(H, 8 * c, 8 * B) << r;
+    } else if (B == 2) {
+        const uint2 H = lds_ld8(hp + 2 * u4);
+        hs[0] = (H.x & 0xFFFFu) << r;
+        hs[1] = (H.x >> 16) << r;
+        hs[2] = (H.y & 0xFFFFu) << r;
+        hs[3] = (H.y >> 16) << r;
+    } else {
+        const uint4 H = lds_ld16(hp + __umul24(u4, B));
+        if (B == 3) {
+            hs[0] = (H.x & 0xFFFFFFu) << r;
+            hs[1] = (__builtin_amdgcn_alignbyte(H.y, H.x, 3) & 0xFFFFFFu) << r;
+            hs[2] = (__builtin_amdgcn_alignbyte(H.z, H.y, 2) & 0xFFFFFFu) << r;
+            hs[3] = (H.z >> 8) << r;
+        } else {
+            hs[0] = H.x; hs[1] = H.y; hs[2] = H.z; hs[3] = H.w;
+        }
+    }
+    const uint32_t bitpos = __umul24(u4, r);
+    const uint32_t X = __builtin_bswap32(lds_ld4(hp + __umul24(B, nj) + (bitpos >> 3))) << (bitpos & 7);
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const uint32_t lo = __builtin_amdgcn_ubfe(X, 32 - r * (c + 1), r);
+        const uint32_t sm = (uint32_t) __builtin_amdgcn_sbfe((int32_t) nib, 3 - c, 1);  // 0 or ~0
+        acc[c] += (hs[c] | lo) ^ sm;
+    }
+}
+
+// Add one operand's values to NB of the lane's slots.  The reads are batched
+// so a wave waits for LDS twice per batch rather than twice per slot: the NB
+// table entries, then the sign byte, high-byte word and low-bit word of every
+// slot, then the arithmetic (accumulate_slot's, branch-free for widths below
+// 16; a slot with no unit or width 0 adds nothing).  Reads of a dead slot land
+// inside the operand's staged bytes or its 16 bytes of slack.
+template <uint32_t NB>
+__device__ __forceinline__ void decode_slots(const uint8_t* ob, const uint32_t* tab, const uint32_t* jb,
+    const uint32_t* cnt, const uint32_t* nj, uint32_t u4, uint32_t uh, uint32_t nsh,
+    uint32_t (*acc)[4], uint32_t* negs)
+{
+    uint32_t e[NB];
+#pragma unroll
+    for (uint32_t i = 0; i < NB; i++)
+        e[i] = tab[jb[i]];
+    uint32_t sg[NB], H[NB], X[NB], w[NB], bit[NB];
+    const uint8_t* hp[NB];
+#pragma unroll
+    for (uint32_t i = 0; i < NB; i++) {
+        w[i] = cnt[i] ? e[i] >> 16 : 0u;
+        const uint8_t* bp = ob + (e[i] & 0xFFFFu);
+        const uint32_t B = w[i] >> 3, r = w[i] & 7;
+        hp[i] = bp + 1 + ((nj[i] + 7) >> 3);
+        bit[i] = __umul24(u4, r);
+        sg[i] = bp[1 + uh];
+        H[i] = lds_ld4(hp[i] + __umul24(u4, B));
+        X[i] = lds_ld4(hp[i] + __umul24(B, nj[i]) + (bit[i] >> 3));
+    }
+#pragma unroll
+    for (uint32_t i = 0; i < NB; i++) {
+        const uint32_t B = w[i] >> 3, r = w[i] & 7;
+        const uint32_t nib = w[i] ? __builtin_amdgcn_ubfe(sg[i], nsh, 4) : 0u;
+        negs[i] += __umul24(nib, 0x00204081u) & 0x01010101u;
+        uint32_t hs[4];
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+            hs[c] = __builtin_amdgcn_ubfe(H[i], 8 * c, 8 * B) << r;
+        if (B >= 2) {
+            // widths of 16 and more: two to four high bytes per value
+            if (B == 2) {
+                const uint2 h2 = lds_ld8(hp[i] + 2 * u4);
+                hs[0] = (h2.x & 0xFFFFu) << r;
+                hs[1] = (h2.x >> 16) << r;
+                hs[2] = (h2.y & 0xFFFFu) << r;
+                hs[3] = (h2.y >> 16) << r;
+            } else {
+                const uint4 h4 = lds_ld16(hp[i] + __umul24(u4, B));
+                if (B == 3) {
+                    hs[0] = (h4.x & 0xFFFFFFu) << r;
+                    hs[1] = (__builtin_amdgcn_alignbyte(h4.y, h4.x, 3) & 0xFFFFFFu) << r;
+                    hs[2] = (__builtin_amdgcn_alignbyte(h4.z, h4.y, 2) & 0xFFFFFFu) << r;
+                    hs[3] = (h4.z >> 8) << r;
+                } else {
+                    hs[0] = h4.x; hs[1] = h4.y; hs[2] = h4.z; hs[3] = h4.w;
+                }
+            }
+        }
+        const uint32_t Xs = __builtin_bswap32(X[i]) << (bit[i] & 7);
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const uint32_t lo = __builtin_amdgcn_ubfe(Xs, 32 - r * (c + 1), r);
+            const uint32_t sm = (uint32_t) __builtin_amdgcn_sbfe((int32_t) nib, 3 - c, 1);
+            acc[i][c] += (hs[c] | lo) ^ sm;
+        }
+    }
+}
+
+// Add one operand's values to the lane's four slots, one slot at a time:
+// batching the reads of several slots (decode_slots<NB>) takes registers
+// that would cost a workgroup per CU, and residency hides more latency.
+__device__ __forceinline__ void decode_operand(const uint8_t* ob, const uint32_t* tab, const uint32_t (&jb)[4],
+    const uint32_t (&cnt)[4], const uint32_t (&nj)[4], uint32_t u4, uint32_t uh, uint32_t nsh,
+    uint32_t (&acc)[4][4], uint32_t (&negs)[4])
+{
+#pragma unroll
+    for (uint32_t i = 0; i < 4; i++) {
+        const uint32_t e = tab[jb[i]];
+        const uint32_t w = cnt[i] ? e >> 16 : 0u;
+        if (w)
+            accumulate_slot(ob + (e & 0xFFFFu), w, nj[i], (nj[i] + 7) >> 3, u4, uh, nsh, acc[i], negs[i]);
+    }
+}
+
+// Encode this lane's unit in the slot layout (emit_unit in zccl_internal.h
+// holds the byte ownership rules); the next lane's fields arrive by a lane
+// shuffle, so every lane of the wave calls this.
+template <uint32_t U>
+__device__ __forceinline__ void encode_unit_pair(uint8_t* buf, uint32_t bo, uint32_t w, uint32_t nj, uint32_t u,
+    uint32_t cnt, const uint32_t d[4])
+{
+    const UnitFields f = unit_fields(w, d);
+    const uint32_t pnib = __shfl_down(f.nib, 1, 64);
+    const uint32_t plv = __shfl_down(f.lv, 1, 64);
+    emit_unit(buf, bo, w, nj, u, U, cnt, f, pnib, plv);
+}
+
+template <uint32_t L>
+__device__ __forceinline__ void homsum_tile_slots(const Geom& g, const Tiling& tl, const OpParams& p, const HomsumIO& io,
+    uint64_t t, unsigned long long* status, WsHeader* ws, uint32_t nchains)
+{
+    using M = SlotMap<L>;
+    constexpr uint32_t G = M::G;
+    __shared__ SlotShared<L> sh;
+    __shared__ OperandShared os;
+    __shared__ StagePlan sp;
+    const Tile x = tile_of(g, tl, t);
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    uint32_t* stage = zk_lds;
+    uint32_t* s_wo = zk_lds + p.stage_bytes / 4;
+    uint8_t* stage8 = (uint8_t*) stage;
+    // unit of this lane inside its block, the same in all four slots
+    const uint32_t jj = lane / M::U, u = lane - jj * M::U;
+    const uint32_t u4 = 4 * u, uh = u >> 1, nsh = (u & 1) ? 0u : 4u;
+    const bool on = lane < M::ACTIVE;
+    const uint32_t last_len = x.nblk ? tile_block_len(g, x, x.nblk - 1) : 0u;
+    uint32_t jb[4], nj[4], cnt[4];
+#pragma unroll
+    for (uint32_t i = 0; i < 4; i++) {
+        const uint32_t j = (wv * 4 + i) * M::BPS + jj;
+        const bool live = on && j < x.nblk;
+        jb[i] = live ? j : 0u;
+        nj[i] = live ? (j == x.nblk - 1 ? last_len : L) : 0u;
+        cnt[i] = 4 * u < nj[i] ? min(4u, nj[i] - 4 * u) : 0u;
+    }
+    for (uint32_t j = tid; j < G; j += blockDim.x)
+        sh.bmax[j] = 0;
+    operand_tables(g, tl, x, p.k, io.in, os, s_wo, ws);
+    uint32_t acc[4][4], negs[4];
+#pragma unroll
+    for (uint32_t i = 0; i < 4; i++) {
+        negs[i] = 0;
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+            acc[i][c] = 0;
+    }
+    // operand r's tile bytes and shift in lane r of every wave: the plan (wave
+    // 0) and the decode read them with readlane instead of LDS round trips
+    const uint32_t op_tot = lane < (uint32_t) p.k && !os.bad[lane] ? os.tot[lane] : 0u;
+    const uint32_t op_shift = lane < (uint32_t) p.k ? os.shift[lane] : 0u;
+    if (tid == 0)
+        plan_stage_by([&](int r, uint32_t& tt, uint32_t& sh) {
+            tt = __builtin_amdgcn_readlane(op_tot, r);
+            sh = __builtin_amdgcn_readlane(op_shift, r);
+        }, p.k, p.stage_bytes / 2, sp);
+    __syncthreads();
+    if (lane < (uint32_t) p.k)
+        os.lds[lane] = sp.lds[lane] + op_shift;
+    // every wave wrote the same values; the staging reads them after this barrier
+    __syncthreads();
+    const uint32_t op_lds = lane < (uint32_t) p.k ? sp.lds[lane] + op_shift : 0u;
+    const uint32_t op_bad = lane < (uint32_t) p.k ? os.bad[lane] : 1u;
+    issue_round(io.in, os, sp, 0, stage8);
+    for (int round = 0; round < sp.nrounds; round++) {
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();  // round `round` is in LDS
+        const bool overlap = round + 1 < sp.nrounds && !sp.whole[round] && !sp.whole[round + 1];
+        if (overlap)
+            issue_round(io.in, os, sp, round + 1, stage8);  // lands in the other half during the decode
+        const int r0 = round ? sp.gend[round - 1] : 0, r1 = sp.gend[round];
+        // every staged block's width byte must equal the index width
+        bool bad = false;
+        for (int r = r0 + (int) wv; r < r1; r += 4) {
+            if (os.bad[r])
+                continue;
+            for (uint32_t j = lane; j < x.nblk; j += 64) {
+                const uint32_t e = s_wo[r * G + j];
+                bad |= stage8[os.lds[r] + (e & 0xFFFFu)] != (e >> 16);
+            }
+        }
+        if (bad)
+            raise_error(ws, ZK_ERR_MISMATCH);
+        for (int r = r0; r < r1; r++) {
+            if (__builtin_amdgcn_readlane(op_bad, r))
+                continue;
+            decode_operand(stage8 + __builtin_amdgcn_readlane(op_lds, r), s_wo + r * G, jb, cnt, nj, u4, uh, nsh,
+                acc, negs);
+        }
+        __syncthreads();  // this round's half is free
+        if (round + 1 < sp.nrounds && !overlap)
+            issue_round(io.in, os, sp, round + 1, stage8);
+    }
+    uint32_t seed = 0;
+    if (x.lt == 0 && tid == 0) {
+        for (int r = 0; r < p.k; r++) {
+            uint64_t so = os.seedoff[r];
+            if (so + 4 <= os.size[r])
+                seed += load_u32_bytes(io.in[r].bytes + so);
+            else
+                raise_error(ws, ZK_ERR_OVERRUN);
+        }
+    }
+    // complete the negative values' two's complement; values past the block's
+    // end decoded garbage: clear them; then block maxima
+#pragma unroll
+    for (uint32_t i = 0; i < 4; i++) {
+        uint32_t umax = 0;
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            acc[i][c] += (negs[i] >> (8 * (3 - c))) & 0xFFu;
+            if ((uint32_t) c >= cnt[i])
+                acc[i][c] = 0;
+            umax = max(umax, umag((int32_t) acc[i][c]));
+        }
+        if (umax)
+            atomicMax(&sh.bmax[jb[i]], umax);
+    }
+    __syncthreads();
+    if (tid < 64) {
+        // wave 0: widths, sizes and offsets of the tile's blocks with one wave scan
+        constexpr uint32_t CPT = (G + 63) / 64;
+        uint32_t loc[CPT];
+        uint32_t run = 0;
+#pragma unroll
+        for (uint32_t c = 0; c < CPT; c++) {
+            const uint32_t j = tid * CPT + c;
+            loc[c] = run;
+            if (j < x.nblk) {
+                const uint32_t w = dwidth(sh.bmax[j]);
+                sh.width[j] = (uint8_t) w;
+                run += block_bytes(w, j == x.nblk - 1 ? last_len : L);
+            }
+        }
+        const uint32_t incl = wave_incl_scan(run);
+#pragma unroll
+        for (uint32_t c = 0; c < CPT; c++) {
+            const uint32_t j = tid * CPT + c;
+            if (j < x.nblk)
+                sh.boff[j] = incl - run + loc[c];
+        }
+        if (tid == 63) {
+            sh.bc[0] = incl;
+            lookback_publish(status, t, 0, incl, 0);  // successors see the aggregate now
+        }
+    }
+    __syncthreads();
+    const uint32_t total = (uint32_t) sh.bc[0];
+#pragma unroll
+    for (uint32_t i = 0; i < 4; i++) {
+        const uint32_t j = jb[i];
+        encode_unit_pair<M::U>(stage8, sh.boff[j], cnt[i] ? sh.width[j] : 0u, nj[i], u, cnt[i], acc[i]);
+    }
+    finish_tile(g, x, t, sh, stage, total, status, ws, seed, io.out, io.out_cap, io.index, io.size_out,
         t == tl.ntiles - 1, nchains);
 }
 
@@ -1126,6 +1484,18 @@ __global__ void __launch_bounds__(kHomsumThreads, 4) k_homsum(HomsumArgs a)
     homsum_tile(a.job.g, a.job.tl, a.job.p, a.job.io, t, status, a.ws, 1);
 }
 
+// Sums whose blocks hold kSlotL values and whose tiles match the slot layout.
+constexpr uint32_t kSlotL = 36;
+
+__global__ void __launch_bounds__(kHomsumThreads, 4) k_homsum_slots(HomsumArgs a)
+{
+    uint64_t t;
+    unsigned long long* status;
+    if (!launch_status(a.ws, a.job.tl.ntiles, t, status))
+        return;
+    homsum_tile_slots<kSlotL>(a.job.g, a.job.tl, a.job.p, a.job.io, t, status, a.ws, 1);
+}
+
 __global__ void __launch_bounds__(kDecompressThreads) k_decompress(DecompressArgs a)
 {
     uint64_t t;
@@ -1186,6 +1556,20 @@ __global__ void __launch_bounds__(kHomsumThreads, 4) k_homsum_batch(BatchArgs b)
     job_of(b, tg, jidx, t);
     stage_desc(b.jobs, jidx, &sio);
     homsum_tile(b.g, b.tl, b.p, sio, t, status + b.job_tile0[jidx] * kStatusStride, b.ws, b.total_chains);
+}
+
+__global__ void __launch_bounds__(kHomsumThreads, 4) k_homsum_batch_slots(BatchArgs b)
+{
+    __shared__ HomsumIO sio;
+    unsigned long long* status;
+    uint64_t tg;
+    if (!launch_status(b.ws, b.total_tiles, tg, status))
+        return;
+    uint32_t jidx;
+    uint64_t t;
+    job_of(b, tg, jidx, t);
+    stage_desc(b.jobs, jidx, &sio);
+    homsum_tile_slots<kSlotL>(b.g, b.tl, b.p, sio, t, status + b.job_tile0[jidx] * kStatusStride, b.ws, b.total_chains);
 }
 
 __global__ void __launch_bounds__(kCompressThreads) k_compress_batch(BatchArgs b)
@@ -1431,10 +1815,20 @@ hipError_t launch_compress(const CompressArgs& a, hipStream_t s)
     return hipGetLastError();
 }
 
+// The slot-layout sum serves the reference's default block length when the
+// tiling is the one its layout assumes (256 threads, 16-block index groups).
+static bool slot_layout(const Geom& g, const Tiling& tl)
+{
+    return g.L == kSlotL && g.IG == 16 && tl.threads == kHomsumThreads && tl.G == SlotMap<kSlotL>::G;
+}
+
 hipError_t launch_homsum(const HomsumArgs& a, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_homsum, dim3((uint32_t) a.job.tl.ntiles), dim3(a.job.tl.threads),
-        operand_lds_bytes(a.job.tl, a.job.p.k, a.job.p.stage_bytes), s, a);
+    const size_t lds = operand_lds_bytes(a.job.tl, a.job.p.k, a.job.p.stage_bytes);
+    if (slot_layout(a.job.g, a.job.tl))
+        hipLaunchKernelGGL(k_homsum_slots, dim3((uint32_t) a.job.tl.ntiles), dim3(a.job.tl.threads), lds, s, a);
+    else
+        hipLaunchKernelGGL(k_homsum, dim3((uint32_t) a.job.tl.ntiles), dim3(a.job.tl.threads), lds, s, a);
     return hipGetLastError();
 }
 
@@ -1456,7 +1850,9 @@ hipError_t launch_batch(BatchKind kind, const BatchArgs& b, uint32_t threads, si
             hipLaunchKernelGGL(k_compress_mixed, grid, block, lds, s, b);
         break;
     case BatchKind::Homsum:
-        if (b.uniform)
+        if (b.uniform && slot_layout(b.g, b.tl) && threads == kHomsumThreads)
+            hipLaunchKernelGGL(k_homsum_batch_slots, grid, block, lds, s, b);
+        else if (b.uniform)
             hipLaunchKernelGGL(k_homsum_batch, grid, block, lds, s, b);
         else
             hipLaunchKernelGGL(k_homsum_mixed, grid, block, lds, s, b);
