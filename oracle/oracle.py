@@ -93,6 +93,21 @@ def lib():
         L.zo_homsum.restype = ctypes.c_int64
         L.zo_parse.argtypes = [vp, u64, u64, i32, i32, vp, vp, vp, vp]
         L.zo_parse.restype = i32
+        f64 = ctypes.c_double
+        L.zo_dims_create.argtypes = [i32, vp]
+        L.zo_dims_create.restype = None
+        L.zo_partition3d.argtypes = [u64, u64, u64, i32, vp]
+        L.zo_partition3d.restype = i32
+        L.zo_quantize3d.argtypes = [vp, u64, f64, vp]
+        L.zo_quantize3d.restype = None
+        L.zo_dequantize3d.argtypes = [vp, u64, f64, vp]
+        L.zo_dequantize3d.restype = None
+        L.zo_worst_case_bytes3d.argtypes = [u64, u64, u64, i32, i32]
+        L.zo_worst_case_bytes3d.restype = u64
+        L.zo_compress3d.argtypes = [vp, u64, u64, u64, i32, f64, i32, vp, u64]
+        L.zo_compress3d.restype = ctypes.c_int64
+        L.zo_decompress3d.argtypes = [vp, u64, u64, u64, u64, i32, f64, i32, vp]
+        L.zo_decompress3d.restype = i32
         _LIB = L
     return _LIB
 
@@ -211,6 +226,56 @@ def parse(stream: np.ndarray, n: int, L: int, T: int):
     return w, bo, so, int(tot.value)
 
 
+# ------------------------------------------------------------------- 3-D codec
+
+def dims_create(T: int):
+    d = np.zeros(3, dtype=np.int32)
+    lib().zo_dims_create(T, _ptr(d))
+    return tuple(int(v) for v in d)
+
+
+def partition3d(dims, T: int) -> np.ndarray:
+    """Sub-volumes (ox, oy, oz, sx, sy, sz) of the T-way split, rows in thread order."""
+    p = np.zeros((T, 6), dtype=np.uint64)
+    _check(lib().zo_partition3d(dims[0], dims[1], dims[2], T, _ptr(p)), "partition3d")
+    return p.astype(np.int64)
+
+
+def quantize3d(x: np.ndarray, eb: float) -> np.ndarray:
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    q = np.empty(x.shape, dtype=np.int32)
+    lib().zo_quantize3d(_ptr(x), x.size, eb, _ptr(q))
+    return q
+
+
+def dequantize3d(q: np.ndarray, eb: float) -> np.ndarray:
+    q = np.ascontiguousarray(q, dtype=np.int32)
+    x = np.empty(q.shape, dtype=np.float32)
+    lib().zo_dequantize3d(_ptr(q), q.size, eb, _ptr(x))
+    return x
+
+
+def worst_case_bytes3d(dims, Bs: int, T: int) -> int:
+    return int(lib().zo_worst_case_bytes3d(dims[0], dims[1], dims[2], Bs, T))
+
+
+def compress3d(x: np.ndarray, dims, Bs: int, eb: float, T: int) -> np.ndarray:
+    x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1)
+    cap = worst_case_bytes3d(dims, Bs, T)
+    if cap == 0:
+        raise OracleError("invalid 3-D geometry")
+    out = np.empty(cap, dtype=np.uint8)
+    n = _check(lib().zo_compress3d(_ptr(x), dims[0], dims[1], dims[2], Bs, eb, T, _ptr(out), cap), "compress3d")
+    return out[:n].copy()
+
+
+def decompress3d(stream: np.ndarray, dims, Bs: int, eb: float, T: int) -> np.ndarray:
+    s = np.ascontiguousarray(stream, dtype=np.uint8)
+    x = np.empty(int(dims[0]) * int(dims[1]) * int(dims[2]), dtype=np.float32)
+    _check(lib().zo_decompress3d(_ptr(s), s.size, dims[0], dims[1], dims[2], Bs, eb, T, _ptr(x)), "decompress3d")
+    return x
+
+
 # ----------------------------------------------------------------- reference
 
 def ref_path() -> str:
@@ -240,6 +305,10 @@ def ref():
         R.ZCCL_float_homomophic_add_openmp_threadblock.restype = None
         R.ZCCL_float_homomophic_add_single_thread.argtypes = [vp, vp, sz, f32, i32, vp, vp]
         R.ZCCL_float_homomophic_add_single_thread.restype = None
+        R.SZp_compress3D_fast_openmp.argtypes = [vp, vp, sz, sz, sz, i32, ctypes.c_double, vp]
+        R.SZp_compress3D_fast_openmp.restype = None
+        R.SZp_decompress3D_fast_openmp.argtypes = [vp, vp, sz, sz, sz, i32, ctypes.c_double]
+        R.SZp_decompress3D_fast_openmp.restype = None
         _GOMP = ctypes.CDLL("libgomp.so.1")
         _GOMP.omp_set_num_threads.argtypes = [ctypes.c_int]
         _GOMP.omp_get_max_threads.restype = ctypes.c_int
@@ -349,3 +418,34 @@ def ref_add2(a: np.ndarray, b: np.ndarray, n: int, L: int, T: int, single_thread
     fn(_ptr(out), ctypes.byref(size), n, ctypes.c_float(0.0), L, _ptr(pa), _ptr(pb))
     m = size.value - initial_size
     return out[:m].copy(), size.value
+
+
+def ref_compress3d(x: np.ndarray, dims, Bs: int, eb: float, T: int) -> np.ndarray:
+    """Reference SZp_compress3D_fast_openmp with an OpenMP team of T.
+
+    Its per-thread scratch holds ceil(4n/T) bytes (ZCCL/src/ZCCL_float.c:1896,
+    1948); callers here keep every sub-volume's payload below that (the oracle's
+    stream is checked first) so the reference never writes past it."""
+    x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1)
+    mine = compress3d(x, dims, Bs, eb, T)
+    offs = np.frombuffer(mine[: 8 * T].tobytes(), dtype=np.uint64).astype(np.int64)
+    sizes = np.append(offs[1:], mine.size - 8 * T) - offs
+    if int(sizes.max()) > _usable_bytes((4 * x.size + T - 1) // T):
+        raise OracleError("reference per-thread buffer would overflow for this input")
+    R = ref()
+    _set_T(T)
+    out = np.zeros(max(worst_case_bytes3d(dims, Bs, T), 4 * x.size) + 64, dtype=np.uint8)
+    size = ctypes.c_size_t(0)
+    R.SZp_compress3D_fast_openmp(_ptr(x), _ptr(out), dims[0], dims[1], dims[2], Bs, eb, ctypes.byref(size))
+    return out[: size.value].copy()
+
+
+def ref_decompress3d(stream: np.ndarray, dims, Bs: int, eb: float, T: int) -> np.ndarray:
+    s = np.ascontiguousarray(stream, dtype=np.uint8)
+    buf = np.zeros(s.size + 256, dtype=np.uint8)  # the unpackers read a byte past a block
+    buf[: s.size] = s
+    R = ref()
+    _set_T(T)
+    x = np.zeros(int(dims[0]) * int(dims[1]) * int(dims[2]), dtype=np.float32)
+    R.SZp_decompress3D_fast_openmp(_ptr(x), _ptr(buf), dims[0], dims[1], dims[2], Bs, eb)
+    return x

@@ -31,6 +31,7 @@
 #include "zccl_oracle.h"
 
 #include <limits.h>
+#include <math.h>
 #include <stdlib.h>
 #include <string.h>
 #ifdef _OPENMP
@@ -184,19 +185,11 @@ static inline uint32_t mag32(int32_t d) { return d < 0 ? (uint32_t) 0 - (uint32_
  * bytes per element, element-major little-endian bytes of |d_j|>>r
  * (:34-47); then the low r = b&7 bits of every |d_j| as one MSB-first bit
  * stream, zero-padded to a byte (:48-88 and the r-bit packers :118-421). */
-uint64_t zo_encode_block(const int32_t *d, uint32_t n, uint8_t *out)
+/* Sign bitmap, high bytes and low bits of n deltas at width b > 0, written
+ * at p (the block without its width byte); returns the bytes written. */
+static uint64_t pack_fields(const int32_t *d, uint32_t n, int b, uint8_t *p0)
 {
-    uint32_t m = 0;
-    for (uint32_t j = 0; j < n; j++) {
-        uint32_t v = mag32(d[j]);
-        if (v > m)
-            m = v;
-    }
-    int b = zo_width(m);
-    out[0] = (uint8_t) b;
-    if (b == 0)
-        return 1;
-    uint8_t *p = out + 1;
+    uint8_t *p = p0;
     uint32_t sb = (n + 7) / 8;
     memset(p, 0, sb);
     for (uint32_t j = 0; j < n; j++)
@@ -227,27 +220,31 @@ uint64_t zo_encode_block(const int32_t *d, uint32_t n, uint8_t *out)
         if (nbits > 0)
             *p++ = (uint8_t) (acc << (8 - nbits));
     }
-    return (uint64_t) (p - out);
+    return (uint64_t) (p - p0);
+}
+
+uint64_t zo_encode_block(const int32_t *d, uint32_t n, uint8_t *out)
+{
+    uint32_t m = 0;
+    for (uint32_t j = 0; j < n; j++) {
+        uint32_t v = mag32(d[j]);
+        if (v > m)
+            m = v;
+    }
+    int b = zo_width(m);
+    out[0] = (uint8_t) b;
+    if (b == 0)
+        return 1;
+    return 1 + pack_fields(d, n, b, out + 1);
 }
 
 /* Inverse of zo_encode_block (ZCCL/src/ZCCL_TypeManager.c:423-517, 906-950;
  * sign applied as 0 - |d| in unsigned arithmetic, ZCCL/src/ZCCLd_float.c:617-623).
  * Returns bytes consumed or ZO_ECORRUPT. */
-int64_t zo_decode_block(const uint8_t *in, uint64_t avail, uint32_t n, int32_t *d)
+/* Inverse of pack_fields: n deltas at width b > 0 from p. */
+static void unpack_fields(const uint8_t *p, uint32_t n, int b, int32_t *d)
 {
-    if (avail < 1)
-        return ZO_ECORRUPT;
-    int b = in[0];
-    if (b == 0) {
-        memset(d, 0, (size_t) n * sizeof(int32_t));
-        return 1;
-    }
-    if (b > 32)
-        return ZO_ECORRUPT;
-    uint64_t sz = zo_block_bytes(b, n);
-    if (sz > avail)
-        return ZO_ECORRUPT;
-    const uint8_t *sg = in + 1;
+    const uint8_t *sg = p;
     const uint8_t *hi = sg + (n + 7) / 8;
     int B = b >> 3, r = b & 7;
     const uint8_t *lo = hi + (uint64_t) B * n;
@@ -266,6 +263,23 @@ int64_t zo_decode_block(const uint8_t *in, uint64_t avail, uint32_t n, int32_t *
         int neg = (sg[j >> 3] >> (7 - (j & 7))) & 1;
         d[j] = neg ? (int32_t) ((uint32_t) 0 - v) : (int32_t) v;
     }
+}
+
+int64_t zo_decode_block(const uint8_t *in, uint64_t avail, uint32_t n, int32_t *d)
+{
+    if (avail < 1)
+        return ZO_ECORRUPT;
+    int b = in[0];
+    if (b == 0) {
+        memset(d, 0, (size_t) n * sizeof(int32_t));
+        return 1;
+    }
+    if (b > 32)
+        return ZO_ECORRUPT;
+    uint64_t sz = zo_block_bytes(b, n);
+    if (sz > avail)
+        return ZO_ECORRUPT;
+    unpack_fields(in + 1, n, b, d);
     return (int64_t) sz;
 }
 
@@ -587,4 +601,292 @@ int64_t zo_homsum(const uint8_t *const *ins, const uint64_t *in_sizes, int k, ui
     free(segbuf);
     free(segsz);
     return res;
+}
+
+/* ------------------------------------------------------------ 3-D SZp codec */
+
+/* Factors T into 3 dims the way My_Dims_create does (ZCCL/src/ZCCL_multidim.c:
+ * 100-140): the prime factors of T, largest first, each multiplied into the
+ * currently smallest dim (the first one on ties); then the dims are sorted in
+ * descending order. */
+void zo_dims_create(int T, int dims[3])
+{
+    int f[64], nf = 0, rem = T;
+    dims[0] = dims[1] = dims[2] = 1;
+    for (int i = 2; i <= rem; i++)
+        while (rem % i == 0 && nf < 64) {
+            f[nf++] = i;
+            rem /= i;
+        }
+    for (int a = 0; a < nf; a++)
+        for (int b = a + 1; b < nf; b++)
+            if (f[b] > f[a]) {
+                int t = f[a];
+                f[a] = f[b];
+                f[b] = t;
+            }
+    for (int i = 0; i < nf; i++) {
+        int m = 0;
+        for (int j = 1; j < 3; j++)
+            if (dims[j] < dims[m])
+                m = j;
+        dims[m] *= f[i];
+    }
+    for (int a = 0; a < 3; a++)
+        for (int b = a + 1; b < 3; b++)
+            if (dims[b] > dims[a]) {
+                int t = dims[a];
+                dims[a] = dims[b];
+                dims[b] = t;
+            }
+}
+
+/* Sub-volume t of the T-way split: the first (d mod td) parts along an axis get
+ * one more plane; parts are numbered x-major (ZCCL/src/ZCCL_float.c:1901-1935).
+ * ZO_EINVAL when a part would be empty. */
+int zo_partition3d(uint64_t d1, uint64_t d2, uint64_t d3, int T, zo_part3 *parts)
+{
+    int td[3];
+    if (T < 1 || d1 < 1 || d2 < 1 || d3 < 1)
+        return ZO_EINVAL;
+    zo_dims_create(T, td);
+    if (d1 < (uint64_t) td[0] || d2 < (uint64_t) td[1] || d3 < (uint64_t) td[2])
+        return ZO_EINVAL;
+    uint64_t bx = d1 / td[0], rx = d1 % td[0], by = d2 / td[1], ry = d2 % td[1], bz = d3 / td[2], rz = d3 % td[2];
+    int r = 0;
+    uint64_t ox = 0;
+    for (int px = 0; px < td[0]; px++) {
+        uint64_t sx = bx + ((uint64_t) px < rx ? 1 : 0), oy = 0;
+        for (int py = 0; py < td[1]; py++) {
+            uint64_t sy = by + ((uint64_t) py < ry ? 1 : 0), oz = 0;
+            for (int pz = 0; pz < td[2]; pz++) {
+                uint64_t sz = bz + ((uint64_t) pz < rz ? 1 : 0);
+                zo_part3 p = {ox, oy, oz, sx, sy, sz};
+                parts[r++] = p;
+                oz += sz;
+            }
+            oy += sy;
+        }
+        ox += sx;
+    }
+    return ZO_OK;
+}
+
+/* q = (int)floor((double)x * (0.5 / eb) + 0.5), product and sum rounded
+ * separately (SZ_quantize, ZCCL/include/ZCCL_float.h:57-61; the reference is
+ * built without FMA).  Out-of-range and NaN give INT_MIN as x86 cvttsd2si. */
+static inline int32_t quant3(float x, double inv)
+{
+    double p = (double) x * inv;
+    p = floor(p + 0.5);
+    if (p >= -2147483648.0 && p < 2147483648.0)
+        return (int32_t) p;
+    return INT32_MIN;
+}
+
+void zo_quantize3d(const float *x, uint64_t n, double eb, int32_t *q)
+{
+    double inv = 0.5 / eb;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t) n; i++)
+        q[i] = quant3(x[i], inv);
+}
+
+/* x' = (float)((double)q * (2 * eb)) (ZCCL/src/ZCCLd_float.c:2885, 3030). */
+void zo_dequantize3d(const int32_t *q, uint64_t n, double eb, float *x)
+{
+    double tw = 2 * eb;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t) n; i++)
+        x[i] = (float) ((double) q[i] * tw);
+}
+
+static uint64_t nblk(uint64_t s, int Bs) { return (s - 1) / (uint64_t) Bs + 1; }
+
+uint64_t zo_worst_case_bytes3d(uint64_t d1, uint64_t d2, uint64_t d3, int Bs, int T)
+{
+    zo_part3 *parts = (zo_part3 *) malloc(sizeof(zo_part3) * (size_t) (T > 0 ? T : 1));
+    if (Bs < 1 || zo_partition3d(d1, d2, d3, T, parts) != ZO_OK) {
+        free(parts);
+        return 0;
+    }
+    uint64_t tot = 8ull * T;
+    for (int t = 0; t < T; t++) {
+        const zo_part3 *p = &parts[t];
+        uint64_t n = p->sx * p->sy * p->sz;
+        tot += nblk(p->sx, Bs) * nblk(p->sy, Bs) * nblk(p->sz, Bs);  /* width bytes */
+        tot += 4 * n;                                                /* B = 4 high bytes */
+        /* sign bytes: at most one partial byte per block beyond n/8 */
+        tot += n / 8 + nblk(p->sx, Bs) * nblk(p->sy, Bs) * nblk(p->sz, Bs);
+    }
+    free(parts);
+    return tot;
+}
+
+/* Residual of sub-volume p at local (lx, ly, lz): the 3-D difference
+ * dx dy dz q with q = 0 outside the sub-volume.  The reference forms it with
+ * running buffers (prevLeft, prevRow, prevSlice; ZCCL/src/ZCCL_float.c:
+ * 1967-1997): err_1 = q - q(z-1), err_2 = err_1 - err_1(y-1),
+ * err_3 = err_2 - err_2(x-1), in 32-bit wrap-around arithmetic. */
+static int32_t resid3(const int32_t *Q, uint64_t d2, uint64_t d3, const zo_part3 *p, uint64_t lx, uint64_t ly,
+    uint64_t lz)
+{
+    uint32_t acc = 0;
+    for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++)
+            for (int c = 0; c < 2; c++) {
+                if ((a && lx == 0) || (b && ly == 0) || (c && lz == 0))
+                    continue;
+                uint64_t X = p->ox + lx - a, Y = p->oy + ly - b, Z = p->oz + lz - c;
+                uint32_t v = (uint32_t) Q[(X * d2 + Y) * d3 + Z];
+                acc += ((a + b + c) & 1) ? 0u - v : v;
+            }
+    return (int32_t) acc;
+}
+
+/* SZp_compress3D_fast_openmp (ZCCL/src/ZCCL_float.c:1862-2048) with a team of
+ * T: header of T u64 payload offsets (relative to the end of the header),
+ * then per sub-volume the width byte of every block (32 - clz(max |r|), 0
+ * when all residuals are 0) followed by the sign bitmap and fixed-length bits
+ * of every block of nonzero width, blocks in x, y, z order and values in
+ * (i, j, k) order inside a block. */
+int64_t zo_compress3d(const float *x, uint64_t d1, uint64_t d2, uint64_t d3, int Bs, double eb, int T,
+    uint8_t *out, uint64_t cap)
+{
+    if (Bs < 1)
+        return ZO_EINVAL;
+    zo_part3 *parts = (zo_part3 *) malloc(sizeof(zo_part3) * (size_t) (T > 0 ? T : 1));
+    if (zo_partition3d(d1, d2, d3, T, parts) != ZO_OK) {
+        free(parts);
+        return ZO_EINVAL;
+    }
+    uint64_t need = zo_worst_case_bytes3d(d1, d2, d3, Bs, T);
+    uint64_t n = d1 * d2 * d3;
+    int32_t *Q = (int32_t *) malloc(sizeof(int32_t) * n);
+    int32_t *res = (int32_t *) malloc(sizeof(int32_t) * (size_t) Bs * Bs * Bs);
+    uint8_t *tmp = (uint8_t *) malloc(need);
+    zo_quantize3d(x, n, eb, Q);
+    uint64_t pos = 8ull * T;
+    for (int t = 0; t < T; t++) {
+        const zo_part3 *p = &parts[t];
+        uint64_t b1 = nblk(p->sx, Bs), b2 = nblk(p->sy, Bs), b3 = nblk(p->sz, Bs);
+        uint8_t *wid = tmp + pos;
+        uint64_t q = pos + b1 * b2 * b3;
+        uint64_t bi = 0;
+        for (uint64_t bx = 0; bx < b1; bx++)
+            for (uint64_t by = 0; by < b2; by++)
+                for (uint64_t bz = 0; bz < b3; bz++, bi++) {
+                    uint64_t ex = p->sx - bx * Bs < (uint64_t) Bs ? p->sx - bx * Bs : (uint64_t) Bs;
+                    uint64_t ey = p->sy - by * Bs < (uint64_t) Bs ? p->sy - by * Bs : (uint64_t) Bs;
+                    uint64_t ez = p->sz - bz * Bs < (uint64_t) Bs ? p->sz - bz * Bs : (uint64_t) Bs;
+                    uint32_t m = 0, cnt = 0;
+                    for (uint64_t i = 0; i < ex; i++)
+                        for (uint64_t j = 0; j < ey; j++)
+                            for (uint64_t k = 0; k < ez; k++) {
+                                int32_t r = resid3(Q, d2, d3, p, bx * Bs + i, by * Bs + j, bz * Bs + k);
+                                res[cnt++] = r;
+                                if (mag32(r) > m)
+                                    m = mag32(r);
+                            }
+                    int w = m ? 32 - __builtin_clz(m) : 0;
+                    wid[bi] = (uint8_t) w;
+                    if (w)
+                        q += pack_fields(res, cnt, w, tmp + q);
+                }
+        uint64_t off = pos - 8ull * T;
+        memcpy(tmp + 8ull * t, &off, 8);
+        pos = q;
+    }
+    int64_t ret = (int64_t) pos;
+    if (pos > cap)
+        ret = ZO_ENOSPC;
+    else
+        memcpy(out, tmp, pos);
+    free(tmp);
+    free(res);
+    free(Q);
+    free(parts);
+    return ret;
+}
+
+/* SZp_decompress3D_fast_openmp (ZCCL/src/ZCCLd_float.c:2873-3040): residuals
+ * of every block, then per sub-volume the inclusive prefix sum over x, y and z
+ * (the reference's colSum / row_acc / prefix buffers, :2996-3027), then
+ * x' = (float)((double)q * 2eb). */
+int zo_decompress3d(const uint8_t *in, uint64_t in_size, uint64_t d1, uint64_t d2, uint64_t d3, int Bs,
+    double eb, int T, float *x)
+{
+    if (Bs < 1)
+        return ZO_EINVAL;
+    zo_part3 *parts = (zo_part3 *) malloc(sizeof(zo_part3) * (size_t) (T > 0 ? T : 1));
+    if (zo_partition3d(d1, d2, d3, T, parts) != ZO_OK || in_size < 8ull * T) {
+        free(parts);
+        return ZO_EINVAL;
+    }
+    int rc = ZO_OK;
+    int32_t *res = (int32_t *) malloc(sizeof(int32_t) * (size_t) Bs * Bs * Bs);
+    double tw = 2 * eb;
+    for (int t = 0; t < T && rc == ZO_OK; t++) {
+        const zo_part3 *p = &parts[t];
+        uint64_t off;
+        memcpy(&off, in + 8ull * t, 8);
+        uint64_t b1 = nblk(p->sx, Bs), b2 = nblk(p->sy, Bs), b3 = nblk(p->sz, Bs), nb = b1 * b2 * b3;
+        uint64_t pos = 8ull * T + off;
+        if (pos + nb > in_size) {
+            rc = ZO_ECORRUPT;
+            break;
+        }
+        const uint8_t *wid = in + pos;
+        pos += nb;
+        uint64_t nsub = p->sx * p->sy * p->sz;
+        uint32_t *R = (uint32_t *) calloc(nsub, sizeof(uint32_t));
+        uint64_t bi = 0;
+        for (uint64_t bx = 0; bx < b1 && rc == ZO_OK; bx++)
+            for (uint64_t by = 0; by < b2 && rc == ZO_OK; by++)
+                for (uint64_t bz = 0; bz < b3; bz++, bi++) {
+                    int w = wid[bi];
+                    if (!w)
+                        continue;
+                    uint64_t ex = p->sx - bx * Bs < (uint64_t) Bs ? p->sx - bx * Bs : (uint64_t) Bs;
+                    uint64_t ey = p->sy - by * Bs < (uint64_t) Bs ? p->sy - by * Bs : (uint64_t) Bs;
+                    uint64_t ez = p->sz - bz * Bs < (uint64_t) Bs ? p->sz - bz * Bs : (uint64_t) Bs;
+                    uint32_t cnt = (uint32_t) (ex * ey * ez);
+                    uint64_t sz = zo_block_bytes(w, cnt) - 1;
+                    if (w > 32 || pos + sz > in_size) {
+                        rc = ZO_ECORRUPT;
+                        break;
+                    }
+                    unpack_fields(in + pos, cnt, w, res);
+                    pos += sz;
+                    uint32_t c = 0;
+                    for (uint64_t i = 0; i < ex; i++)
+                        for (uint64_t j = 0; j < ey; j++)
+                            for (uint64_t k = 0; k < ez; k++)
+                                R[((bx * Bs + i) * p->sy + by * Bs + j) * p->sz + bz * Bs + k] = (uint32_t) res[c++];
+                }
+        if (rc == ZO_OK) {
+            /* inclusive prefix sums along z, then y, then x (wrap-around) */
+            for (uint64_t i = 0; i < p->sx; i++)
+                for (uint64_t j = 0; j < p->sy; j++)
+                    for (uint64_t k = 1; k < p->sz; k++)
+                        R[(i * p->sy + j) * p->sz + k] += R[(i * p->sy + j) * p->sz + k - 1];
+            for (uint64_t i = 0; i < p->sx; i++)
+                for (uint64_t j = 1; j < p->sy; j++)
+                    for (uint64_t k = 0; k < p->sz; k++)
+                        R[(i * p->sy + j) * p->sz + k] += R[(i * p->sy + j - 1) * p->sz + k];
+            for (uint64_t i = 1; i < p->sx; i++)
+                for (uint64_t j = 0; j < p->sy; j++)
+                    for (uint64_t k = 0; k < p->sz; k++)
+                        R[(i * p->sy + j) * p->sz + k] += R[((i - 1) * p->sy + j) * p->sz + k];
+            for (uint64_t i = 0; i < p->sx; i++)
+                for (uint64_t j = 0; j < p->sy; j++)
+                    for (uint64_t k = 0; k < p->sz; k++)
+                        x[((p->ox + i) * d2 + p->oy + j) * d3 + p->oz + k] =
+                            (float) ((double) (int32_t) R[(i * p->sy + j) * p->sz + k] * tw);
+        }
+        free(R);
+    }
+    free(res);
+    free(parts);
+    return rc;
 }

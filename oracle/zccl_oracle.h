@@ -81,6 +81,24 @@ int zo_parse(const uint8_t *in, uint64_t in_size, uint64_t n, int L, int T, uint
 /* Naive reference encoders used only to pin the packers: one bit at a time. */
 uint64_t zo_encode_block_bitwise(const int32_t *d, uint32_t n, uint8_t *out);
 
+/* 3-D SZp codec (SZp_compress3D_fast_openmp / SZp_decompress3D_fast_openmp):
+ * dims d1 (slowest) x d2 x d3, cubic blocks of side Bs, absolute bound eb,
+ * a T-way split of the volume (sub-volume t compressed by thread t). */
+typedef struct zo_part3 {
+    uint64_t ox, oy, oz;  /* first plane of the sub-volume along each axis */
+    uint64_t sx, sy, sz;  /* planes along each axis */
+} zo_part3;
+
+void zo_dims_create(int T, int dims[3]);
+int zo_partition3d(uint64_t d1, uint64_t d2, uint64_t d3, int T, zo_part3 *parts);
+void zo_quantize3d(const float *x, uint64_t n, double eb, int32_t *q);
+void zo_dequantize3d(const int32_t *q, uint64_t n, double eb, float *x);
+uint64_t zo_worst_case_bytes3d(uint64_t d1, uint64_t d2, uint64_t d3, int Bs, int T);
+int64_t zo_compress3d(const float *x, uint64_t d1, uint64_t d2, uint64_t d3, int Bs, double eb, int T,
+    uint8_t *out, uint64_t cap);
+int zo_decompress3d(const uint8_t *in, uint64_t in_size, uint64_t d1, uint64_t d2, uint64_t d3, int Bs,
+    double eb, int T, float *x);
+
 #ifdef __cplusplus
 }
 #endif

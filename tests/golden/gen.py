@@ -134,3 +134,20 @@ def perturb(x: np.ndarray, seed: int) -> np.ndarray:
     mask = np.repeat(rng.random((x.size + 255) // 256) < 0.6, 256)[: x.size]
     y[mask & finite] += noise[mask & finite]
     return y.astype(np.float32)
+
+
+def smooth3d(dims, seed: int) -> np.ndarray:
+    """3-D field smooth along every axis: separable random sinusoids plus
+    N(0, 1e-3) noise, float32, C order (dims[0] slowest)."""
+    rng = np.random.default_rng(seed)
+    x = np.zeros(dims, dtype=np.float64)
+    for _ in range(6):
+        f = rng.uniform(0.02, 0.3, 3)
+        ph = rng.uniform(0, 2 * np.pi, 3)
+        a = rng.uniform(0.2, 1.0)
+        gx = np.sin(2 * np.pi * f[0] * np.arange(dims[0]) + ph[0])[:, None, None]
+        gy = np.sin(2 * np.pi * f[1] * np.arange(dims[1]) + ph[1])[None, :, None]
+        gz = np.sin(2 * np.pi * f[2] * np.arange(dims[2]) + ph[2])[None, None, :]
+        x += a * gx * gy * gz
+    x += rng.normal(0.0, 1e-3, dims)
+    return np.ascontiguousarray(x.astype(np.float32))
