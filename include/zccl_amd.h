@@ -215,6 +215,37 @@ int zccl_tune_get_threads(int op);
 int zccl_case_count_dev(const void *index1, const void *index2, size_t nbEle, int blockSize,
     int nchunks, unsigned long long *d_counts4, void *stream);
 
+/* ------------------------------------------------------------ 3-D SZp codec
+ *
+ * Volume dim1 x dim2 x dim3 (dim3 fastest), cubic blocks of side
+ * blockSideLength (1..64), absolute bound errorBound, split over the OpenMP
+ * team size T as the reference splits it over its threads (the drop-ins take
+ * T from omp_get_max_threads(), or zccl_set_num_chunks()). */
+
+/* ZCCL/include/ZCCL_float.h:98-101 (ZCCL/src/ZCCL_float.c:1862-2048).
+ * *cmpSize is set to the stream size; oriData and cmpData may be host or
+ * device pointers. */
+void SZp_compress3D_fast_openmp(const float *oriData, unsigned char *cmpData, size_t dim1, size_t dim2,
+    size_t dim3, int blockSideLength, double errorBound, size_t *cmpSize);
+
+/* ZCCL/include/ZCCLd_float.h:45-48 (ZCCL/src/ZCCLd_float.c:2873-3040). */
+void SZp_decompress3D_fast_openmp(float *decData, unsigned char *cmpData, size_t dim1, size_t dim2,
+    size_t dim3, int blockSideLength, double errorBound);
+
+/* Worst-case stream bytes and device workspace bytes of one 3-D call. */
+int zccl_sz3d_sizes(size_t dim1, size_t dim2, size_t dim3, int blockSideLength, int nchunks,
+    size_t *worst_bytes, size_t *ws_bytes);
+/* Device API: device pointers, asynchronous on `stream`.  The stream size is
+ * written to *d_size (device, optional) and to the workspace's status, which
+ * zccl_sz3d_status() reads (blocking) together with the sticky error bits. */
+int zccl_sz3d_compress_dev(const float *x, size_t dim1, size_t dim2, size_t dim3, int blockSideLength,
+    double errorBound, int nchunks, unsigned char *out, size_t out_cap, void *ws, size_t ws_bytes,
+    unsigned long long *d_size, void *stream);
+int zccl_sz3d_decompress_dev(const unsigned char *in, size_t in_bytes, size_t dim1, size_t dim2,
+    size_t dim3, int blockSideLength, double errorBound, int nchunks, float *out, void *ws,
+    size_t ws_bytes, void *stream);
+int zccl_sz3d_status(const void *ws, size_t *stream_bytes);
+
 #ifdef __cplusplus
 }
 #endif

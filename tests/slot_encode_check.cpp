@@ -67,7 +67,9 @@ static void encode_slots(const std::vector<int32_t>& d, uint32_t nj, uint32_t U,
         // __shfl_down(x, 1) hands each lane the next lane's value
         const uint32_t pnib = u + 1 < U ? f[u + 1].nib : 0xFFFFFFFFu;
         const uint32_t plv = u + 1 < U ? f[u + 1].lv : 0xFFFFFFFFu;
-        emit_unit(buf, bo, w, nj, u, U, cnt[u], f[u], pnib, plv);
+        if (u == 0 && cnt[u])
+            buf[bo] = (uint8_t) w;  // the kernel's unit-0 lane writes the width byte
+        emit_unit(buf, bo + 1, w, nj, u, U, cnt[u], f[u], pnib, plv);
     }
 }
 

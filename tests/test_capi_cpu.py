@@ -17,7 +17,7 @@ def declared_symbols():
     text = open(os.path.join(ROOT, "include", "zccl_amd.h")).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     text = re.sub(r"^#.*$", "", text, flags=re.M)
-    return sorted(set(re.findall(r"\b((?:zccl|ZCCL)_\w+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b((?:zccl|ZCCL|SZp)_\w+)\s*\(", text)))
 
 
 def test_library_loads_and_exports_every_declared_symbol():
@@ -53,3 +53,21 @@ def test_chunk_count_defaults_to_openmp_team_and_can_be_overridden():
     assert lib.zccl_get_num_chunks() == 5
     lib.zccl_set_num_chunks(0)
     assert lib.zccl_get_num_chunks() == gomp.omp_get_max_threads()
+
+
+@pytest.mark.parametrize("dims,Bs,T", [((16, 16, 16), 4, 8), ((13, 11, 17), 5, 3), ((512, 512, 512), 4, 16),
+                                       ((40, 1, 1), 4, 2), ((9, 8, 10), 3, 12)])
+def test_3d_sizes_agree_with_oracle(dims, Bs, T):
+    lib = _lib.lib()
+    worst, ws = ctypes.c_size_t(0), ctypes.c_size_t(0)
+    assert lib.zccl_sz3d_sizes(*dims, Bs, T, ctypes.byref(worst), ctypes.byref(ws)) == 0
+    assert worst.value == O.worst_case_bytes3d(dims, Bs, T)
+    assert ws.value > 0
+
+
+def test_3d_invalid_geometry_rejected():
+    lib = _lib.lib()
+    worst = ctypes.c_size_t(0)
+    assert lib.zccl_sz3d_sizes(2, 2, 2, 4, 16, ctypes.byref(worst), None) != 0   # empty sub-volume
+    assert lib.zccl_sz3d_sizes(8, 8, 8, 0, 1, ctypes.byref(worst), None) != 0    # block side < 1
+    assert lib.zccl_sz3d_sizes(8, 8, 8, 65, 1, ctypes.byref(worst), None) != 0   # block side > 64

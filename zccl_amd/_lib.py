@@ -41,6 +41,8 @@ EXPORTS = (
     "zccl_batch_homsum_plan", "zccl_batch_compress_plan", "zccl_batch_decompress_plan",
     "zccl_batch_desc_bytes", "zccl_batch_desc_host", "zccl_batch_bind",
     "zccl_tune_threads", "zccl_tune_get_threads",
+    "SZp_compress3D_fast_openmp", "SZp_decompress3D_fast_openmp",
+    "zccl_sz3d_sizes", "zccl_sz3d_compress_dev", "zccl_sz3d_decompress_dev", "zccl_sz3d_status",
 )
 
 
@@ -111,6 +113,12 @@ def lib() -> ctypes.CDLL:
         "zccl_batch_desc_bytes": ([vp], sz),
         "zccl_batch_desc_host": ([vp], vp),
         "zccl_batch_bind": ([vp, vp], i32),
+        "SZp_compress3D_fast_openmp": ([vp, vp, sz, sz, sz, i32, ctypes.c_double, sz_p], None),
+        "SZp_decompress3D_fast_openmp": ([vp, vp, sz, sz, sz, i32, ctypes.c_double], None),
+        "zccl_sz3d_sizes": ([sz, sz, sz, i32, i32, sz_p, sz_p], i32),
+        "zccl_sz3d_compress_dev": ([vp, sz, sz, sz, i32, ctypes.c_double, i32, vp, sz, vp, sz, vp, vp], i32),
+        "zccl_sz3d_decompress_dev": ([vp, sz, sz, sz, sz, i32, ctypes.c_double, i32, vp, vp, sz, vp], i32),
+        "zccl_sz3d_status": ([vp, sz_p], i32),
         "zccl_tune_threads": ([i32, i32], i32),
         "zccl_tune_get_threads": ([i32], i32),
     }

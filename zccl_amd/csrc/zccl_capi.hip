@@ -1016,3 +1016,266 @@ extern "C" int zccl_homomorphic_add_sized(unsigned char* final_cmpBytes, size_t 
     return set_last(dropin_add(final_cmpBytes, final_cap, true, final_size, false, nbEle, blockSize, nchunks,
         cmpBytes, cmpSize, cmpBytes2, cmpSize2, true, nullptr));
 }
+
+// ------------------------------------------------------------ 3-D SZp codec
+
+static bool geom3_of(size_t d1, size_t d2, size_t d3, int Bs, int T, Geom3* g, std::vector<Part3>* parts)
+{
+    if (T < 1 || T > kMaxParts3)
+        return false;
+    parts->resize((size_t) T);
+    return make_geom3(d1, d2, d3, Bs, T, g, parts->data());
+}
+
+extern "C" int zccl_sz3d_sizes(size_t d1, size_t d2, size_t d3, int Bs, int T, size_t* worst_bytes,
+    size_t* ws_bytes)
+{
+    Geom3 g;
+    std::vector<Part3> parts;
+    if (!geom3_of(d1, d2, d3, Bs, T, &g, &parts))
+        return set_last(ZCCL_EINVAL);
+    if (worst_bytes)
+        *worst_bytes = (size_t) worst_case_bytes3(g, parts.data());
+    if (ws_bytes)
+        *ws_bytes = (size_t) ws3_layout(g.nblocks, g.T).bytes;
+    return set_last(ZCCL_OK);
+}
+
+static int sz3_args(size_t d1, size_t d2, size_t d3, int Bs, int T, void* ws, size_t ws_bytes, Sz3Args* a,
+    hipStream_t s)
+{
+    std::vector<Part3> parts;
+    if (!ws || !geom3_of(d1, d2, d3, Bs, T, &a->g, &parts))
+        return ZCCL_EINVAL;
+    const Ws3 L = ws3_layout(a->g.nblocks, a->g.T);
+    if (ws_bytes < L.bytes)
+        return ZCCL_EWORKSPACE;
+    char* w = (char*) ws;
+    a->status = (unsigned long long*) w;
+    a->words = (uint32_t*) (w + L.words);
+    a->offs = (unsigned long long*) (w + L.offs);
+    a->part_sums = (unsigned long long*) (w + L.sums);
+    a->parts = (const Part3*) (w + L.parts);
+    HIP_TRY(launch_sz3_table(a->g, (Part3*) a->parts, a->status, s));
+    return ZCCL_OK;
+}
+
+extern "C" int zccl_sz3d_compress_dev(const float* x, size_t d1, size_t d2, size_t d3, int Bs, double eb, int T,
+    unsigned char* out, size_t out_cap, void* ws, size_t ws_bytes, unsigned long long* d_size, void* stream)
+{
+    if (!x || !out || !(eb > 0))
+        return set_last(ZCCL_EINVAL);
+    Sz3Args a = {};
+    int rc = sz3_args(d1, d2, d3, Bs, T, ws, ws_bytes, &a, (hipStream_t) stream);
+    if (rc)
+        return set_last(rc);
+    a.x = x;
+    a.out = out;
+    a.out_cap = out_cap;
+    a.inv = 0.5 / eb;  // SZp_compress3D_fast_openmp's inver_eb (ZCCL/src/ZCCL_float.c:1869)
+    a.size_out = d_size;
+    if (launch_sz3_compress(a, (hipStream_t) stream) != hipSuccess) {
+        (void) hipGetLastError();
+        return set_last(ZCCL_EDEVICE);
+    }
+    return set_last(ZCCL_OK);
+}
+
+extern "C" int zccl_sz3d_decompress_dev(const unsigned char* in, size_t in_bytes, size_t d1, size_t d2, size_t d3,
+    int Bs, double eb, int T, float* out, void* ws, size_t ws_bytes, void* stream)
+{
+    if (!in || !out || !(eb > 0) || in_bytes < 8ull * (T > 0 ? T : 1))
+        return set_last(ZCCL_EINVAL);
+    Sz3Args a = {};
+    int rc = sz3_args(d1, d2, d3, Bs, T, ws, ws_bytes, &a, (hipStream_t) stream);
+    if (rc)
+        return set_last(rc);
+    a.in = in;
+    a.in_size = in_bytes;
+    a.y = out;
+    a.tw = 2 * eb;  // twice_eb (ZCCL/src/ZCCLd_float.c:2885)
+    if (launch_sz3_decompress(a, (hipStream_t) stream) != hipSuccess) {
+        (void) hipGetLastError();
+        return set_last(ZCCL_EDEVICE);
+    }
+    return set_last(ZCCL_OK);
+}
+
+extern "C" int zccl_sz3d_status(const void* ws, size_t* stream_bytes)
+{
+    if (!ws)
+        return set_last(ZCCL_EINVAL);
+    unsigned long long st[2] = {0, 0};
+    if (hipMemcpy(st, ws, sizeof(st), hipMemcpyDeviceToHost) != hipSuccess) {
+        (void) hipGetLastError();
+        return set_last(ZCCL_EDEVICE);
+    }
+    if (stream_bytes)
+        *stream_bytes = (size_t) st[0];
+    return set_last(map_errors(st[1]));
+}
+
+namespace {
+
+struct Ctx3 {
+    std::mutex mu;
+    hipStream_t stream = nullptr;
+    DevBuf x, out, ws;
+};
+
+Ctx3* ctx3_current(int* rc)
+{
+    static std::mutex mu;
+    static std::vector<Ctx3*> ctxs;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) {
+        (void) hipGetLastError();
+        *rc = ZCCL_EDEVICE;
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lk(mu);
+    if ((int) ctxs.size() <= dev)
+        ctxs.resize(dev + 1, nullptr);
+    if (!ctxs[dev]) {
+        Ctx3* c = new Ctx3();
+        if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+            (void) hipGetLastError();
+            delete c;
+            *rc = ZCCL_EDEVICE;
+            return nullptr;
+        }
+        ctxs[dev] = c;
+    }
+    *rc = ZCCL_OK;
+    return ctxs[dev];
+}
+
+int dropin_compress3(const float* x, unsigned char* out, size_t d1, size_t d2, size_t d3, int Bs, double eb,
+    size_t* size)
+{
+    if (!x || !out || !size)
+        return ZCCL_EINVAL;
+    *size = 0;  // ZCCL/src/ZCCL_float.c:1881
+    const int T = zccl_get_num_chunks();
+    Geom3 g;
+    std::vector<Part3> parts;
+    if (!geom3_of(d1, d2, d3, Bs, T, &g, &parts))
+        return ZCCL_EINVAL;
+    const uint64_t n = (uint64_t) d1 * d2 * d3, worst = worst_case_bytes3(g, parts.data());
+    int rc;
+    Ctx3* c = ctx3_current(&rc);
+    if (!c)
+        return rc;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if ((rc = ensure(c->ws, ws3_layout(g.nblocks, g.T).bytes)))
+        return rc;
+    size_t avail_x = 0, avail_o = 0;
+    const bool xdev = device_ptr(x, &avail_x), odev = device_ptr(out, &avail_o);
+    const float* dx = x;
+    if (!xdev) {
+        if ((rc = ensure(c->x, 4 * n)))
+            return rc;
+        HIP_TRY(hipMemcpyAsync(c->x.p, x, 4 * n, hipMemcpyHostToDevice, c->stream));
+        dx = (const float*) c->x.p;
+    } else if (avail_x < 4 * n) {
+        return ZCCL_EINVAL;
+    }
+    unsigned char* dout = out;
+    size_t cap = avail_o;
+    if (!odev) {
+        if ((rc = ensure(c->out, worst)))
+            return rc;
+        dout = (unsigned char*) c->out.p;
+        cap = worst;
+    }
+    if ((rc = zccl_sz3d_compress_dev(dx, d1, d2, d3, Bs, eb, T, dout, cap, c->ws.p, c->ws.cap, nullptr, c->stream)))
+        return rc;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    size_t bytes = 0;
+    if ((rc = zccl_sz3d_status(c->ws.p, &bytes)))
+        return rc;
+    if (!odev)
+        HIP_TRY(hipMemcpy(out, dout, bytes, hipMemcpyDeviceToHost));
+    *size = bytes;
+    return ZCCL_OK;
+}
+
+// Bytes of a host stream: the last sub-volume's offset from the header, then
+// its width bytes give its payload size (ZCCL/src/ZCCLd_float.c:2946-2951).
+uint64_t host_stream_bytes3(const unsigned char* in, const Geom3& g, const std::vector<Part3>& parts)
+{
+    const Part3& p = parts[g.T - 1];
+    uint64_t off;
+    memcpy(&off, in + 8ull * (g.T - 1), 8);
+    const uint64_t w0 = 8ull * g.T + off, nb = (uint64_t) p.b1 * p.b2 * p.b3;
+    uint64_t tot = w0 + nb, lb = 0;
+    for (uint32_t bx = 0; bx < p.b1; bx++)
+        for (uint32_t by = 0; by < p.b2; by++)
+            for (uint32_t bz = 0; bz < p.b3; bz++, lb++) {
+                const uint32_t ex = std::min(g.Bs, p.sx - bx * g.Bs), ey = std::min(g.Bs, p.sy - by * g.Bs),
+                               ez = std::min(g.Bs, p.sz - bz * g.Bs);
+                const uint32_t w = in[w0 + lb];
+                tot += payload3(w > 32 ? 32 : w, ex * ey * ez);
+            }
+    return tot;
+}
+
+int dropin_decompress3(float* y, const unsigned char* in, size_t d1, size_t d2, size_t d3, int Bs, double eb)
+{
+    if (!y || !in)
+        return ZCCL_EINVAL;
+    const int T = zccl_get_num_chunks();
+    Geom3 g;
+    std::vector<Part3> parts;
+    if (!geom3_of(d1, d2, d3, Bs, T, &g, &parts))
+        return ZCCL_EINVAL;
+    const uint64_t n = (uint64_t) d1 * d2 * d3;
+    int rc;
+    Ctx3* c = ctx3_current(&rc);
+    if (!c)
+        return rc;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if ((rc = ensure(c->ws, ws3_layout(g.nblocks, g.T).bytes)))
+        return rc;
+    size_t avail_i = 0, avail_y = 0;
+    const bool idev = device_ptr(in, &avail_i), ydev = device_ptr(y, &avail_y);
+    const unsigned char* din = in;
+    uint64_t in_bytes = avail_i;
+    if (!idev) {
+        in_bytes = host_stream_bytes3(in, g, parts);
+        if ((rc = ensure(c->out, in_bytes)))
+            return rc;
+        HIP_TRY(hipMemcpyAsync(c->out.p, in, in_bytes, hipMemcpyHostToDevice, c->stream));
+        din = (const unsigned char*) c->out.p;
+    }
+    float* dy = y;
+    if (!ydev) {
+        if ((rc = ensure(c->x, 4 * n)))
+            return rc;
+        dy = (float*) c->x.p;
+    } else if (avail_y < 4 * n) {
+        return ZCCL_EINVAL;
+    }
+    if ((rc = zccl_sz3d_decompress_dev(din, in_bytes, d1, d2, d3, Bs, eb, T, dy, c->ws.p, c->ws.cap, c->stream)))
+        return rc;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if ((rc = zccl_sz3d_status(c->ws.p, nullptr)))
+        return rc;
+    if (!ydev)
+        HIP_TRY(hipMemcpy(y, dy, 4 * n, hipMemcpyDeviceToHost));
+    return ZCCL_OK;
+}
+
+}  // namespace
+
+extern "C" void SZp_compress3D_fast_openmp(const float* oriData, unsigned char* cmpData, size_t dim1, size_t dim2,
+    size_t dim3, int blockSideLength, double errorBound, size_t* cmpSize)
+{
+    set_last(dropin_compress3(oriData, cmpData, dim1, dim2, dim3, blockSideLength, errorBound, cmpSize));
+}
+
+extern "C" void SZp_decompress3D_fast_openmp(float* decData, unsigned char* cmpData, size_t dim1, size_t dim2,
+    size_t dim3, int blockSideLength, double errorBound)
+{
+    set_last(dropin_decompress3(decData, cmpData, dim1, dim2, dim3, blockSideLength, errorBound));
+}

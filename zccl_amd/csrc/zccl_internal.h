@@ -279,14 +279,15 @@ ZK_HD void plan_stage(const uint32_t* tot, const uint32_t* shift, int k, uint32_
 // neighbouring lanes).  unit_fields computes, for deltas d[] (two's
 // complement, zero past the block's end) at width w: the magnitudes, the
 // MSB-first sign nibble and the unit's low bits as one 4r-bit field with value
-// 0 on top (r = w & 7).  emit_unit writes the bytes unit u of a block owns
-// (width byte at buf[bo], nj values, cnt of them in this unit, U units in a
-// full block); pnib / plv are the fields of unit u+1 of the same block
-// (ignored when u+1 == U).  Ownership: unit 0 writes the width byte; an even
-// unit writes sign byte u/2 and the r bytes of low bits it shares with unit
-// u+1 (ceil of the bits actually present, zero padded); every unit writes its
-// own high bytes.  Every byte of the block then has exactly one writer, so the
-// buffer needs neither clearing nor atomics.
+// 0 on top (r = w & 7).  emit_unit writes the bytes unit u of a block owns:
+// the block's fields start at buf[fo] (its first sign byte; the width byte, if
+// the format has one inline, is the caller's), it has nj values, cnt of them
+// in this unit, and U units; pnib / plv are the fields of unit u+1 of the same
+// block (ignored when u+1 == U).  Ownership: an even unit writes sign byte u/2
+// and the r bytes of low bits it shares with unit u+1 (ceil of the bits
+// actually present, zero padded); every unit writes its own high bytes.  Every
+// byte of the block then has exactly one writer, so the buffer needs neither
+// clearing nor atomics.
 // ZCCL/src/ZCCL_TypeManager.c:12-91 (fixed-length bits), :824-848 (signs).
 struct UnitFields {
     uint32_t m[4];
@@ -329,25 +330,22 @@ ZK_HD void put_bytes(uint8_t* p, uint64_t v, uint32_t n)
         *p = (uint8_t) v;
 }
 
-ZK_HD void emit_unit(uint8_t* buf, uint32_t bo, uint32_t w, uint32_t nj, uint32_t u, uint32_t U, uint32_t cnt,
+ZK_HD void emit_unit(uint8_t* buf, uint64_t fo, uint32_t w, uint32_t nj, uint32_t u, uint32_t U, uint32_t cnt,
     const UnitFields& f, uint32_t pnib, uint32_t plv)
 {
-    if (!cnt)
-        return;
-    if (u == 0)
-        buf[bo] = (uint8_t) w;
-    if (!w)
+    if (!cnt || !w)
         return;
     const uint32_t B = w >> 3, r = w & 7;
-    const uint32_t hi0 = bo + 1 + ((nj + 7) >> 3);
+    const uint64_t hi0 = fo + ((nj + 7) >> 3);
     if ((u & 1) == 0) {
         const bool has_p = u + 1 < U;
         const uint32_t up = 4 * (u + 1);
         const uint32_t pc = has_p && up < nj ? (nj - up < 4 ? nj - up : 4u) : 0u;
-        buf[bo + 1 + (u >> 1)] = (uint8_t) ((f.nib << 4) | (has_p ? pnib : 0u));
+        buf[fo + (u >> 1)] = (uint8_t) ((f.nib << 4) | (has_p ? pnib : 0u));
         if (r) {
             const uint64_t cv = ((uint64_t) f.lv << (4 * r)) | (has_p ? plv : 0u);
-            put_bytes(buf + hi0 + B * nj + (u >> 1) * r, __builtin_bswap64(cv << (64 - 8 * r)), ((cnt + pc) * r + 7) >> 3);
+            put_bytes(buf + hi0 + (uint64_t) B * nj + (u >> 1) * r, __builtin_bswap64(cv << (64 - 8 * r)),
+                ((cnt + pc) * r + 7) >> 3);
         }
     }
     if (B == 1 && cnt == 4) {
@@ -357,7 +355,7 @@ ZK_HD void emit_unit(uint8_t* buf, uint32_t bo, uint32_t w, uint32_t nj, uint32_
         for (uint32_t c = 0; c < cnt; c++) {
             const uint32_t h = f.m[c] >> r;
             for (uint32_t b = 0; b < B; b++)
-                buf[hi0 + (4 * u + c) * B + b] = (uint8_t) (h >> (8 * b));
+                buf[hi0 + (uint64_t) (4 * u + c) * B + b] = (uint8_t) (h >> (8 * b));
         }
     }
 }

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include "zccl_internal.h"
+#include "zccl_sz3d.h"
 
 namespace zk {
 
@@ -127,6 +128,27 @@ struct ParseArgs {
     unsigned long long* size_out;
 };
 
+// One call of the 3-D codec (zccl_sz3d.hip).  The workspace holds the status
+// words (status[0] stream size, status[1] error bits), the per-block size
+// words, their scan, the scan's group sums and the sub-volume table.
+struct Sz3Args {
+    Geom3 g;
+    const Part3* parts;
+    double inv;                      // compress: 0.5 / eb
+    double tw;                       // decompress: 2 * eb
+    const float* x;                  // compress input
+    uint8_t* out;                    // compress output
+    uint64_t out_cap;
+    const uint8_t* in;               // decompress input
+    uint64_t in_size;
+    float* y;                        // decompress output (int32 residuals until the last pass)
+    uint32_t* words;                 // per block: payload bytes | width << 26
+    unsigned long long* offs;        // per block: exclusive scan of payload bytes
+    unsigned long long* part_sums;   // per scan group (+ the total)
+    unsigned long long* status;
+    unsigned long long* size_out;    // optional device copy of the stream size
+};
+
 size_t compress_lds_bytes(const Tiling& tl);
 uint32_t stage_bytes_for(const Tiling& tl, int k);
 size_t operand_lds_bytes(const Tiling& tl, int k, uint32_t stage);
@@ -139,5 +161,8 @@ hipError_t launch_parse(const ParseArgs& a, hipStream_t s);
 hipError_t launch_case_count(const uint8_t* w1, const uint8_t* w2, uint64_t nblocks,
     unsigned long long* counts, hipStream_t s);
 hipError_t launch_ws_init(WsHeader* ws, uint64_t capacity, hipStream_t s);
+hipError_t launch_sz3_table(const Geom3& g, Part3* parts, unsigned long long* status, hipStream_t s);
+hipError_t launch_sz3_compress(const Sz3Args& a, hipStream_t s);
+hipError_t launch_sz3_decompress(const Sz3Args& a, hipStream_t s);
 
 }  // namespace zk

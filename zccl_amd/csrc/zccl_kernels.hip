@@ -1191,8 +1191,9 @@ __device__ __forceinline__ void decode_operand(const uint8_t* ob, const uint32_t
 }
 
 // Encode this lane's unit in the slot layout (emit_unit in zccl_internal.h
-// holds the byte ownership rules); the next lane's fields arrive by a lane
-// shuffle, so every lane of the wave calls this.
+// holds the byte ownership rules; the unit-0 lane writes the width byte); the
+// next lane's fields arrive by a lane shuffle, so every lane of the wave
+// calls this.
 template <uint32_t U>
 __device__ __forceinline__ void encode_unit_pair(uint8_t* buf, uint32_t bo, uint32_t w, uint32_t nj, uint32_t u,
     uint32_t cnt, const uint32_t d[4])
@@ -1200,7 +1201,9 @@ __device__ __forceinline__ void encode_unit_pair(uint8_t* buf, uint32_t bo, uint
     const UnitFields f = unit_fields(w, d);
     const uint32_t pnib = __shfl_down(f.nib, 1, 64);
     const uint32_t plv = __shfl_down(f.lv, 1, 64);
-    emit_unit(buf, bo, w, nj, u, U, cnt, f, pnib, plv);
+    if (u == 0 && cnt)
+        buf[bo] = (uint8_t) w;
+    emit_unit(buf, bo + 1, w, nj, u, U, cnt, f, pnib, plv);
 }
 
 template <uint32_t L>

@@ -1,0 +1,495 @@
+// zccl_sz3d.hip -- CDNA4 (gfx950) kernels of the 3-D SZp codec
+// (SZp_compress3D_fast_openmp, ZCCL/src/ZCCL_float.c:1862-2048, and
+// SZp_decompress3D_fast_openmp, ZCCL/src/ZCCLd_float.c:2873-3040).
+//
+// The reference walks each sub-volume with one CPU thread and running
+// buffers; its residual is the exact 3-D difference dx dy dz q (zero outside
+// the sub-volume) and its decoder the exact 3-D prefix sum, so both are done
+// here with full parallelism:
+//   compress   k3_table (sub-volume table) -> k3_widths (one wave per block:
+//              residuals from x, block max, width, payload size) -> scan of the
+//              payload sizes -> k3_encode (one wave per block: width header
+//              byte, sub-volume offset, sign bitmap and fixed-length bits with
+//              the one-writer-per-byte unit encoder of the 1-D sum)
+//   decompress k3_table -> k3_dsizes (widths read from the stream's own
+//              header) -> scan -> k3_decode (residuals into the output buffer
+//              as int32) -> k3_zpass / k3_ypass (inclusive prefix sums along z
+//              and y) -> k3_xpass (along x, then x' = (float)(q * 2 eb) in place)
+// Arithmetic: q = (int)floor((double)x * (0.5 / eb) + 0.5) with the product and
+// the sum rounded separately (the reference is built without FMA; this file is
+// built with -ffp-contract=off); residuals and sums wrap in 32 bits.
+#include <hip/hip_runtime.h>
+
+#include "zccl_internal.h"
+#include "zccl_kernels.h"
+#include "zccl_sz3d.h"
+
+namespace zk {
+
+namespace {
+
+__device__ __forceinline__ int32_t quant3(float v, double inv)
+{
+    const double p = floor(__dadd_rn(__dmul_rn((double) v, inv), 0.5));
+    return (p >= -2147483648.0 && p < 2147483648.0) ? (int32_t) p : INT32_MIN;
+}
+
+__device__ __forceinline__ void raise3(unsigned long long* status, unsigned long long bit)
+{
+    atomicOr(&status[1], bit);
+}
+
+// Block of a sub-volume: position, extents and value count.
+struct Blk3 {
+    uint32_t bx, by, bz;
+    uint32_t ex, ey, ez;
+    uint32_t cnt;
+};
+
+__device__ __forceinline__ Blk3 block_of(const Part3& p, uint32_t lb, uint32_t Bs)
+{
+    Blk3 b;
+    const uint32_t yz = p.b2 * p.b3;
+    b.bx = lb / yz;
+    const uint32_t rest = lb - b.bx * yz;
+    b.by = rest / p.b3;
+    b.bz = rest - b.by * p.b3;
+    b.ex = min(Bs, p.sx - b.bx * Bs);
+    b.ey = min(Bs, p.sy - b.by * Bs);
+    b.ez = min(Bs, p.sz - b.bz * Bs);
+    b.cnt = b.ex * b.ey * b.ez;
+    return b;
+}
+
+// Residual of the value at local (lx, ly, lz) of sub-volume p: dx dy dz q with
+// q = 0 outside the sub-volume (the reference's err_1 / err_2 / err_3 chain,
+// ZCCL/src/ZCCL_float.c:1983-1990).
+__device__ __forceinline__ uint32_t resid3(const float* x, const Geom3& g, const Part3& p, double inv,
+    uint32_t lx, uint32_t ly, uint32_t lz)
+{
+    const uint64_t sx = (uint64_t) g.d2 * g.d3, sy = g.d3;
+    const uint64_t i0 = ((uint64_t) (p.ox + lx) * g.d2 + (p.oy + ly)) * g.d3 + p.oz + lz;
+    uint32_t acc = (uint32_t) quant3(x[i0], inv);
+    if (lz)
+        acc -= (uint32_t) quant3(x[i0 - 1], inv);
+    if (ly) {
+        acc -= (uint32_t) quant3(x[i0 - sy], inv);
+        if (lz)
+            acc += (uint32_t) quant3(x[i0 - sy - 1], inv);
+    }
+    if (lx) {
+        acc -= (uint32_t) quant3(x[i0 - sx], inv);
+        if (lz)
+            acc += (uint32_t) quant3(x[i0 - sx - 1], inv);
+        if (ly) {
+            acc += (uint32_t) quant3(x[i0 - sx - sy], inv);
+            if (lz)
+                acc -= (uint32_t) quant3(x[i0 - sx - sy - 1], inv);
+        }
+    }
+    return acc;
+}
+
+// Local coordinates of value e of block b (values in i, j, k order).
+__device__ __forceinline__ void value_pos(const Blk3& b, uint32_t Bs, uint32_t e, uint32_t& lx, uint32_t& ly,
+    uint32_t& lz)
+{
+    const uint32_t yz = b.ey * b.ez;
+    const uint32_t i = e / yz, rest = e - i * yz, j = rest / b.ez, k = rest - j * b.ez;
+    lx = b.bx * Bs + i;
+    ly = b.by * Bs + j;
+    lz = b.bz * Bs + k;
+}
+
+__device__ __forceinline__ uint32_t wave_max(uint32_t v)
+{
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1)
+        v = max(v, (uint32_t) __shfl_xor(v, d, 64));
+    return v;
+}
+
+__device__ __forceinline__ unsigned long long wave_incl_scan64(unsigned long long v)
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const unsigned long long y = __shfl_up(v, d, 64);
+        if (lane >= d)
+            v += y;
+    }
+    return v;
+}
+
+__device__ __forceinline__ uint32_t wave_incl_scan32(uint32_t v)
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t y = __shfl_up(v, d, 64);
+        if (lane >= d)
+            v += y;
+    }
+    return v;
+}
+
+constexpr uint32_t kSizeMask = (1u << 26) - 1;
+
+// Sub-volume table, built on the device by one thread (no host copy on the
+// stream the caller passes), and the status words cleared.
+__global__ void k3_table(uint32_t d1, uint32_t d2, uint32_t d3, int Bs, int T, Part3* parts,
+    unsigned long long* status)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        Geom3 g;
+        make_geom3(d1, d2, d3, Bs, T, &g, parts);
+        status[0] = 0;
+        status[1] = 0;
+    }
+}
+
+// Width and payload size of every block: one wave per block.
+__global__ void __launch_bounds__(256) k3_widths(Sz3Args a)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t gb = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (gb >= a.g.nblocks)
+        return;
+    const uint32_t t = __builtin_amdgcn_readfirstlane(part_of_block(a.parts, a.g.T, gb));
+    const Part3 p = a.parts[t];
+    const Blk3 b = block_of(p, gb - p.g0, a.g.Bs);
+    uint32_t m = 0;
+    for (uint32_t e = lane; e < b.cnt; e += 64) {
+        uint32_t lx, ly, lz;
+        value_pos(b, a.g.Bs, e, lx, ly, lz);
+        m = max(m, mag32(resid3(a.x, a.g, p, a.inv, lx, ly, lz)));
+    }
+    m = wave_max(m);
+    if (lane == 0) {
+        const uint32_t w = m ? 32u - (uint32_t) __clz((int) m) : 0u;
+        a.words[gb] = payload3(w, b.cnt) | (w << 26);
+    }
+}
+
+// Payload sizes of a stream's blocks from its own header: one thread per block.
+__global__ void __launch_bounds__(256) k3_dsizes(Sz3Args a)
+{
+    const uint32_t gb = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gb >= a.g.nblocks)
+        return;
+    const uint32_t t = part_of_block(a.parts, a.g.T, gb);
+    const Part3 p = a.parts[t];
+    const Blk3 b = block_of(p, gb - p.g0, a.g.Bs);
+    uint64_t h;
+    __builtin_memcpy(&h, a.in + 8ull * t, 8);
+    const uint64_t wpos = 8ull * a.g.T + h + (gb - p.g0);
+    uint32_t w = 0;
+    if (wpos < a.in_size) {
+        w = a.in[wpos];
+        if (w > 32) {
+            raise3(a.status, ZK_ERR_WIDTH);
+            w = 0;
+        }
+    } else {
+        raise3(a.status, ZK_ERR_OVERRUN);
+    }
+    a.words[gb] = payload3(w, b.cnt) | (w << 26);
+}
+
+// Exclusive scan of the payload sizes (low 26 bits of words[]) into offs[]:
+// group sums, a scan of the group sums by one workgroup, then each group's
+// local scan.  256 threads, kScan3Group = 1024 entries per group.
+__global__ void __launch_bounds__(256) k3_scan_groups(Sz3Args a)
+{
+    __shared__ unsigned long long s_w[4];
+    const uint32_t base = blockIdx.x * kScan3Group;
+    unsigned long long v = 0;
+    for (uint32_t i = threadIdx.x; i < kScan3Group; i += 256)
+        if (base + i < a.g.nblocks)
+            v += a.words[base + i] & kSizeMask;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1)
+        v += __shfl_xor(v, d, 64);
+    if ((threadIdx.x & 63) == 0)
+        s_w[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        a.part_sums[blockIdx.x] = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+}
+
+// One workgroup: exclusive scan of the group sums in place; the stream size
+// (header, width bytes, payloads) goes to status[0] and the optional size
+// word; a compress whose stream exceeds the output capacity raises NOSPACE.
+__global__ void __launch_bounds__(256) k3_scan_top(Sz3Args a, uint32_t ngroups, int check_cap)
+{
+    __shared__ unsigned long long s_w[4];
+    __shared__ unsigned long long s_carry;
+    if (threadIdx.x == 0)
+        s_carry = 0;
+    __syncthreads();
+    for (uint32_t c0 = 0; c0 < ngroups; c0 += 256) {
+        const uint32_t i = c0 + threadIdx.x;
+        const unsigned long long v = i < ngroups ? a.part_sums[i] : 0ull;
+        const unsigned long long inc = wave_incl_scan64(v);
+        if ((threadIdx.x & 63) == 63)
+            s_w[threadIdx.x >> 6] = inc;
+        __syncthreads();
+        unsigned long long pre = s_carry;
+        for (uint32_t w = 0; w < (threadIdx.x >> 6); w++)
+            pre += s_w[w];
+        if (i < ngroups)
+            a.part_sums[i] = pre + inc - v;
+        __syncthreads();
+        if (threadIdx.x == 255)
+            s_carry = pre + inc;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const unsigned long long total = 8ull * a.g.T + a.g.nblocks + s_carry;
+        a.status[0] = total;
+        a.part_sums[ngroups] = s_carry;
+        if (check_cap && total > a.out_cap)
+            raise3(a.status, ZK_ERR_NOSPACE);
+        if (a.size_out)
+            *a.size_out = (check_cap && total > a.out_cap) ? 0ull : total;
+    }
+}
+
+__global__ void __launch_bounds__(256) k3_scan_apply(Sz3Args a)
+{
+    __shared__ unsigned long long s_w[4];
+    const uint32_t base = blockIdx.x * kScan3Group + threadIdx.x * 4;
+    uint32_t v[4], run = 0;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        v[c] = base + c < a.g.nblocks ? a.words[base + c] & kSizeMask : 0u;
+        run += v[c];
+    }
+    const unsigned long long inc = wave_incl_scan64(run);
+    if ((threadIdx.x & 63) == 63)
+        s_w[threadIdx.x >> 6] = inc;
+    __syncthreads();
+    unsigned long long pre = a.part_sums[blockIdx.x] + inc - run;
+    for (uint32_t w = 0; w < (threadIdx.x >> 6); w++)
+        pre += s_w[w];
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        if (base + c < a.g.nblocks)
+            a.offs[base + c] = pre;
+        pre += v[c];
+    }
+}
+
+// Header byte, sub-volume offset and payload of every block: one wave per
+// block, 4-value units in consecutive lanes (so unit pairs share a wave).
+__global__ void __launch_bounds__(256) k3_encode(Sz3Args a)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t gb = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (gb >= a.g.nblocks || (a.status[1] & ZK_ERR_NOSPACE))
+        return;
+    const uint32_t t = __builtin_amdgcn_readfirstlane(part_of_block(a.parts, a.g.T, gb));
+    const Part3 p = a.parts[t];
+    const uint32_t lb = gb - p.g0, Bs = a.g.Bs;
+    const Blk3 b = block_of(p, lb, Bs);
+    const uint32_t word = a.words[gb], w = word >> 26;
+    const uint64_t sub = (uint64_t) p.nbc + a.offs[p.g0];  // sub-volume start after the header
+    const uint64_t hdr = 8ull * a.g.T;
+    if (lane == 0) {
+        a.out[hdr + sub + lb] = (uint8_t) w;
+        if (lb == 0)
+            __builtin_memcpy(a.out + 8ull * t, &sub, 8);
+    }
+    if (!w)
+        return;
+    const uint64_t fo = hdr + p.nbc + (uint64_t) p.b1 * p.b2 * p.b3 + a.offs[gb];
+    const uint32_t U = (b.cnt + 3) / 4;
+    for (uint32_t u0 = 0; u0 < U; u0 += 64) {
+        const uint32_t u = u0 + lane;
+        const uint32_t cnt = u < U ? min(4u, b.cnt - 4 * u) : 0u;
+        uint32_t d[4] = {0, 0, 0, 0};
+        for (uint32_t c = 0; c < cnt; c++) {
+            uint32_t lx, ly, lz;
+            value_pos(b, Bs, 4 * u + c, lx, ly, lz);
+            d[c] = resid3(a.x, a.g, p, a.inv, lx, ly, lz);
+        }
+        const UnitFields f = unit_fields(w, d);
+        const uint32_t pnib = __shfl_down(f.nib, 1, 64);
+        const uint32_t plv = __shfl_down(f.lv, 1, 64);
+        emit_unit(a.out, fo, w, b.cnt, u, U, cnt, f, pnib, plv);
+    }
+}
+
+// Byte j of a payload (0 past its end, which no well-formed block reads).
+__device__ __forceinline__ uint32_t pbyte(const uint8_t* p, uint32_t size, uint32_t j)
+{
+    return j < size ? p[j] : 0u;
+}
+
+// Residuals of every block into the output buffer as int32 at their positions
+// in the volume: one wave per block, one 4-value unit per lane.
+// ZCCL/src/ZCCL_TypeManager.c:423-517 (fixed-length bits), :906-950 (signs).
+__global__ void __launch_bounds__(256) k3_decode(Sz3Args a)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t gb = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (gb >= a.g.nblocks)
+        return;
+    const uint32_t t = __builtin_amdgcn_readfirstlane(part_of_block(a.parts, a.g.T, gb));
+    const Part3 p = a.parts[t];
+    const uint32_t lb = gb - p.g0, Bs = a.g.Bs;
+    const Blk3 b = block_of(p, lb, Bs);
+    const uint32_t word = a.words[gb], w = word >> 26, size = word & kSizeMask;
+    uint64_t h;
+    __builtin_memcpy(&h, a.in + 8ull * t, 8);
+    const uint64_t fo = 8ull * a.g.T + h + (uint64_t) p.b1 * p.b2 * p.b3 + (a.offs[gb] - a.offs[p.g0]);
+    const bool inside = fo + size <= a.in_size;
+    if (w && !inside && lane == 0)
+        raise3(a.status, ZK_ERR_OVERRUN);
+    const uint8_t* pl = a.in + fo;
+    const uint32_t B = w >> 3, r = w & 7, sb = (b.cnt + 7) / 8;
+    int32_t* q = (int32_t*) a.y;
+    const uint32_t U = (b.cnt + 3) / 4;
+    for (uint32_t u = lane; u < U; u += 64) {
+        const uint32_t cnt = min(4u, b.cnt - 4 * u);
+        for (uint32_t c = 0; c < cnt; c++) {
+            const uint32_t e = 4 * u + c;
+            uint32_t v = 0;
+            if (w && inside) {
+                for (uint32_t k = 0; k < B; k++)
+                    v |= pbyte(pl, size, sb + e * B + k) << (8 * k);
+                uint32_t lo = 0;
+                for (uint32_t bit = e * r, s = 0; s < r; s++, bit++)
+                    lo = (lo << 1) | ((pbyte(pl, size, sb + B * b.cnt + (bit >> 3)) >> (7 - (bit & 7))) & 1u);
+                v = (v << r) | lo;
+                if ((pbyte(pl, size, e >> 3) >> (7 - (e & 7))) & 1u)
+                    v = 0u - v;
+            }
+            uint32_t lx, ly, lz;
+            value_pos(b, Bs, e, lx, ly, lz);
+            q[((uint64_t) (p.ox + lx) * a.g.d2 + (p.oy + ly)) * a.g.d3 + p.oz + lz] = (int32_t) v;
+        }
+    }
+}
+
+// Inclusive prefix sums along z: one wave per (sub-volume, x, y) row.
+__global__ void __launch_bounds__(256) k3_zpass(Sz3Args a)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= a.g.zrows)
+        return;
+    uint32_t lo = 0, hi = a.g.T;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (a.parts[mid].zr0 <= row)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    const Part3 p = a.parts[__builtin_amdgcn_readfirstlane(lo)];
+    const uint32_t r = row - p.zr0, lx = r / p.sy, ly = r - lx * p.sy;
+    uint32_t* q = (uint32_t*) a.y + ((uint64_t) (p.ox + lx) * a.g.d2 + (p.oy + ly)) * a.g.d3 + p.oz;
+    uint32_t carry = 0;
+    for (uint32_t z0 = 0; z0 < p.sz; z0 += 64) {
+        const uint32_t z = z0 + lane;
+        const uint32_t v = z < p.sz ? q[z] : 0u;
+        const uint32_t inc = wave_incl_scan32(v) + carry;
+        if (z < p.sz)
+            q[z] = inc;
+        carry = __shfl(inc, 63, 64);
+    }
+}
+
+// Along y: one thread per (sub-volume, x, z) column.
+__global__ void __launch_bounds__(256) k3_ypass(Sz3Args a)
+{
+    const uint32_t col = blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= a.g.ycols)
+        return;
+    uint32_t lo = 0, hi = a.g.T;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (a.parts[mid].yc0 <= col)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    const Part3 p = a.parts[lo];
+    const uint32_t c = col - p.yc0, lx = c / p.sz, lz = c - lx * p.sz;
+    uint32_t* q = (uint32_t*) a.y + ((uint64_t) (p.ox + lx) * a.g.d2 + p.oy) * a.g.d3 + p.oz + lz;
+    uint32_t run = q[0];
+    for (uint32_t y = 1; y < p.sy; y++) {
+        run += q[(uint64_t) y * a.g.d3];
+        q[(uint64_t) y * a.g.d3] = run;
+    }
+}
+
+// Along x, then dequantize in place: one thread per (sub-volume, y, z) column.
+// x' = (float)((double)q * 2 eb), ZCCL/src/ZCCLd_float.c:3030.
+__global__ void __launch_bounds__(256) k3_xpass(Sz3Args a)
+{
+    const uint32_t col = blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= a.g.xcols)
+        return;
+    uint32_t lo = 0, hi = a.g.T;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (a.parts[mid].xc0 <= col)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    const Part3 p = a.parts[lo];
+    const uint32_t c = col - p.xc0, ly = c / p.sz, lz = c - ly * p.sz;
+    const uint64_t sx = (uint64_t) a.g.d2 * a.g.d3;
+    uint32_t* q = (uint32_t*) a.y + ((uint64_t) p.ox * a.g.d2 + (p.oy + ly)) * a.g.d3 + p.oz + lz;
+    uint32_t run = 0;
+    for (uint32_t x = 0; x < p.sx; x++) {
+        run += q[x * sx];
+        a.y[(q - (uint32_t*) a.y) + x * sx] = (float) __dmul_rn((double) (int32_t) run, a.tw);
+    }
+}
+
+}  // namespace
+
+static hipError_t scan3(const Sz3Args& a, int check_cap, hipStream_t s)
+{
+    const uint32_t groups = (uint32_t) ceil_div(a.g.nblocks ? a.g.nblocks : 1, kScan3Group);
+    hipLaunchKernelGGL(k3_scan_groups, dim3(groups), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k3_scan_top, dim3(1), dim3(256), 0, s, a, groups, check_cap);
+    hipLaunchKernelGGL(k3_scan_apply, dim3(groups), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_sz3_table(const Geom3& g, Part3* parts, unsigned long long* status, hipStream_t s)
+{
+    hipLaunchKernelGGL(k3_table, dim3(1), dim3(64), 0, s, g.d1, g.d2, g.d3, (int) g.Bs, (int) g.T, parts, status);
+    return hipGetLastError();
+}
+
+hipError_t launch_sz3_compress(const Sz3Args& a, hipStream_t s)
+{
+    const uint32_t waves = (uint32_t) ceil_div(a.g.nblocks, 4);
+    hipLaunchKernelGGL(k3_widths, dim3(waves), dim3(256), 0, s, a);
+    hipError_t e = scan3(a, 1, s);
+    if (e != hipSuccess)
+        return e;
+    hipLaunchKernelGGL(k3_encode, dim3(waves), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_sz3_decompress(const Sz3Args& a, hipStream_t s)
+{
+    hipLaunchKernelGGL(k3_dsizes, dim3((uint32_t) ceil_div(a.g.nblocks, 256)), dim3(256), 0, s, a);
+    hipError_t e = scan3(a, 0, s);
+    if (e != hipSuccess)
+        return e;
+    hipLaunchKernelGGL(k3_decode, dim3((uint32_t) ceil_div(a.g.nblocks, 4)), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k3_zpass, dim3((uint32_t) ceil_div(a.g.zrows, 4)), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k3_ypass, dim3((uint32_t) ceil_div(a.g.ycols, 256)), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k3_xpass, dim3((uint32_t) ceil_div(a.g.xcols, 256)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+}  // namespace zk
