@@ -60,6 +60,9 @@ def parse_args():
     ap.add_argument("--batch-buffers", type=int, default=4096)
     ap.add_argument("--batch-ranks", type=int, default=16)
     ap.add_argument("--serve-requests", type=int, default=400)
+    ap.add_argument("--no-3d", action="store_true", help="skip the 3-D codec measurement")
+    ap.add_argument("--side3d", type=int, default=512, help="grid side of the 3-D codec's field")
+    ap.add_argument("--bs3d", type=int, default=4, help="block side of the 3-D codec")
     return ap.parse_args()
 
 
@@ -120,6 +123,98 @@ def cpu_baseline(streams0, n, L, T, gpu_sum, seconds):
                    f"{len(hs)} streams of {n} values (L={L}, T={T}), median of {len(times)} runs"),
         "bitexact_vs_gpu": bitexact,
     }
+
+
+def bench_sz3d(side, Bs, T, rel_eb, steps, warmup, cpu_seconds, with_cpu):
+    """3-D SZp codec (SZp_compress3D_fast_openmp / SZp_decompress3D_fast_openmp)
+    on one side^3 FP32 GRF (k^-3): GPU compress and decompress with inputs in
+    HBM, HIP events on the launch stream; the round trip is checked against
+    its closed form (float)(floor(x * 0.5/eb + 0.5) * 2 eb).  The compiled
+    reference (OpenMP team of T on the host cores) is timed on a bounded
+    (side/2)^3 sample and its stream compared byte for byte with the GPU's."""
+    import numpy as np
+    import torch
+
+    from oracle import oracle as O
+    from zccl_amd import synth, sz3d
+
+    x = synth.grf3d(side, seed=20261203).reshape(side, side, side).contiguous()
+    eb = float((x.max() - x.min()).item()) * rel_eb
+    worst, wsb = sz3d.sizes(x.shape, Bs, T)
+    data = torch.empty(worst, dtype=torch.uint8, device="cuda")
+    ws = torch.empty(wsb, dtype=torch.uint8, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(warmup):
+        s = sz3d.compress(x, Bs, eb, T, out=data, ws=ws)
+        sz3d.decompress(s, out=y, ws=ws)
+    stream = torch.cuda.current_stream()
+
+    def timed_events(fn):
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+        for a, b in ev:
+            a.record(stream)
+            fn()
+            b.record(stream)
+        torch.cuda.synchronize()
+        return float(np.mean([a.elapsed_time(b) for a, b in ev])) / 1e3
+
+    t_c = timed_events(lambda: sz3d.compress(x, Bs, eb, T, out=data, ws=ws))
+    s = sz3d.Stream3D(data, ws, tuple(x.shape), Bs, eb, T)
+    size = s.size
+    t_d = timed_events(lambda: sz3d.decompress(s, out=y, ws=ws, nbytes=size))
+    sz3d.check(ws)
+    q = torch.floor(x.double() * (0.5 / eb) + 0.5)
+    closed = bool(torch.equal(y.view(torch.int32), (q * (2 * eb)).float().view(torch.int32)))
+    del q
+    n = side ** 3
+    res = {
+        "dims": [side, side, side], "Bs": Bs, "T": T, "abs_eb": eb, "rel_eb": rel_eb,
+        "compression_ratio": round(4 * n / size, 3),
+        "compress_GBps_input": round(4 * n / 1e9 / t_c, 1),
+        "decompress_GBps_output": round(4 * n / 1e9 / t_d, 1),
+        "compress_ms": round(t_c * 1e3, 3), "decompress_ms": round(t_d * 1e3, 3),
+        "alg_GBps_compress": round((4 * n + size) / 1e9 / t_c, 1),
+        "alg_GBps_decompress": round((4 * n + size) / 1e9 / t_d, 1),
+        "round_trip_equals_closed_form": closed,
+        "cpu_reference": None,
+    }
+    if with_cpu and O.ref_available():
+        h = side // 2
+        xs = x[:h, :h, :h].contiguous()
+        xh = xs.cpu().numpy()
+        gs = sz3d.compress(xs, Bs, eb, T)
+        ghost = gs.to_host()
+        ref = O.ref_compress3d(xh, (h, h, h), Bs, eb, T)  # checks the reference's buffer fits, then runs it
+        # time the reference's own calls on preallocated buffers
+        import ctypes
+
+        R = O.ref()
+        O._set_T(T)
+        cbuf = np.zeros(max(ref.size, 4 * h ** 3) + 4096, dtype=np.uint8)
+        dbuf = np.zeros(ref.size + 256, dtype=np.uint8)
+        dbuf[: ref.size] = ref
+        yh = np.zeros(h ** 3, dtype=np.float32)
+        csize = ctypes.c_size_t(0)
+        tc, td = [], []
+        t_end = time.perf_counter() + cpu_seconds
+        while len(tc) < 3 or (time.perf_counter() < t_end and len(tc) < 20):
+            t0 = time.perf_counter()
+            R.SZp_compress3D_fast_openmp(xh.ctypes.data, cbuf.ctypes.data, h, h, h, Bs, eb, ctypes.byref(csize))
+            t1 = time.perf_counter()
+            R.SZp_decompress3D_fast_openmp(yh.ctypes.data, dbuf.ctypes.data, h, h, h, Bs, eb)
+            tc.append(t1 - t0)
+            td.append(time.perf_counter() - t1)
+        ref = cbuf[: csize.value]
+        res["cpu_reference"] = {
+            "compress_GBps_input": round(4 * h ** 3 / 1e9 / float(np.median(tc)), 3),
+            "decompress_GBps_output": round(4 * h ** 3 / 1e9 / float(np.median(td)), 3),
+            "cores": T, "kind": "reference",
+            "sample": f"{h}^3 corner of the field, median of {len(tc)} runs",
+            "bitexact_vs_gpu": bool(np.array_equal(ref, ghost)),
+        }
+    del x, y, data, ws
+    torch.cuda.empty_cache()
+    return res
 
 
 def main():
@@ -258,6 +353,10 @@ def main():
     t_doc = timed(lambda: ar.reduce_scatter_doc(xs), 2)
     t_homo_rs = t_comp + ms_per_step / 1e3
 
+    # the uniform batch of 36-value blocks at 256 threads runs the slot-layout sum
+    from zccl_amd._lib import lib as _zl
+    homsum_kernel = ("zk::k_homsum_batch_slots" if L == 36 and _zl().zccl_tune_get_threads(1) == 256
+                     else "zk::k_homsum_batch")
     roof = {
         "bound": "hbm",
         "achieved": round(achieved, 1),
@@ -265,7 +364,7 @@ def main():
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBPS, 4),
         "traffic": None,
-        "kernel": "zk::k_homsum_batch",
+        "kernel": homsum_kernel,
         "alg_bytes_per_launch": int(alg_per_launch),
         "avg_launch_us": round(avg_launch_s * 1e6, 2),
     }
@@ -287,6 +386,12 @@ def main():
     if not args.no_serve:
         log("serving workload")
         served = workloads.serving(nreq=args.serve_requests, L=L, T=T)
+
+    sz3 = None
+    if not args.no_3d:
+        log("3-D codec")
+        sz3 = bench_sz3d(args.side3d, args.bs3d, T, args.rel_eb, args.steps, args.warmup,
+                         min(args.cpu_seconds, 10.0), not args.no_cpu_baseline)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -333,6 +438,7 @@ def main():
         "n_err_over_P_eb": n_over,
         "batched_uneven": batched,
         "serve": served,
+        "sz3d": sz3,
     }
     if served:
         res["serve_p50_ms"] = served["loads"][-1]["p50_ms"]

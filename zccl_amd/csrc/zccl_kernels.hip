@@ -26,6 +26,7 @@
 
 #include "zccl_internal.h"
 #include "zccl_kernels.h"
+#include "zccl_unit.h"
 
 namespace zk {
 
@@ -292,76 +293,6 @@ __device__ __forceinline__ void chain_finished(WsHeader* ws, uint32_t nchains)
 }
 
 // ------------------------------------------------------------ unit codec
-
-// Unaligned LDS reads: gfx950 serves a byte-aligned ds_read_b32/b64/b128
-// as one instruction.
-__device__ __forceinline__ uint32_t lds_ld4(const uint8_t* p)
-{
-    uint32_t v;
-    __builtin_memcpy(&v, p, 4);
-    return v;
-}
-
-__device__ __forceinline__ uint2 lds_ld8(const uint8_t* p)
-{
-    uint2 v;
-    __builtin_memcpy(&v, p, 8);
-    return v;
-}
-
-__device__ __forceinline__ uint4 lds_ld16(const uint8_t* p)
-{
-    uint4 v;
-    __builtin_memcpy(&v, p, 16);
-    return v;
-}
-
-// Add the up-to-4 deltas of unit u of a block (width byte at bp, width w > 0,
-// nj elements, sb = ceil(nj/8) sign bytes) to acc[].  Magnitude = (high
-// bytes << r) | low r bits; the sign bit s (MSB-first bitmap) is applied as
-// (m ^ -s) + s.  Reads reach at most 16 bytes past the block's last byte.
-// ZCCL/src/ZCCL_TypeManager.c:423-517 (extract) and :906-950 (sign bitmap).
-__device__ __forceinline__ void accumulate_unit(const uint8_t* bp, uint32_t w, uint32_t nj, uint32_t sb,
-    uint32_t u, uint32_t acc[4])
-{
-    const uint32_t B = w >> 3, r = w & 7;
-    const uint32_t nib = __builtin_amdgcn_ubfe((uint32_t) bp[1 + (u >> 1)], (~u & 1u) << 2, 4);
-    const uint8_t* hp = bp + 1 + sb;
-    uint32_t m[4];
-    if (B <= 1) {
-        // widths below 16, the common case: one 4-byte read; B == 0 reads
-        // the low-bit bytes and masks them to nothing
-        const uint32_t H = lds_ld4(hp + 4 * u * B);
-#pragma unroll
-        for (int c = 0; c < 4; c++)
-            m[c] = __builtin_amdgcn_ubfe(H, 8 * c, 8 * B);
-    } else if (B == 2) {
-        uint2 H = lds_ld8(hp + 8 * u);
-        m[0] = H.x & 0xFFFFu;
-        m[1] = H.x >> 16;
-        m[2] = H.y & 0xFFFFu;
-        m[3] = H.y >> 16;
-    } else {
-        uint4 H = lds_ld16(hp + 4 * u * B);
-        if (B == 3) {
-            m[0] = H.x & 0xFFFFFFu;
-            m[1] = __builtin_amdgcn_alignbyte(H.y, H.x, 3) & 0xFFFFFFu;
-            m[2] = __builtin_amdgcn_alignbyte(H.z, H.y, 2) & 0xFFFFFFu;
-            m[3] = H.z >> 8;
-        } else {
-            m[0] = H.x; m[1] = H.y; m[2] = H.z; m[3] = H.w;
-        }
-    }
-    // low bits for every width (r == 0 extracts zero-width fields)
-    const uint32_t bitpos = 4 * u * r;
-    const uint32_t X = __builtin_bswap32(lds_ld4(hp + B * nj + (bitpos >> 3))) << (bitpos & 7);
-#pragma unroll
-    for (int c = 0; c < 4; c++) {
-        const uint32_t mc = (m[c] << r) | __builtin_amdgcn_ubfe(X, 32 - r * (c + 1), r);
-        const uint32_t s = (uint32_t) __builtin_amdgcn_sbfe((int32_t) nib, 3 - c, 1);  // 0 or ~0
-        acc[c] += (mc ^ s) - s;
-    }
-}
 
 // OR the encoding of unit u (cnt deltas d[], two's complement) of a block of
 // nj elements at width w whose width byte sits at LDS byte bo.

@@ -23,6 +23,7 @@
 #include "zccl_internal.h"
 #include "zccl_kernels.h"
 #include "zccl_sz3d.h"
+#include "zccl_unit.h"
 
 namespace zk {
 
@@ -280,19 +281,34 @@ __global__ void __launch_bounds__(256) k3_scan_apply(Sz3Args a)
     }
 }
 
+// LDS bytes per wave for one block's payload: blocks whose payload fits are
+// encoded (decoded) in LDS and moved with dword copies; larger ones work on
+// global memory byte by byte.  Side 9 (729 values, at most 3008 bytes) fits.
+constexpr uint32_t kStage3 = 4096;
+
+// Make a wave's LDS writes visible to its other lanes.
+__device__ __forceinline__ void wave_sync_lds()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Header byte, sub-volume offset and payload of every block: one wave per
 // block, 4-value units in consecutive lanes (so unit pairs share a wave).
 __global__ void __launch_bounds__(256) k3_encode(Sz3Args a)
 {
-    const uint32_t lane = threadIdx.x & 63;
-    const uint32_t gb = blockIdx.x * 4 + (threadIdx.x >> 6);
+    __shared__ uint32_t s_stage[4][kStage3 / 4];
+    __shared__ uint32_t s_res[4][256];
+    const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const uint32_t gb = blockIdx.x * 4 + wv;
     if (gb >= a.g.nblocks || (a.status[1] & ZK_ERR_NOSPACE))
         return;
     const uint32_t t = __builtin_amdgcn_readfirstlane(part_of_block(a.parts, a.g.T, gb));
     const Part3 p = a.parts[t];
     const uint32_t lb = gb - p.g0, Bs = a.g.Bs;
     const Blk3 b = block_of(p, lb, Bs);
-    const uint32_t word = a.words[gb], w = word >> 26;
+    const uint32_t word = a.words[gb], w = word >> 26, size = word & kSizeMask;
     const uint64_t sub = (uint64_t) p.nbc + a.offs[p.g0];  // sub-volume start after the header
     const uint64_t hdr = 8ull * a.g.T;
     if (lane == 0) {
@@ -303,20 +319,52 @@ __global__ void __launch_bounds__(256) k3_encode(Sz3Args a)
     if (!w)
         return;
     const uint64_t fo = hdr + p.nbc + (uint64_t) p.b1 * p.b2 * p.b3 + a.offs[gb];
+    const bool staged = size <= kStage3;
+    uint8_t* st = (uint8_t*) s_stage[wv];
     const uint32_t U = (b.cnt + 3) / 4;
+    uint32_t* res = s_res[wv];
     for (uint32_t u0 = 0; u0 < U; u0 += 64) {
+        // residuals of the next 256 values, one per lane per step, then each
+        // unit lane takes its four
+#pragma unroll
+        for (uint32_t c = 0; c < 4; c++) {
+            const uint32_t e = 4 * u0 + 64 * c + lane;
+            uint32_t r = 0;
+            if (e < b.cnt) {
+                uint32_t lx, ly, lz;
+                value_pos(b, Bs, e, lx, ly, lz);
+                r = resid3(a.x, a.g, p, a.inv, lx, ly, lz);
+            }
+            res[64 * c + lane] = r;
+        }
+        wave_sync_lds();
         const uint32_t u = u0 + lane;
         const uint32_t cnt = u < U ? min(4u, b.cnt - 4 * u) : 0u;
-        uint32_t d[4] = {0, 0, 0, 0};
-        for (uint32_t c = 0; c < cnt; c++) {
-            uint32_t lx, ly, lz;
-            value_pos(b, Bs, 4 * u + c, lx, ly, lz);
-            d[c] = resid3(a.x, a.g, p, a.inv, lx, ly, lz);
-        }
+        uint32_t d[4];
+#pragma unroll
+        for (uint32_t c = 0; c < 4; c++)
+            d[c] = c < cnt ? res[4 * lane + c] : 0u;
+        wave_sync_lds();
         const UnitFields f = unit_fields(w, d);
         const uint32_t pnib = __shfl_down(f.nib, 1, 64);
         const uint32_t plv = __shfl_down(f.lv, 1, 64);
-        emit_unit(a.out, fo, w, b.cnt, u, U, cnt, f, pnib, plv);
+        if (staged)
+            emit_unit(st, 0, w, b.cnt, u, U, cnt, f, pnib, plv);
+        else
+            emit_unit(a.out, fo, w, b.cnt, u, U, cnt, f, pnib, plv);
+    }
+    if (staged) {
+        // every payload byte has one writer; dword stores cover only whole
+        // dwords of this payload, the tail goes out byte by byte
+        wave_sync_lds();
+        uint8_t* g = a.out + fo;
+        const uint32_t nw = size / 4;
+        for (uint32_t i = lane; i < nw; i += 64) {
+            const uint32_t v = ((const uint32_t*) st)[i];
+            __builtin_memcpy(g + 4 * i, &v, 4);
+        }
+        for (uint32_t i = 4 * nw + lane; i < size; i += 64)
+            g[i] = st[i];
     }
 }
 
@@ -327,12 +375,17 @@ __device__ __forceinline__ uint32_t pbyte(const uint8_t* p, uint32_t size, uint3
 }
 
 // Residuals of every block into the output buffer as int32 at their positions
-// in the volume: one wave per block, one 4-value unit per lane.
+// in the volume: one wave per block, one 4-value unit per lane.  A payload of
+// at most kStage3 bytes is copied into LDS (dword loads, 16 zero bytes of
+// slack) and decoded with the 1-D unit decoder; a larger one is read byte by
+// byte from global memory.
 // ZCCL/src/ZCCL_TypeManager.c:423-517 (fixed-length bits), :906-950 (signs).
 __global__ void __launch_bounds__(256) k3_decode(Sz3Args a)
 {
-    const uint32_t lane = threadIdx.x & 63;
-    const uint32_t gb = blockIdx.x * 4 + (threadIdx.x >> 6);
+    __shared__ uint32_t s_stage[4][kStage3 / 4 + 8];
+    __shared__ uint32_t s_res[4][256];
+    const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const uint32_t gb = blockIdx.x * 4 + wv;
     if (gb >= a.g.nblocks)
         return;
     const uint32_t t = __builtin_amdgcn_readfirstlane(part_of_block(a.parts, a.g.T, gb));
@@ -348,27 +401,60 @@ __global__ void __launch_bounds__(256) k3_decode(Sz3Args a)
         raise3(a.status, ZK_ERR_OVERRUN);
     const uint8_t* pl = a.in + fo;
     const uint32_t B = w >> 3, r = w & 7, sb = (b.cnt + 7) / 8;
-    int32_t* q = (int32_t*) a.y;
-    const uint32_t U = (b.cnt + 3) / 4;
-    for (uint32_t u = lane; u < U; u += 64) {
-        const uint32_t cnt = min(4u, b.cnt - 4 * u);
-        for (uint32_t c = 0; c < cnt; c++) {
-            const uint32_t e = 4 * u + c;
-            uint32_t v = 0;
-            if (w && inside) {
-                for (uint32_t k = 0; k < B; k++)
-                    v |= pbyte(pl, size, sb + e * B + k) << (8 * k);
-                uint32_t lo = 0;
-                for (uint32_t bit = e * r, s = 0; s < r; s++, bit++)
-                    lo = (lo << 1) | ((pbyte(pl, size, sb + B * b.cnt + (bit >> 3)) >> (7 - (bit & 7))) & 1u);
-                v = (v << r) | lo;
-                if ((pbyte(pl, size, e >> 3) >> (7 - (e & 7))) & 1u)
-                    v = 0u - v;
-            }
-            uint32_t lx, ly, lz;
-            value_pos(b, Bs, e, lx, ly, lz);
-            q[((uint64_t) (p.ox + lx) * a.g.d2 + (p.oy + ly)) * a.g.d3 + p.oz + lz] = (int32_t) v;
+    const bool staged = w && inside && size <= kStage3;
+    uint8_t* st = (uint8_t*) s_stage[wv];
+    if (staged) {
+        const uint32_t nw = size / 4;
+        for (uint32_t i = lane; i < nw; i += 64) {
+            uint32_t v;
+            __builtin_memcpy(&v, pl + 4 * i, 4);
+            ((uint32_t*) st)[i] = v;
         }
+        for (uint32_t i = 4 * nw + lane; i < size; i += 64)
+            st[i] = pl[i];
+        if (lane < 16)
+            st[size + lane] = 0;
+        wave_sync_lds();
+    }
+    int32_t* q = (int32_t*) a.y;
+    uint32_t* res = s_res[wv];
+    const uint32_t U = (b.cnt + 3) / 4;
+    for (uint32_t u0 = 0; u0 < U; u0 += 64) {
+        const uint32_t u = u0 + lane;
+        const uint32_t cnt = u < U ? min(4u, b.cnt - 4 * u) : 0u;
+        uint32_t v[4] = {0, 0, 0, 0};
+        if (!cnt) {
+        } else if (staged) {
+            accumulate_unit(st - 1, w, b.cnt, sb, u, v);  // fields start right after the (absent) width byte
+        } else if (w && inside) {
+            for (uint32_t c = 0; c < cnt; c++) {
+                const uint32_t e = 4 * u + c;
+                uint32_t m = 0;
+                for (uint32_t k = 0; k < B; k++)
+                    m |= pbyte(pl, size, sb + e * B + k) << (8 * k);
+                uint32_t lo = 0;
+                for (uint32_t bit = e * r, s2 = 0; s2 < r; s2++, bit++)
+                    lo = (lo << 1) | ((pbyte(pl, size, sb + B * b.cnt + (bit >> 3)) >> (7 - (bit & 7))) & 1u);
+                m = (m << r) | lo;
+                v[c] = ((pbyte(pl, size, e >> 3) >> (7 - (e & 7))) & 1u) ? 0u - m : m;
+            }
+        }
+        // the unit lanes' values go through LDS so all lanes write them out,
+        // one value each per step
+#pragma unroll
+        for (uint32_t c = 0; c < 4; c++)
+            res[4 * lane + c] = v[c];
+        wave_sync_lds();
+#pragma unroll
+        for (uint32_t c = 0; c < 4; c++) {
+            const uint32_t e = 4 * u0 + 64 * c + lane;
+            if (e < b.cnt) {
+                uint32_t lx, ly, lz;
+                value_pos(b, Bs, e, lx, ly, lz);
+                q[((uint64_t) (p.ox + lx) * a.g.d2 + (p.oy + ly)) * a.g.d3 + p.oz + lz] = (int32_t) res[64 * c + lane];
+            }
+        }
+        wave_sync_lds();
     }
 }
 
