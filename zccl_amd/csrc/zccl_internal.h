@@ -352,10 +352,14 @@ ZK_HD void emit_unit(uint8_t* buf, uint64_t fo, uint32_t w, uint32_t nj, uint32_
         const uint32_t h = (f.m[0] >> r) | ((f.m[1] >> r) << 8) | ((f.m[2] >> r) << 16) | ((f.m[3] >> r) << 24);
         __builtin_memcpy(buf + hi0 + 4 * u, &h, 4);
     } else if (B) {
-        for (uint32_t c = 0; c < cnt; c++) {
-            const uint32_t h = f.m[c] >> r;
-            for (uint32_t b = 0; b < B; b++)
-                buf[hi0 + (uint64_t) (4 * u + c) * B + b] = (uint8_t) (h >> (8 * b));
+        // constant indexes into f.m (a runtime bound would put f in scratch memory)
+#pragma unroll
+        for (uint32_t c = 0; c < 4; c++) {
+            if (c < cnt) {
+                const uint32_t h = f.m[c] >> r;
+                for (uint32_t b = 0; b < B; b++)
+                    buf[hi0 + (uint64_t) (4 * u + c) * B + b] = (uint8_t) (h >> (8 * b));
+            }
         }
     }
 }
