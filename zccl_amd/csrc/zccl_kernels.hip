@@ -1038,86 +1038,23 @@ __device__ __forceinline__ void accumulate_slot(const uint8_t* bp, uint32_t w, u
     }
 }
 
-// Add one operand's values to NB of the lane's slots.  The reads are batched
-// so a wave waits for LDS twice per batch rather than twice per slot: the NB
-// table entries, then the sign byte, high-byte word and low-bit word of every
-// slot, then the arithmetic (accumulate_slot's, branch-free for widths below
-// 16; a slot with no unit or width 0 adds nothing).  Reads of a dead slot land
-// inside the operand's staged bytes or its 16 bytes of slack.
-template <uint32_t NB>
-__device__ __forceinline__ void decode_slots(const uint8_t* ob, const uint32_t* tab, const uint32_t* jb,
-    const uint32_t* cnt, const uint32_t* nj, uint32_t u4, uint32_t uh, uint32_t nsh,
-    uint32_t (*acc)[4], uint32_t* negs)
-{
-    uint32_t e[NB];
-#pragma unroll
-    for (uint32_t i = 0; i < NB; i++)
-        e[i] = tab[jb[i]];
-    uint32_t sg[NB], H[NB], X[NB], w[NB], bit[NB];
-    const uint8_t* hp[NB];
-#pragma unroll
-    for (uint32_t i = 0; i < NB; i++) {
-        w[i] = cnt[i] ? e[i] >> 16 : 0u;
-        const uint8_t* bp = ob + (e[i] & 0xFFFFu);
-        const uint32_t B = w[i] >> 3, r = w[i] & 7;
-        hp[i] = bp + 1 + ((nj[i] + 7) >> 3);
-        bit[i] = __umul24(u4, r);
-        sg[i] = bp[1 + uh];
-        H[i] = lds_ld4(hp[i] + __umul24(u4, B));
-        X[i] = lds_ld4(hp[i] + __umul24(B, nj[i]) + (bit[i] >> 3));
-    }
-#pragma unroll
-    for (uint32_t i = 0; i < NB; i++) {
-        const uint32_t B = w[i] >> 3, r = w[i] & 7;
-        const uint32_t nib = w[i] ? __builtin_amdgcn_ubfe(sg[i], nsh, 4) : 0u;
-        negs[i] += __umul24(nib, 0x00204081u) & 0x01010101u;
-        uint32_t hs[4];
-#pragma unroll
-        for (int c = 0; c < 4; c++)
-            hs[c] = __builtin_amdgcn_ubfe(H[i], 8 * c, 8 * B) << r;
-        if (B >= 2) {
-            // widths of 16 and more: two to four high bytes per value
-            if (B == 2) {
-                const uint2 h2 = lds_ld8(hp[i] + 2 * u4);
-                hs[0] = (h2.x & 0xFFFFu) << r;
-                hs[1] = (h2.x >> 16) << r;
-                hs[2] = (h2.y & 0xFFFFu) << r;
-                hs[3] = (h2.y >> 16) << r;
-            } else {
-                const uint4 h4 = lds_ld16(hp[i] + __umul24(u4, B));
-                if (B == 3) {
-                    hs[0] = (h4.x & 0xFFFFFFu) << r;
-                    hs[1] = (__builtin_amdgcn_alignbyte(h4.y, h4.x, 3) & 0xFFFFFFu) << r;
-                    hs[2] = (__builtin_amdgcn_alignbyte(h4.z, h4.y, 2) & 0xFFFFFFu) << r;
-                    hs[3] = (h4.z >> 8) << r;
-                } else {
-                    hs[0] = h4.x; hs[1] = h4.y; hs[2] = h4.z; hs[3] = h4.w;
-                }
-            }
-        }
-        const uint32_t Xs = __builtin_bswap32(X[i]) << (bit[i] & 7);
-#pragma unroll
-        for (int c = 0; c < 4; c++) {
-            const uint32_t lo = __builtin_amdgcn_ubfe(Xs, 32 - r * (c + 1), r);
-            const uint32_t sm = (uint32_t) __builtin_amdgcn_sbfe((int32_t) nib, 3 - c, 1);
-            acc[i][c] += (hs[c] | lo) ^ sm;
-        }
-    }
-}
-
-// Add one operand's values to the lane's four slots, one slot at a time:
-// batching the reads of several slots (decode_slots<NB>) takes registers
-// that would cost a workgroup per CU, and residency hides more latency.
+// Add one operand's values to the lane's four slots: the four table entries
+// are read together, the data of each slot one slot at a time (batching the
+// data reads of all four slots as well was measured slower: it takes
+// registers that cost workgroups per CU, and residency hides more latency).
 __device__ __forceinline__ void decode_operand(const uint8_t* ob, const uint32_t* tab, const uint32_t (&jb)[4],
     const uint32_t (&cnt)[4], const uint32_t (&nj)[4], uint32_t u4, uint32_t uh, uint32_t nsh,
     uint32_t (&acc)[4][4], uint32_t (&negs)[4])
 {
+    uint32_t e[4];
+#pragma unroll
+    for (uint32_t i = 0; i < 4; i++)
+        e[i] = tab[jb[i]];
 #pragma unroll
     for (uint32_t i = 0; i < 4; i++) {
-        const uint32_t e = tab[jb[i]];
-        const uint32_t w = cnt[i] ? e >> 16 : 0u;
+        const uint32_t w = cnt[i] ? e[i] >> 16 : 0u;
         if (w)
-            accumulate_slot(ob + (e & 0xFFFFu), w, nj[i], (nj[i] + 7) >> 3, u4, uh, nsh, acc[i], negs[i]);
+            accumulate_slot(ob + (e[i] & 0xFFFFu), w, nj[i], (nj[i] + 7) >> 3, u4, uh, nsh, acc[i], negs[i]);
     }
 }
 
