@@ -220,8 +220,9 @@ def bench_sz3d(side, Bs, T, rel_eb, steps, warmup, cpu_seconds, with_cpu):
 def main():
     args = parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported "
-                                   "(multi-GPU sharding is described in DESIGN.md, not built)"}))
+        print(json.dumps({"error": "one GPU only: --gpus above 1 is not supported; the multi-GPU "
+                                   "segment exchange is designed but not built, and the value "
+                                   "at N=1 is one GPU's share of it"}))
         sys.exit(2)
 
     import numpy as np

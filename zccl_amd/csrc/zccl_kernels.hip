@@ -996,45 +996,56 @@ struct SlotShared {
 // complement of each negative value is counted in negs instead, one byte per
 // value (byte 3 - c counts value c; at most 32 operands, so no byte
 // overflows), and added once after all operands.  u4 = 4u, uh = u >> 1 and
-// nsh = (u & 1) ? 0 : 4 are the lane's unit constants.  Same fields and reads
-// as accumulate_unit: ZCCL/src/ZCCL_TypeManager.c:423-517 and :906-950.
+// nsh = (u & 1) ? 0 : 4 are the lane's unit constants.  The sign byte, the
+// unit's first 4 high bytes and the dword holding its low bits are read
+// together, so a slot of widths below 16 costs one LDS round trip; the reads
+// reach at most 20 bytes past the block's last byte.  Same fields as accumulate_unit:
+// ZCCL/src/ZCCL_TypeManager.c:423-517 and :906-950.
 __device__ __forceinline__ void accumulate_slot(const uint8_t* bp, uint32_t w, uint32_t nj, uint32_t sb,
     uint32_t u4, uint32_t uh, uint32_t nsh, uint32_t acc[4], uint32_t& negs)
 {
     const uint32_t B = w >> 3, r = w & 7;
-    const uint32_t nib = __builtin_amdgcn_ubfe((uint32_t) bp[1 + uh], nsh, 4);
-    negs += __umul24(nib, 0x00204081u) & 0x01010101u;  // bit 3 - c of nib -> byte 3 - c
     const uint8_t* hp = bp + 1 + sb;
+    const uint32_t bitpos = __umul24(u4, r);  // a multiple of 4
+    const uint8_t* hu = hp + __umul24(u4, B);
+    const uint32_t sbyte = bp[1 + uh];
+    const uint32_t H0 = lds_ld4(hu);
+    const uint32_t Xr = lds_ld4(hp + __umul24(B, nj) + (bitpos >> 3));
+    const uint32_t nib = __builtin_amdgcn_ubfe(sbyte, nsh, 4);
+    negs += __umul24(nib, 0x00204081u) & 0x01010101u;  // bit 3 - c of nib -> byte 3 - c
     uint32_t hs[4];
     if (B <= 1) {
-        const uint32_t H = lds_ld4(hp + __umul24(u4, B));
 #pragma unroll
         for (int c = 0; c < 4; c++)
-            hs[c] = __builtin_amdgcn_ubfe(H, 8 * c, 8 * B) << r;
+            hs[c] = __builtin_amdgcn_ubfe(H0, 8 * c, 8 * B);
     } else if (B == 2) {
-        const uint2 H = lds_ld8(hp + 2 * u4);
-        hs[0] = (H.x & 0xFFFFu) << r;
-        hs[1] = (H.x >> 16) << r;
-        hs[2] = (H.y & 0xFFFFu) << r;
-        hs[3] = (H.y >> 16) << r;
+        const uint2 H = lds_ld8(hu);
+        hs[0] = H.x & 0xFFFFu;
+        hs[1] = H.x >> 16;
+        hs[2] = H.y & 0xFFFFu;
+        hs[3] = H.y >> 16;
     } else {
-        const uint4 H = lds_ld16(hp + __umul24(u4, B));
+        // widths of 24 and more: rare, a second read
+        const uint4 W = lds_ld16(hu);
         if (B == 3) {
-            hs[0] = (H.x & 0xFFFFFFu) << r;
-            hs[1] = (__builtin_amdgcn_alignbyte(H.y, H.x, 3) & 0xFFFFFFu) << r;
-            hs[2] = (__builtin_amdgcn_alignbyte(H.z, H.y, 2) & 0xFFFFFFu) << r;
-            hs[3] = (H.z >> 8) << r;
+            hs[0] = W.x & 0xFFFFFFu;
+            hs[1] = __builtin_amdgcn_alignbyte(W.y, W.x, 3) & 0xFFFFFFu;
+            hs[2] = __builtin_amdgcn_alignbyte(W.z, W.y, 2) & 0xFFFFFFu;
+            hs[3] = W.z >> 8;
         } else {
-            hs[0] = H.x; hs[1] = H.y; hs[2] = H.z; hs[3] = H.w;
+            hs[0] = W.x; hs[1] = W.y; hs[2] = W.z; hs[3] = W.w;
         }
     }
-    const uint32_t bitpos = __umul24(u4, r);
-    const uint32_t X = __builtin_bswap32(lds_ld4(hp + __umul24(B, nj) + (bitpos >> 3))) << (bitpos & 7);
+    // the unit's 4r low bits, MSB-first from bit (bitpos & 7) of the
+    // big-endian dword: value c's field starts at bit 32 - (bitpos & 7) - r(c+1)
+    const uint32_t X = __builtin_bswap32(Xr);
+    uint32_t pos = 32u - (bitpos & 4u);
 #pragma unroll
     for (int c = 0; c < 4; c++) {
-        const uint32_t lo = __builtin_amdgcn_ubfe(X, 32 - r * (c + 1), r);
+        pos -= r;
+        const uint32_t m = (hs[c] << r) | __builtin_amdgcn_ubfe(X, pos, r);
         const uint32_t sm = (uint32_t) __builtin_amdgcn_sbfe((int32_t) nib, 3 - c, 1);  // 0 or ~0
-        acc[c] += (hs[c] | lo) ^ sm;
+        acc[c] += m ^ sm;
     }
 }
 
@@ -1358,7 +1369,7 @@ __global__ void __launch_bounds__(kHomsumThreads, 4) k_homsum(HomsumArgs a)
 // Sums whose blocks hold kSlotL values and whose tiles match the slot layout.
 constexpr uint32_t kSlotL = 36;
 
-__global__ void __launch_bounds__(kHomsumThreads, 4) k_homsum_slots(HomsumArgs a)
+__global__ void __launch_bounds__(kHomsumThreads, 6) k_homsum_slots(HomsumArgs a)
 {
     uint64_t t;
     unsigned long long* status;
@@ -1429,7 +1440,7 @@ __global__ void __launch_bounds__(kHomsumThreads, 4) k_homsum_batch(BatchArgs b)
     homsum_tile(b.g, b.tl, b.p, sio, t, status + b.job_tile0[jidx] * kStatusStride, b.ws, b.total_chains);
 }
 
-__global__ void __launch_bounds__(kHomsumThreads, 4) k_homsum_batch_slots(BatchArgs b)
+__global__ void __launch_bounds__(kHomsumThreads, 6) k_homsum_batch_slots(BatchArgs b)
 {
     __shared__ HomsumIO sio;
     unsigned long long* status;

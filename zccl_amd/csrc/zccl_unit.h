@@ -8,33 +8,38 @@
 
 namespace zk {
 
-// Unaligned LDS reads: gfx950 serves a byte-aligned ds_read_b32/b64/b128
-// as one instruction.
+// LDS reads at any byte address, built from 4-byte aligned dword reads and
+// byte shifts (v_alignbyte).  A ds_read at a misaligned address is legal on
+// gfx950 but slow: the fused sum ran 1.47x faster with the decoder's reads
+// aligned this way.  Each reads up to 4 bytes past the last byte it returns.
 __device__ __forceinline__ uint32_t lds_ld4(const uint8_t* p)
 {
-    uint32_t v;
-    __builtin_memcpy(&v, p, 4);
-    return v;
+    const uint32_t a = (uint32_t) (uintptr_t) p & 3u;
+    const uint32_t* q = (const uint32_t*) (p - a);
+    return __builtin_amdgcn_alignbyte(q[1], q[0], a);
 }
 
 __device__ __forceinline__ uint2 lds_ld8(const uint8_t* p)
 {
-    uint2 v;
-    __builtin_memcpy(&v, p, 8);
-    return v;
+    const uint32_t a = (uint32_t) (uintptr_t) p & 3u;
+    const uint32_t* q = (const uint32_t*) (p - a);
+    const uint32_t d0 = q[0], d1 = q[1], d2 = q[2];
+    return make_uint2(__builtin_amdgcn_alignbyte(d1, d0, a), __builtin_amdgcn_alignbyte(d2, d1, a));
 }
 
 __device__ __forceinline__ uint4 lds_ld16(const uint8_t* p)
 {
-    uint4 v;
-    __builtin_memcpy(&v, p, 16);
-    return v;
+    const uint32_t a = (uint32_t) (uintptr_t) p & 3u;
+    const uint32_t* q = (const uint32_t*) (p - a);
+    const uint32_t d0 = q[0], d1 = q[1], d2 = q[2], d3 = q[3], d4 = q[4];
+    return make_uint4(__builtin_amdgcn_alignbyte(d1, d0, a), __builtin_amdgcn_alignbyte(d2, d1, a),
+        __builtin_amdgcn_alignbyte(d3, d2, a), __builtin_amdgcn_alignbyte(d4, d3, a));
 }
 
 // Add the up-to-4 deltas of unit u of a block (width byte at bp, width w > 0,
 // nj elements, sb = ceil(nj/8) sign bytes) to acc[].  Magnitude = (high
 // bytes << r) | low r bits; the sign bit s (MSB-first bitmap) is applied as
-// (m ^ -s) + s.  Reads reach at most 16 bytes past the block's last byte.
+// (m ^ -s) + s.  Reads reach at most 20 bytes past the block's last byte.
 // ZCCL/src/ZCCL_TypeManager.c:423-517 (extract) and :906-950 (sign bitmap).
 __device__ __forceinline__ void accumulate_unit(const uint8_t* bp, uint32_t w, uint32_t nj, uint32_t sb,
     uint32_t u, uint32_t acc[4])
