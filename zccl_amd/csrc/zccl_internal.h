@@ -314,20 +314,10 @@ ZK_HD UnitFields unit_fields(uint32_t w, const uint32_t d[4])
 // Store the n (< 8) low bytes of v at p, any alignment.
 ZK_HD void put_bytes(uint8_t* p, uint64_t v, uint32_t n)
 {
-    if (n & 4) {
-        const uint32_t lo = (uint32_t) v;
-        __builtin_memcpy(p, &lo, 4);
-        p += 4;
-        v >>= 32;
-    }
-    if (n & 2) {
-        const uint16_t h = (uint16_t) v;
-        __builtin_memcpy(p, &h, 2);
-        p += 2;
-        v >>= 16;
-    }
-    if (n & 1)
-        *p = (uint8_t) v;
+    // byte stores: a misaligned multi-byte LDS store costs more than the bytes
+    for (uint32_t i = 0; i < 7; i++)
+        if (i < n)
+            p[i] = (uint8_t) (v >> (8 * i));
 }
 
 ZK_HD void emit_unit(uint8_t* buf, uint64_t fo, uint32_t w, uint32_t nj, uint32_t u, uint32_t U, uint32_t cnt,
@@ -350,7 +340,11 @@ ZK_HD void emit_unit(uint8_t* buf, uint64_t fo, uint32_t w, uint32_t nj, uint32_
     }
     if (B == 1 && cnt == 4) {
         const uint32_t h = (f.m[0] >> r) | ((f.m[1] >> r) << 8) | ((f.m[2] >> r) << 16) | ((f.m[3] >> r) << 24);
-        __builtin_memcpy(buf + hi0 + 4 * u, &h, 4);
+        uint8_t* q = buf + hi0 + 4 * u;
+        q[0] = (uint8_t) h;
+        q[1] = (uint8_t) (h >> 8);
+        q[2] = (uint8_t) (h >> 16);
+        q[3] = (uint8_t) (h >> 24);
     } else if (B) {
         // constant indexes into f.m (a runtime bound would put f in scratch memory)
 #pragma unroll

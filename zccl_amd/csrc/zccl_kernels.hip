@@ -633,9 +633,10 @@ __device__ __forceinline__ void compress_tile(const Geom& g, const Tiling& tl, c
     for (uint32_t i = 0; i < kUnitsPerThread; i++) {
         const uint32_t pk = un.pk[i], cnt = unit_cnt(pk);
         const uint32_t rel = 1 + unit_j(pk) * g.L + 4 * unit_u(pk);  // index of the unit's first value
-        float v[4], pv;
-        __builtin_memcpy(v, xs + 4u * rel, 16);
-        __builtin_memcpy(&pv, xs + 4u * (rel - 1), 4);
+        // dword reads: xs is 4-byte aligned, a 16-byte read from it is not
+        const float* xf = (const float*) (xs + 4u * rel);
+        const float v[4] = {xf[0], xf[1], xf[2], xf[3]};
+        const float pv = xf[-1];
         const uint32_t prev = cnt ? (uint32_t) quant(pv, p.inv) : 0u;
 #pragma unroll
         for (int c = 0; c < 4; c++)
