@@ -76,6 +76,21 @@ def test_random_shapes_equal_oracle(sz, T, Bs):
             2.0 ** -23 * float(np.abs(x).max())
 
 
+@pytest.mark.parametrize("dims,Bs,T", [((9, 10, 300), 4, 2), ((5, 7, 517), 4, 1), ((12, 9, 150), 8, 3),
+                                       ((17, 16, 64), 8, 1), ((8, 8, 65), 8, 1)])
+def test_chunked_columns_equal_oracle(sz, dims, Bs, T):
+    """Columns of blocks longer than one LDS chunk along z (256 values for
+    block side 4, 64 for 8), with partial last chunks and blocks: stream
+    bytes and decoded bits equal the oracle's."""
+    x = gen.smooth3d(dims, 7 * Bs + T) * np.float32(20)
+    say("chunked", dims, Bs, T)
+    s = sz.compress(torch.from_numpy(x).cuda(), Bs, 1e-3, T)
+    want = O.compress3d(x, dims, Bs, 1e-3, T)
+    assert np.array_equal(s.to_host(), want)
+    y = sz.decompress(s).cpu().numpy().reshape(-1)
+    assert np.array_equal(bits(y), bits(O.decompress3d(want, dims, Bs, 1e-3, T)))
+
+
 def test_special_values_and_wide_residuals(sz):
     """Zeros, huge values (widths up to 32), NaN and infinities (INT_MIN
     quantization) give the oracle's bytes."""

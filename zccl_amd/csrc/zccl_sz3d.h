@@ -35,6 +35,8 @@ struct Part3 {
     uint32_t zr0;          // first z-row (x, y pair) of the sub-volume in the z pass
     uint32_t yc0;          // first y-column (x, z pair) in the y pass
     uint32_t xc0;          // first x-column (y, z pair) in the x pass
+    uint32_t u0;           // first chunk of the tiled compress (block sides 4, 8)
+    uint32_t c0;           // first block column (bx, by) of the tiled decoder
 };
 
 struct Geom3 {
@@ -42,7 +44,14 @@ struct Geom3 {
     uint32_t Bs, T;
     uint32_t nblocks;      // all blocks of all sub-volumes
     uint32_t zrows, ycols, xcols;
+    uint32_t nunits;       // chunks of the tiled compress (0: no tiled path)
+    uint32_t ncols;        // block columns of the tiled decoder
 };
+
+// Blocks per chunk along z of the tiled kernels: a chunk is one (bx, by)
+// column of blocks cut to tile_nz3 blocks, whose quantized values with a
+// one-value halo fit in LDS.  0 where the tiled path has no instance.
+ZK_HD constexpr uint32_t tile_nz3(uint32_t Bs) { return Bs == 4 ? 64u : Bs == 8 ? 8u : 0u; }
 
 // Factorise T over three dims as My_Dims_create (ZCCL/src/ZCCL_multidim.c:
 // 100-140): prime factors of T, largest first, each multiplied into the
@@ -101,7 +110,8 @@ ZK_HD bool make_geom3(uint64_t d1, uint64_t d2, uint64_t d3, int Bs, int T, Geom
     g->T = (uint32_t) T;
     const uint32_t bx = g->d1 / td[0], rx = g->d1 % td[0], by = g->d2 / td[1], ry = g->d2 % td[1];
     const uint32_t bz = g->d3 / td[2], rz = g->d3 % td[2];
-    uint32_t r = 0, ox = 0, nb = 0, zr = 0, yc = 0, xc = 0;
+    const uint32_t nz = tile_nz3((uint32_t) Bs);
+    uint32_t r = 0, ox = 0, nb = 0, zr = 0, yc = 0, xc = 0, nu = 0, nc = 0;
     for (int px = 0; px < td[0]; px++) {
         const uint32_t sx = bx + ((uint32_t) px < rx ? 1u : 0u);
         uint32_t oy = 0;
@@ -119,6 +129,10 @@ ZK_HD bool make_geom3(uint64_t d1, uint64_t d2, uint64_t d3, int Bs, int T, Geom
                 p.zr0 = zr;
                 p.yc0 = yc;
                 p.xc0 = xc;
+                p.u0 = nu;
+                p.c0 = nc;
+                nu += nz ? p.b1 * p.b2 * ((p.b3 + nz - 1) / nz) : 0u;
+                nc += p.b1 * p.b2;
                 nb += p.b1 * p.b2 * p.b3;
                 zr += sx * sy;
                 yc += sx * sz;
@@ -133,6 +147,8 @@ ZK_HD bool make_geom3(uint64_t d1, uint64_t d2, uint64_t d3, int Bs, int T, Geom
     g->zrows = zr;
     g->ycols = yc;
     g->xcols = xc;
+    g->nunits = nu;
+    g->ncols = nc;
     return true;
 }
 

@@ -537,6 +537,330 @@ __global__ void __launch_bounds__(256) k3_xpass(Sz3Args a)
     }
 }
 
+// ------------------------------------------------------------ tiled path
+// Block sides 4 and 8.  A workgroup takes one chunk: a (bx, by) column of
+// blocks of one sub-volume, cut to Tile3::NZ blocks along z.  Compress
+// quantizes the chunk's values and a one-value halo (x - 1, y - 1, z - 1; zero
+// outside the sub-volume) once into LDS and takes every residual from there;
+// the decoder decodes a chunk's blocks into an LDS tile of z rows and scans
+// them along z there (carrying across the column's chunks), so only the y and
+// x passes remain.
+
+template <uint32_t BS>
+struct Tile3 {
+    static constexpr uint32_t NZ = tile_nz3(BS);             // blocks per chunk
+    static constexpr uint32_t WZ = NZ * BS;                  // z values per chunk
+    static constexpr uint32_t SZ = WZ + 1;                   // with the z halo
+    static constexpr uint32_t S1 = BS + 1;                   // x and y with the halo
+    static constexpr uint32_t U = BS * BS * BS / 4;          // units of a whole block
+    static constexpr uint32_t BPW = U >= 64 ? 1u : 64u / U;  // blocks per wave step
+    static constexpr uint32_t STEPS = U >= 64 ? U / 64 : 1u; // wave steps per block
+    static constexpr uint32_t UPR = BS / 4;                  // units per z row of a block
+    // worst-case payload (signs + 32-bit values), 16-byte multiple, + slack
+    static constexpr uint32_t PAY = ((BS * BS * BS + 7) / 8 + 4 * BS * BS * BS + 15) / 16 * 16 + 32;
+};
+
+__device__ __forceinline__ uint32_t part_of_unit(const Part3* parts, uint32_t T, uint32_t v, bool cols)
+{
+    uint32_t lo = 0, hi = T;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if ((cols ? parts[mid].c0 : parts[mid].u0) <= v)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// Quantized values of a chunk and its halo into qt[(i * S1 + j) * SZ + k]:
+// tile (i, j, k) holds q at local (bx Bs + i - 1, by Bs + j - 1, lz0 + k - 1).
+template <uint32_t BS>
+__device__ __forceinline__ void load_qtile(const Sz3Args& a, const Part3& p, uint32_t bx, uint32_t by, uint32_t lz0,
+    uint32_t wz, uint32_t ex, uint32_t ey, int32_t* qt)
+{
+    using C = Tile3<BS>;
+    const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (uint32_t row = wv; row < C::S1 * C::S1; row += 4) {
+        const uint32_t i = row / C::S1, j = row - i * C::S1;
+        if (i > ex || j > ey)
+            continue;
+        int32_t* dst = qt + row * C::SZ;
+        if ((i == 0 && bx == 0) || (j == 0 && by == 0)) {
+            for (uint32_t k = lane; k <= wz; k += 64)
+                dst[k] = 0;
+            continue;
+        }
+        const uint32_t lx = bx * BS + i - 1, ly = by * BS + j - 1;
+        const float* src = a.x + ((uint64_t) (p.ox + lx) * a.g.d2 + (p.oy + ly)) * a.g.d3 + p.oz + lz0;
+        for (uint32_t k = lane; k <= wz; k += 64)
+            dst[k] = (k == 0 && lz0 == 0) ? 0 : quant3(src[(int) k - 1], a.inv);
+    }
+}
+
+// Column term of the residual at tile z index tk: the x-y difference of the
+// four rows around (i, j) (value coordinates).
+template <uint32_t BS>
+__device__ __forceinline__ uint32_t col_term(const int32_t* qt, uint32_t i, uint32_t j, uint32_t tk)
+{
+    using C = Tile3<BS>;
+    const int32_t* a0 = qt + ((i + 1) * C::S1 + (j + 1)) * C::SZ + tk;
+    const int32_t* a2 = a0 - C::S1 * C::SZ;
+    return (uint32_t) a0[0] - (uint32_t) a0[-(int) C::SZ] - (uint32_t) a2[0] + (uint32_t) a2[-(int) C::SZ];
+}
+
+// Residuals of unit u of a block whose first value has chunk-local z kb:
+// returns the unit's value count (0..4) and its residuals in r[].
+template <uint32_t BS>
+__device__ __forceinline__ uint32_t unit_resid(const int32_t* qt, uint32_t ex, uint32_t ey, uint32_t ez, uint32_t cnt,
+    uint32_t kb, uint32_t u, uint32_t r[4])
+{
+    using C = Tile3<BS>;
+    const uint32_t n = 4 * u < cnt ? min(4u, cnt - 4 * u) : 0u;
+    if (!n)
+        return 0;
+    if (cnt == BS * BS * BS) {
+        // whole block: the unit is 4 consecutive values of one z row
+        const uint32_t i = u / (BS * C::UPR), j = (u / C::UPR) % BS, k0 = kb + (u % C::UPR) * 4;
+        uint32_t prev = col_term<BS>(qt, i, j, k0);
+#pragma unroll
+        for (uint32_t c = 0; c < 4; c++) {
+            const uint32_t cur = col_term<BS>(qt, i, j, k0 + c + 1);
+            r[c] = cur - prev;
+            prev = cur;
+        }
+    } else {
+        const uint32_t yz = ey * ez;
+#pragma unroll
+        for (uint32_t c = 0; c < 4; c++) {
+            r[c] = 0;
+            if (c < n) {
+                const uint32_t e = 4 * u + c, i = e / yz, rest = e - i * yz, j = rest / ez, k = rest - j * ez;
+                r[c] = col_term<BS>(qt, i, j, kb + k + 1) - col_term<BS>(qt, i, j, kb + k);
+            }
+        }
+    }
+    return n;
+}
+
+struct Chunk3 {
+    uint32_t bx, by, ch;   // block column and chunk along z
+    uint32_t ex, ey;       // extents of the column's blocks in x and y
+    uint32_t lz0, wz, nbz; // chunk's first local z, z values, blocks
+    uint32_t gcol;         // global index of the chunk's first block
+};
+
+template <uint32_t BS>
+__device__ __forceinline__ Chunk3 chunk_of(const Part3& p, uint32_t rel)
+{
+    using C = Tile3<BS>;
+    Chunk3 c;
+    const uint32_t nch = (p.b3 + C::NZ - 1) / C::NZ;
+    const uint32_t col = rel / nch;
+    c.ch = rel - col * nch;
+    c.bx = col / p.b2;
+    c.by = col - c.bx * p.b2;
+    c.ex = min(BS, p.sx - c.bx * BS);
+    c.ey = min(BS, p.sy - c.by * BS);
+    c.lz0 = c.ch * C::WZ;
+    c.wz = min(C::WZ, p.sz - c.lz0);
+    c.nbz = (c.wz + BS - 1) / BS;
+    c.gcol = p.g0 + (c.bx * p.b2 + c.by) * p.b3 + c.ch * C::NZ;
+    return c;
+}
+
+// Width and payload size of every block of a chunk.
+template <uint32_t BS>
+__global__ void __launch_bounds__(256) k3t_widths(Sz3Args a)
+{
+    using C = Tile3<BS>;
+    __shared__ int32_t qt[C::S1 * C::S1 * C::SZ];
+    const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const uint32_t t = part_of_unit(a.parts, a.g.T, blockIdx.x, false);
+    const Part3 p = a.parts[t];
+    const Chunk3 c = chunk_of<BS>(p, blockIdx.x - p.u0);
+    load_qtile<BS>(a, p, c.bx, c.by, c.lz0, c.wz, c.ex, c.ey, qt);
+    __syncthreads();
+    const uint32_t ul = C::BPW > 1 ? lane % C::U : lane;
+    for (uint32_t base = wv * C::BPW; base < c.nbz; base += 4 * C::BPW) {
+        const uint32_t bzl = base + (C::BPW > 1 ? lane / C::U : 0u);
+        const uint32_t ez = bzl < c.nbz ? min(BS, c.wz - bzl * BS) : 0u;
+        const uint32_t cnt = c.ex * c.ey * ez;
+        uint32_t m = 0;
+        for (uint32_t s = 0; s < C::STEPS; s++) {
+            uint32_t r[4];
+            const uint32_t n = unit_resid<BS>(qt, c.ex, c.ey, ez, cnt, bzl * BS, C::BPW > 1 ? ul : s * 64 + lane, r);
+#pragma unroll
+            for (uint32_t k = 0; k < 4; k++)
+                if (k < n)
+                    m = max(m, mag32(r[k]));
+        }
+        if (C::BPW > 1) {
+#pragma unroll
+            for (uint32_t d = C::U / 2; d >= 1; d >>= 1)
+                m = max(m, (uint32_t) __shfl_xor(m, d, 64));
+        } else {
+            m = wave_max(m);
+        }
+        if (ul == 0 && bzl < c.nbz) {
+            const uint32_t w = m ? 32u - (uint32_t) __clz((int) m) : 0u;
+            a.words[c.gcol + bzl] = payload3(w, cnt) | (w << 26);
+        }
+    }
+}
+
+// Header byte, sub-volume offset and payload of every block of a chunk; the
+// payloads are built in LDS by the unit encoder of the 1-D sum and stored with
+// dword stores that cover only the block's own bytes.
+template <uint32_t BS>
+__global__ void __launch_bounds__(256) k3t_encode(Sz3Args a)
+{
+    using C = Tile3<BS>;
+    __shared__ int32_t qt[C::S1 * C::S1 * C::SZ];
+    __shared__ uint32_t s_pay[4][C::BPW][C::PAY / 4];
+    if (a.status[1] & ZK_ERR_NOSPACE)
+        return;
+    const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const uint32_t t = part_of_unit(a.parts, a.g.T, blockIdx.x, false);
+    const Part3 p = a.parts[t];
+    const uint32_t rel = blockIdx.x - p.u0;
+    const Chunk3 c = chunk_of<BS>(p, rel);
+    const uint64_t hdr = 8ull * a.g.T;
+    const uint64_t sub = (uint64_t) p.nbc + a.offs[p.g0];  // sub-volume start after the header
+    if (rel == 0 && threadIdx.x == 0)
+        __builtin_memcpy(a.out + 8ull * t, &sub, 8);
+    load_qtile<BS>(a, p, c.bx, c.by, c.lz0, c.wz, c.ex, c.ey, qt);
+    __syncthreads();
+    const uint32_t ul = C::BPW > 1 ? lane % C::U : lane;
+    const uint32_t step = C::BPW > 1 ? C::U : 64u;
+    uint8_t* st = (uint8_t*) s_pay[wv][C::BPW > 1 ? lane / C::U : 0u];
+    for (uint32_t base = wv * C::BPW; base < c.nbz; base += 4 * C::BPW) {
+        const uint32_t bzl = base + (C::BPW > 1 ? lane / C::U : 0u);
+        const bool valid = bzl < c.nbz;
+        const uint32_t gb = c.gcol + bzl;
+        const uint32_t word = valid ? a.words[gb] : 0u, w = word >> 26, size = word & kSizeMask;
+        const uint32_t ez = valid ? min(BS, c.wz - bzl * BS) : 0u;
+        const uint32_t cnt = c.ex * c.ey * ez, Ub = (cnt + 3) / 4;
+        if (valid && ul == 0)
+            a.out[hdr + sub + (gb - p.g0)] = (uint8_t) w;
+        for (uint32_t s = 0; s < C::STEPS; s++) {
+            const uint32_t u = C::BPW > 1 ? ul : s * 64 + lane;
+            uint32_t d[4] = {0, 0, 0, 0};
+            const uint32_t n = w ? unit_resid<BS>(qt, c.ex, c.ey, ez, cnt, bzl * BS, u, d) : 0u;
+            const UnitFields f = unit_fields(w, d);
+            const uint32_t pnib = __shfl_down(f.nib, 1, 64);
+            const uint32_t plv = __shfl_down(f.lv, 1, 64);
+            emit_unit(st, 0, w, cnt, u, Ub, n, f, pnib, plv);
+        }
+        wave_sync_lds();
+        if (valid && w) {
+            uint8_t* g = a.out + hdr + p.nbc + (uint64_t) p.b1 * p.b2 * p.b3 + a.offs[gb];
+            const uint32_t nw = size / 4;
+            for (uint32_t i = ul; i < nw; i += step) {
+                const uint32_t v = ((const uint32_t*) st)[i];
+                __builtin_memcpy(g + 4 * i, &v, 4);
+            }
+            for (uint32_t i = 4 * nw + ul; i < size; i += step)
+                g[i] = st[i];
+        }
+        wave_sync_lds();
+    }
+}
+
+// Residuals of a block column, chunk by chunk: blocks decoded into an LDS tile
+// of z rows, each row scanned along z (carried across chunks) and written to
+// the output buffer as int32.  ZCCL/src/ZCCL_TypeManager.c:423-517, :906-950.
+template <uint32_t BS>
+__global__ void __launch_bounds__(256) k3t_decode(Sz3Args a)
+{
+    using C = Tile3<BS>;
+    __shared__ uint4 rt4[BS * BS * C::WZ / 4];
+    __shared__ uint32_t s_pay[4][C::BPW][C::PAY / 4];
+    __shared__ uint32_t s_carry[BS * BS];
+    uint32_t* rt = (uint32_t*) rt4;
+    const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const uint32_t t = part_of_unit(a.parts, a.g.T, blockIdx.x, true);
+    const Part3 p = a.parts[t];
+    const uint32_t nch = (p.b3 + C::NZ - 1) / C::NZ;
+    uint64_t h;
+    __builtin_memcpy(&h, a.in + 8ull * t, 8);
+    const uint64_t pbase = 8ull * a.g.T + h + (uint64_t) p.b1 * p.b2 * p.b3;
+    const unsigned long long o0 = a.offs[p.g0];
+    for (uint32_t i = threadIdx.x; i < BS * BS; i += 256)
+        s_carry[i] = 0;
+    const uint32_t ul = C::BPW > 1 ? lane % C::U : lane;
+    const uint32_t step = C::BPW > 1 ? C::U : 64u;
+    uint8_t* st = (uint8_t*) s_pay[wv][C::BPW > 1 ? lane / C::U : 0u];
+    for (uint32_t ch = 0; ch < nch; ch++) {
+        const Chunk3 c = chunk_of<BS>(p, (blockIdx.x - p.c0) * nch + ch);
+        for (uint32_t base = wv * C::BPW; base < c.nbz; base += 4 * C::BPW) {
+            const uint32_t bzl = base + (C::BPW > 1 ? lane / C::U : 0u);
+            const bool valid = bzl < c.nbz;
+            const uint32_t gb = c.gcol + bzl;
+            const uint32_t word = valid ? a.words[gb] : 0u, w = word >> 26, size = word & kSizeMask;
+            const uint32_t ez = valid ? min(BS, c.wz - bzl * BS) : 0u;
+            const uint32_t cnt = c.ex * c.ey * ez, sb = (cnt + 7) / 8;
+            const uint64_t fo = pbase + ((valid ? a.offs[gb] : o0) - o0);
+            const bool inside = fo + size <= a.in_size;
+            if (valid && w && !inside && ul == 0)
+                raise3(a.status, ZK_ERR_OVERRUN);
+            const bool ok = valid && w && inside;
+            if (ok) {
+                const uint8_t* pl = a.in + fo;
+                const uint32_t nw = size / 4;
+                for (uint32_t i = ul; i < nw; i += step) {
+                    uint32_t v;
+                    __builtin_memcpy(&v, pl + 4 * i, 4);
+                    ((uint32_t*) st)[i] = v;
+                }
+                for (uint32_t i = 4 * nw + ul; i < size; i += step)
+                    st[i] = pl[i];
+            }
+            wave_sync_lds();
+            for (uint32_t s = 0; s < C::STEPS; s++) {
+                const uint32_t u = C::BPW > 1 ? ul : s * 64 + lane;
+                const uint32_t n = 4 * u < cnt ? min(4u, cnt - 4 * u) : 0u;
+                uint32_t v[4] = {0, 0, 0, 0};
+                if (ok && n)
+                    accumulate_unit(st - 1, w, cnt, sb, u, v);  // fields start right after the (absent) width byte
+                if (n == 4 && cnt == BS * BS * BS) {
+                    const uint32_t i = u / (BS * C::UPR), j = (u / C::UPR) % BS, k0 = bzl * BS + (u % C::UPR) * 4;
+                    rt4[((i * BS + j) * C::WZ + k0) / 4] = make_uint4(v[0], v[1], v[2], v[3]);
+                } else if (n) {
+                    const uint32_t yz = c.ey * ez;
+#pragma unroll
+                    for (uint32_t k = 0; k < 4; k++) {
+                        if (k < n) {
+                            const uint32_t e = 4 * u + k, i = e / yz, rest = e - i * yz, j = rest / ez;
+                            rt[(i * BS + j) * C::WZ + bzl * BS + (rest - j * ez)] = v[k];
+                        }
+                    }
+                }
+            }
+            wave_sync_lds();
+        }
+        __syncthreads();
+        for (uint32_t row = wv; row < c.ex * c.ey; row += 4) {
+            const uint32_t i = row / c.ey, j = row - i * c.ey;
+            const uint32_t* src = rt + (i * BS + j) * C::WZ;
+            uint32_t carry = s_carry[i * BS + j];
+            uint32_t* q = (uint32_t*) a.y + ((uint64_t) (p.ox + c.bx * BS + i) * a.g.d2 + (p.oy + c.by * BS + j)) * a.g.d3 +
+                p.oz + c.lz0;
+            for (uint32_t k0 = 0; k0 < c.wz; k0 += 64) {
+                const uint32_t k = k0 + lane;
+                const uint32_t v = k < c.wz ? src[k] : 0u;
+                const uint32_t inc = wave_incl_scan32(v) + carry;
+                if (k < c.wz)
+                    q[k] = inc;
+                carry = __shfl(inc, 63, 64);
+            }
+            if (lane == 0)
+                s_carry[i * BS + j] = carry;
+        }
+        __syncthreads();
+    }
+}
+
 }  // namespace
 
 static hipError_t scan3(const Sz3Args& a, int check_cap, hipStream_t s)
@@ -554,8 +878,36 @@ hipError_t launch_sz3_table(const Geom3& g, Part3* parts, unsigned long long* st
     return hipGetLastError();
 }
 
+template <uint32_t BS>
+static hipError_t compress_tiled(const Sz3Args& a, hipStream_t s)
+{
+    hipLaunchKernelGGL(k3t_widths<BS>, dim3(a.g.nunits), dim3(256), 0, s, a);
+    hipError_t e = scan3(a, 1, s);
+    if (e != hipSuccess)
+        return e;
+    hipLaunchKernelGGL(k3t_encode<BS>, dim3(a.g.nunits), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+template <uint32_t BS>
+static hipError_t decompress_tiled(const Sz3Args& a, hipStream_t s)
+{
+    hipLaunchKernelGGL(k3_dsizes, dim3((uint32_t) ceil_div(a.g.nblocks, 256)), dim3(256), 0, s, a);
+    hipError_t e = scan3(a, 0, s);
+    if (e != hipSuccess)
+        return e;
+    hipLaunchKernelGGL(k3t_decode<BS>, dim3(a.g.ncols), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k3_ypass, dim3((uint32_t) ceil_div(a.g.ycols, 256)), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k3_xpass, dim3((uint32_t) ceil_div(a.g.xcols, 256)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
 hipError_t launch_sz3_compress(const Sz3Args& a, hipStream_t s)
 {
+    if (a.g.nunits && a.g.Bs == 4)
+        return compress_tiled<4>(a, s);
+    if (a.g.nunits && a.g.Bs == 8)
+        return compress_tiled<8>(a, s);
     const uint32_t waves = (uint32_t) ceil_div(a.g.nblocks, 4);
     hipLaunchKernelGGL(k3_widths, dim3(waves), dim3(256), 0, s, a);
     hipError_t e = scan3(a, 1, s);
@@ -567,6 +919,10 @@ hipError_t launch_sz3_compress(const Sz3Args& a, hipStream_t s)
 
 hipError_t launch_sz3_decompress(const Sz3Args& a, hipStream_t s)
 {
+    if (a.g.nunits && a.g.Bs == 4)
+        return decompress_tiled<4>(a, s);
+    if (a.g.nunits && a.g.Bs == 8)
+        return decompress_tiled<8>(a, s);
     hipLaunchKernelGGL(k3_dsizes, dim3((uint32_t) ceil_div(a.g.nblocks, 256)), dim3(256), 0, s, a);
     hipError_t e = scan3(a, 0, s);
     if (e != hipSuccess)
