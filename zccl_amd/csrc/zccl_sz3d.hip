@@ -575,26 +575,47 @@ __device__ __forceinline__ uint32_t part_of_unit(const Part3* parts, uint32_t T,
 
 // Quantized values of a chunk and its halo into qt[(i * S1 + j) * SZ + k]:
 // tile (i, j, k) holds q at local (bx Bs + i - 1, by Bs + j - 1, lz0 + k - 1).
+// A wave takes RG rows at a time and issues all their loads before it
+// quantizes any, so each wave keeps up to RG * KS loads in flight.
 template <uint32_t BS>
 __device__ __forceinline__ void load_qtile(const Sz3Args& a, const Part3& p, uint32_t bx, uint32_t by, uint32_t lz0,
     uint32_t wz, uint32_t ex, uint32_t ey, int32_t* qt)
 {
     using C = Tile3<BS>;
+    constexpr uint32_t KS = (C::SZ + 63) / 64, RG = 4;
     const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    for (uint32_t row = wv; row < C::S1 * C::S1; row += 4) {
-        const uint32_t i = row / C::S1, j = row - i * C::S1;
-        if (i > ex || j > ey)
-            continue;
-        int32_t* dst = qt + row * C::SZ;
-        if ((i == 0 && bx == 0) || (j == 0 && by == 0)) {
-            for (uint32_t k = lane; k <= wz; k += 64)
-                dst[k] = 0;
-            continue;
+    for (uint32_t r0 = RG * wv; r0 < C::S1 * C::S1; r0 += 4 * RG) {
+        float v[RG][KS];
+        bool live[RG];
+#pragma unroll
+        for (uint32_t h = 0; h < RG; h++) {
+            const uint32_t row = r0 + h, i = row / C::S1, j = row - i * C::S1;
+            live[h] = row < C::S1 * C::S1 && i <= ex && j <= ey && (i > 0 || bx > 0) && (j > 0 || by > 0);
+            // element of (i - 1, j - 1, lz0) in the volume; every load below is
+            // unconditional (clamped into the row, or element 0 for a dead row)
+            // so the compiler keeps them all in flight
+            const uint64_t rb = live[h] ?
+                ((uint64_t) (p.ox + bx * BS + i - 1) * a.g.d2 + (p.oy + by * BS + j - 1)) * a.g.d3 + p.oz + lz0 : 0ull;
+#pragma unroll
+            for (uint32_t s = 0; s < KS; s++) {
+                const uint32_t kk = min(lane + 64 * s, wz);
+                const uint64_t e = rb + kk - ((kk > 0 || lz0 > 0) ? 1u : 0u);
+                v[h][s] = a.x[live[h] ? e : 0ull];
+            }
         }
-        const uint32_t lx = bx * BS + i - 1, ly = by * BS + j - 1;
-        const float* src = a.x + ((uint64_t) (p.ox + lx) * a.g.d2 + (p.oy + ly)) * a.g.d3 + p.oz + lz0;
-        for (uint32_t k = lane; k <= wz; k += 64)
-            dst[k] = (k == 0 && lz0 == 0) ? 0 : quant3(src[(int) k - 1], a.inv);
+#pragma unroll
+        for (uint32_t h = 0; h < RG; h++) {
+            const uint32_t row = r0 + h;
+            if (row >= C::S1 * C::S1)
+                continue;
+            int32_t* dst = qt + row * C::SZ;
+#pragma unroll
+            for (uint32_t s = 0; s < KS; s++) {
+                const uint32_t k = lane + 64 * s;
+                if (k <= wz)
+                    dst[k] = live[h] && (k > 0 || lz0 > 0) ? quant3(v[h][s], a.inv) : 0;
+            }
+        }
     }
 }
 
@@ -806,12 +827,23 @@ __global__ void __launch_bounds__(256) k3t_decode(Sz3Args a)
                 raise3(a.status, ZK_ERR_OVERRUN);
             const bool ok = valid && w && inside;
             if (ok) {
+                // all of the payload's dword loads in flight before any store
+                constexpr uint32_t PS = (C::PAY / 4 + (C::BPW > 1 ? C::U : 64u) - 1) / (C::BPW > 1 ? C::U : 64u);
                 const uint8_t* pl = a.in + fo;
                 const uint32_t nw = size / 4;
-                for (uint32_t i = ul; i < nw; i += step) {
-                    uint32_t v;
-                    __builtin_memcpy(&v, pl + 4 * i, 4);
-                    ((uint32_t*) st)[i] = v;
+                uint32_t v[PS];
+                // unconditional loads (clamped into the payload, or the stream's
+                // first bytes when it has no whole dword) stay in flight together
+#pragma unroll
+                for (uint32_t m = 0; m < PS; m++) {
+                    const uint32_t i = min(ul + m * step, nw ? nw - 1 : 0u);
+                    __builtin_memcpy(&v[m], nw ? pl + 4 * i : a.in, 4);
+                }
+#pragma unroll
+                for (uint32_t m = 0; m < PS; m++) {
+                    const uint32_t i = ul + m * step;
+                    if (i < nw)
+                        ((uint32_t*) st)[i] = v[m];
                 }
                 for (uint32_t i = 4 * nw + ul; i < size; i += step)
                     st[i] = pl[i];
