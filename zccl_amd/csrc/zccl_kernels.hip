@@ -1054,17 +1054,19 @@ __device__ __forceinline__ void accumulate_slot(const uint8_t* bp, uint32_t w, u
 // are read together, the data of each slot one slot at a time (batching the
 // data reads of all four slots as well was measured slower: it takes
 // registers that cost workgroups per CU, and residency hides more latency).
-__device__ __forceinline__ void decode_operand(const uint8_t* ob, const uint32_t* tab, const uint32_t (&jb)[4],
-    const uint32_t (&cnt)[4], const uint32_t (&nj)[4], uint32_t u4, uint32_t uh, uint32_t nsh,
-    uint32_t (&acc)[4][4], uint32_t (&negs)[4])
+// tab points at the table entry of the lane's block in slot 0; slot i's
+// block is BPS entries further.
+template <uint32_t BPS>
+__device__ __forceinline__ void decode_operand(const uint8_t* ob, const uint32_t* tab, const uint32_t (&nj)[4],
+    uint32_t u4, uint32_t uh, uint32_t nsh, uint32_t (&acc)[4][4], uint32_t (&negs)[4])
 {
     uint32_t e[4];
 #pragma unroll
     for (uint32_t i = 0; i < 4; i++)
-        e[i] = tab[jb[i]];
+        e[i] = tab[i * BPS];
 #pragma unroll
     for (uint32_t i = 0; i < 4; i++) {
-        const uint32_t w = cnt[i] ? e[i] >> 16 : 0u;
+        const uint32_t w = u4 < nj[i] ? e[i] >> 16 : 0u;
         if (w)
             accumulate_slot(ob + (e[i] & 0xFFFFu), w, nj[i], (nj[i] + 7) >> 3, u4, uh, nsh, acc[i], negs[i]);
     }
@@ -1101,18 +1103,20 @@ __device__ __forceinline__ void homsum_tile_slots(const Geom& g, const Tiling& t
     uint32_t* s_wo = zk_lds + p.stage_bytes / 4;
     uint8_t* stage8 = (uint8_t*) stage;
     // unit of this lane inside its block, the same in all four slots
-    const uint32_t jj = lane / M::U, u = lane - jj * M::U;
-    const uint32_t u4 = 4 * u, uh = u >> 1, nsh = (u & 1) ? 0u : 4u;
+    // (the idle lanes past the wave's BPS blocks take unit 0 of the slots'
+    // first blocks with no values)
     const bool on = lane < M::ACTIVE;
+    const uint32_t jj = on ? lane / M::U : 0u, u = on ? lane - jj * M::U : 0u;
+    const uint32_t u4 = 4 * u, uh = u >> 1, nsh = (u & 1) ? 0u : 4u;
     const uint32_t last_len = x.nblk ? tile_block_len(g, x, x.nblk - 1) : 0u;
-    uint32_t jb[4], nj[4], cnt[4];
+    // slot i's block is jb0 + i * BPS; nj[i] its values (0: no block), and
+    // the lane's value count min(4, nj - 4u) is derived where needed
+    const uint32_t jb0 = wv * 4 * M::BPS + jj;
+    uint32_t nj[4];
 #pragma unroll
     for (uint32_t i = 0; i < 4; i++) {
-        const uint32_t j = (wv * 4 + i) * M::BPS + jj;
-        const bool live = on && j < x.nblk;
-        jb[i] = live ? j : 0u;
-        nj[i] = live ? (j == x.nblk - 1 ? last_len : L) : 0u;
-        cnt[i] = 4 * u < nj[i] ? min(4u, nj[i] - 4 * u) : 0u;
+        const uint32_t j = jb0 + i * M::BPS;
+        nj[i] = on && j < x.nblk ? (j == x.nblk - 1 ? last_len : L) : 0u;
     }
     for (uint32_t j = tid; j < G; j += blockDim.x)
         sh.bmax[j] = 0;
@@ -1164,8 +1168,8 @@ __device__ __forceinline__ void homsum_tile_slots(const Geom& g, const Tiling& t
         for (int r = r0; r < r1; r++) {
             if (__builtin_amdgcn_readlane(op_bad, r))
                 continue;
-            decode_operand(stage8 + __builtin_amdgcn_readlane(op_lds, r), s_wo + r * G, jb, cnt, nj, u4, uh, nsh,
-                acc, negs);
+            decode_operand<M::BPS>(stage8 + __builtin_amdgcn_readlane(op_lds, r), s_wo + r * G + jb0, nj, u4, uh,
+                nsh, acc, negs);
         }
         __syncthreads();  // this round's half is free
         if (round + 1 < sp.nrounds && !overlap)
@@ -1185,16 +1189,17 @@ __device__ __forceinline__ void homsum_tile_slots(const Geom& g, const Tiling& t
     // end decoded garbage: clear them; then block maxima
 #pragma unroll
     for (uint32_t i = 0; i < 4; i++) {
+        const uint32_t cnt = u4 < nj[i] ? min(4u, nj[i] - u4) : 0u;
         uint32_t umax = 0;
 #pragma unroll
         for (int c = 0; c < 4; c++) {
             acc[i][c] += (negs[i] >> (8 * (3 - c))) & 0xFFu;
-            if ((uint32_t) c >= cnt[i])
+            if ((uint32_t) c >= cnt)
                 acc[i][c] = 0;
             umax = max(umax, umag((int32_t) acc[i][c]));
         }
         if (umax)
-            atomicMax(&sh.bmax[jb[i]], umax);
+            atomicMax(&sh.bmax[jb0 + i * M::BPS], umax);
     }
     __syncthreads();
     if (tid < 64) {
@@ -1228,8 +1233,8 @@ __device__ __forceinline__ void homsum_tile_slots(const Geom& g, const Tiling& t
     const uint32_t total = (uint32_t) sh.bc[0];
 #pragma unroll
     for (uint32_t i = 0; i < 4; i++) {
-        const uint32_t j = jb[i];
-        encode_unit_pair<M::U>(stage8, sh.boff[j], cnt[i] ? sh.width[j] : 0u, nj[i], u, cnt[i], acc[i]);
+        const uint32_t j = jb0 + i * M::BPS, cnt = u4 < nj[i] ? min(4u, nj[i] - u4) : 0u;
+        encode_unit_pair<M::U>(stage8, sh.boff[j], cnt ? sh.width[j] : 0u, nj[i], u, cnt, acc[i]);
     }
     finish_tile(g, x, t, sh, stage, total, status, ws, seed, io.out, io.out_cap, io.index, io.size_out,
         t == tl.ntiles - 1, nchains);
@@ -1452,7 +1457,9 @@ __global__ void __launch_bounds__(kHomsumThreads, 6) k_homsum_batch_slots(BatchA
     uint64_t t;
     job_of(b, tg, jidx, t);
     stage_desc(b.jobs, jidx, &sio);
-    homsum_tile_slots<kSlotL>(b.g, b.tl, b.p, sio, t, status + b.job_tile0[jidx] * kStatusStride, b.ws, b.total_chains);
+    // a uniform batch's jobs have equal tile counts: job j's tiles start at j * ntiles
+    homsum_tile_slots<kSlotL>(b.g, b.tl, b.p, sio, t, status + (uint64_t) jidx * b.tl.ntiles * kStatusStride, b.ws,
+        b.total_chains);
 }
 
 __global__ void __launch_bounds__(kCompressThreads) k_compress_batch(BatchArgs b)
