@@ -273,6 +273,29 @@ def test_errors_are_reported_not_faulted(codec, capi):
     assert capi.zccl_last_error() != 0
 
 
+@pytest.mark.parametrize("L", [36, 32])
+def test_sum_reports_width_byte_that_disagrees_with_index(codec, L):
+    """A stream byte changed under its index (a block's width byte set to
+    another valid width) is reported by the sum (slot layout for L = 36, the
+    generic kernel otherwise), not summed silently."""
+    n, eb, T = 60_000, 1e-3, 4
+    say("width mismatch", L)
+    x = gen.make("grf", n, 5)
+    y = gen.perturb(x, 6)
+    sx = codec.compress(dev(x), eb, L, T)
+    sy = codec.compress(dev(y), eb, L, T)
+    torch.cuda.synchronize()
+    codec.check_errors()
+    cx = sx.to_host()
+    w, bo, so, tot = O.parse(cx, n, L, T)
+    j = int(np.nonzero(w)[0][7])
+    sx.data[int(bo[j])] = (int(w[j]) % 32) + 1
+    codec.homsum([sx, sy])
+    torch.cuda.synchronize()
+    with pytest.raises(Exception):
+        codec.check_errors()
+
+
 # ------------------------------------------------------------ batch plans
 
 def test_batch_plans_uniform_and_mixed(codec):

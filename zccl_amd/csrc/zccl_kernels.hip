@@ -1055,10 +1055,12 @@ __device__ __forceinline__ void accumulate_slot(const uint8_t* bp, uint32_t w, u
 // data reads of all four slots as well was measured slower: it takes
 // registers that cost workgroups per CU, and residency hides more latency).
 // tab points at the table entry of the lane's block in slot 0; slot i's
-// block is BPS entries further.
+// block is BPS entries further.  Every block's width byte in the stream must
+// equal its index width: the byte is read with the slot's other reads and a
+// mismatch sets `bad`.
 template <uint32_t BPS>
 __device__ __forceinline__ void decode_operand(const uint8_t* ob, const uint32_t* tab, const uint32_t (&nj)[4],
-    uint32_t u4, uint32_t uh, uint32_t nsh, uint32_t (&acc)[4][4], uint32_t (&negs)[4])
+    uint32_t u4, uint32_t uh, uint32_t nsh, uint32_t (&acc)[4][4], uint32_t (&negs)[4], bool& bad)
 {
     uint32_t e[4];
 #pragma unroll
@@ -1066,9 +1068,13 @@ __device__ __forceinline__ void decode_operand(const uint8_t* ob, const uint32_t
         e[i] = tab[i * BPS];
 #pragma unroll
     for (uint32_t i = 0; i < 4; i++) {
-        const uint32_t w = u4 < nj[i] ? e[i] >> 16 : 0u;
+        const bool has = u4 < nj[i];
+        const uint32_t w = has ? e[i] >> 16 : 0u;
+        const uint8_t* bp = ob + (e[i] & 0xFFFFu);  // inside the operand's staged bytes for any table entry
+        const uint32_t wb = bp[0];
         if (w)
-            accumulate_slot(ob + (e[i] & 0xFFFFu), w, nj[i], (nj[i] + 7) >> 3, u4, uh, nsh, acc[i], negs[i]);
+            accumulate_slot(bp, w, nj[i], (nj[i] + 7) >> 3, u4, uh, nsh, acc[i], negs[i]);
+        bad |= has && wb != (e[i] >> 16);
     }
 }
 
@@ -1145,6 +1151,7 @@ __device__ __forceinline__ void homsum_tile_slots(const Geom& g, const Tiling& t
     __syncthreads();
     const uint32_t op_lds = lane < (uint32_t) p.k ? sp.lds[lane] + op_shift : 0u;
     const uint32_t op_bad = lane < (uint32_t) p.k ? os.bad[lane] : 1u;
+    bool bad = false;  // a staged width byte differs from its index width
     issue_round(io.in, os, sp, 0, stage8);
     for (int round = 0; round < sp.nrounds; round++) {
         __builtin_amdgcn_s_waitcnt(0);
@@ -1153,28 +1160,18 @@ __device__ __forceinline__ void homsum_tile_slots(const Geom& g, const Tiling& t
         if (overlap)
             issue_round(io.in, os, sp, round + 1, stage8);  // lands in the other half during the decode
         const int r0 = round ? sp.gend[round - 1] : 0, r1 = sp.gend[round];
-        // every staged block's width byte must equal the index width
-        bool bad = false;
-        for (int r = r0 + (int) wv; r < r1; r += 4) {
-            if (os.bad[r])
-                continue;
-            for (uint32_t j = lane; j < x.nblk; j += 64) {
-                const uint32_t e = s_wo[r * G + j];
-                bad |= stage8[os.lds[r] + (e & 0xFFFFu)] != (e >> 16);
-            }
-        }
-        if (bad)
-            raise_error(ws, ZK_ERR_MISMATCH);
         for (int r = r0; r < r1; r++) {
             if (__builtin_amdgcn_readlane(op_bad, r))
                 continue;
             decode_operand<M::BPS>(stage8 + __builtin_amdgcn_readlane(op_lds, r), s_wo + r * G + jb0, nj, u4, uh,
-                nsh, acc, negs);
+                nsh, acc, negs, bad);
         }
         __syncthreads();  // this round's half is free
         if (round + 1 < sp.nrounds && !overlap)
             issue_round(io.in, os, sp, round + 1, stage8);
     }
+    if (bad)
+        raise_error(ws, ZK_ERR_MISMATCH);
     uint32_t seed = 0;
     if (x.lt == 0 && tid == 0) {
         for (int r = 0; r < p.k; r++) {
