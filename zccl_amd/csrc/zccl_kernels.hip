@@ -1269,19 +1269,28 @@ __device__ __forceinline__ void decompress_tile(const Geom& g, const Tiling& tl,
     uint32_t pre[kUnitsPerThread][4], us[kUnitsPerThread];
     const uint32_t last_j = x.nblk ? x.nblk - 1 : 0u;
     const uint32_t last_len = x.nblk ? tile_block_len(g, x, last_j) : 0u;
-    check_width_bytes((const uint8_t*) stage, os, s_wo, tl.G, x.nblk, 0, 1, ws);
     const uint8_t* base = (const uint8_t*) stage + os.lds[0];
     const bool bad0 = __builtin_amdgcn_readfirstlane(os.bad[0]);
+    // the units' table entries read together; every unit decoded without a
+    // branch on its width (a zero width adds nothing, a lane without values
+    // reads block 0 and masks); the width byte of every block in the stream
+    // must equal its index width
+    uint32_t e[kUnitsPerThread];
+#pragma unroll
+    for (uint32_t i = 0; i < kUnitsPerThread; i++)
+        e[i] = s_wo[unit_j(un.pk[i])];
+    bool badw = false;
 #pragma unroll
     for (uint32_t i = 0; i < kUnitsPerThread; i++) {
         uint32_t d[4] = {0, 0, 0, 0};
         const uint32_t pk = un.pk[i], cnt = unit_cnt(pk);
         const uint32_t j = unit_j(pk);
-        if (cnt && !bad0) {
-            const uint32_t e = s_wo[j];
+        if (!bad0) {
             const uint32_t nj = j == last_j ? last_len : g.L;
-            if (e >> 16)
-                accumulate_unit(base + (e & 0xFFFFu), e >> 16, nj, (nj + 7) >> 3, unit_u(pk), d);
+            const uint8_t* bp = base + (e[i] & 0xFFFFu);
+            const uint32_t wb = bp[0];
+            accumulate_unit(bp, e[i] >> 16, nj, (nj + 7) >> 3, unit_u(pk), d);
+            badw |= cnt && wb != (e[i] >> 16);
         }
         uint32_t run = 0;
 #pragma unroll
@@ -1291,6 +1300,8 @@ __device__ __forceinline__ void decompress_tile(const Geom& g, const Tiling& tl,
         }
         us[i] = run;
     }
+    if (badw)
+        raise_error(ws, ZK_ERR_MISMATCH);
     // scan of unit sums in unit order v = i*threads + tid
     const int lane = tid & 63, wid = tid >> 6, nw = blockDim.x >> 6;
     uint32_t winc[kUnitsPerThread];

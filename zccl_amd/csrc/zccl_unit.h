@@ -36,33 +36,39 @@ __device__ __forceinline__ uint4 lds_ld16(const uint8_t* p)
         __builtin_amdgcn_alignbyte(d3, d2, a), __builtin_amdgcn_alignbyte(d4, d3, a));
 }
 
-// Add the up-to-4 deltas of unit u of a block (width byte at bp, width w > 0,
+// Add the up-to-4 deltas of unit u of a block (width byte at bp, width w,
 // nj elements, sb = ceil(nj/8) sign bytes) to acc[].  Magnitude = (high
 // bytes << r) | low r bits; the sign bit s (MSB-first bitmap) is applied as
-// (m ^ -s) + s.  Reads reach at most 20 bytes past the block's last byte.
-// ZCCL/src/ZCCL_TypeManager.c:423-517 (extract) and :906-950 (sign bitmap).
+// (m ^ -s) + s.  The sign byte, the unit's first 4 high bytes and the dword
+// holding its low bits are read together (one LDS round trip below width 16);
+// a zero width adds nothing.  Reads reach at most 20 bytes past the block's
+// last byte.  ZCCL/src/ZCCL_TypeManager.c:423-517 (extract) and :906-950
+// (sign bitmap).
 __device__ __forceinline__ void accumulate_unit(const uint8_t* bp, uint32_t w, uint32_t nj, uint32_t sb,
     uint32_t u, uint32_t acc[4])
 {
     const uint32_t B = w >> 3, r = w & 7;
-    const uint32_t nib = __builtin_amdgcn_ubfe((uint32_t) bp[1 + (u >> 1)], (~u & 1u) << 2, 4);
     const uint8_t* hp = bp + 1 + sb;
+    const uint8_t* hu = hp + 4 * u * B;
+    const uint32_t bitpos = 4 * u * r;  // a multiple of 4
+    const uint32_t sbyte = bp[1 + (u >> 1)];
+    const uint32_t H0 = lds_ld4(hu);
+    const uint32_t Xr = lds_ld4(hp + B * nj + (bitpos >> 3));
+    const uint32_t nib = w ? __builtin_amdgcn_ubfe(sbyte, (~u & 1u) << 2, 4) : 0u;
     uint32_t m[4];
     if (B <= 1) {
-        // widths below 16, the common case: one 4-byte read; B == 0 reads
-        // the low-bit bytes and masks them to nothing
-        const uint32_t H = lds_ld4(hp + 4 * u * B);
+        // widths below 16, the common case; B == 0 masks the bytes to nothing
 #pragma unroll
         for (int c = 0; c < 4; c++)
-            m[c] = __builtin_amdgcn_ubfe(H, 8 * c, 8 * B);
+            m[c] = __builtin_amdgcn_ubfe(H0, 8 * c, 8 * B);
     } else if (B == 2) {
-        uint2 H = lds_ld8(hp + 8 * u);
+        const uint2 H = lds_ld8(hu);
         m[0] = H.x & 0xFFFFu;
         m[1] = H.x >> 16;
         m[2] = H.y & 0xFFFFu;
         m[3] = H.y >> 16;
     } else {
-        uint4 H = lds_ld16(hp + 4 * u * B);
+        const uint4 H = lds_ld16(hu);
         if (B == 3) {
             m[0] = H.x & 0xFFFFFFu;
             m[1] = __builtin_amdgcn_alignbyte(H.y, H.x, 3) & 0xFFFFFFu;
@@ -72,12 +78,14 @@ __device__ __forceinline__ void accumulate_unit(const uint8_t* bp, uint32_t w, u
             m[0] = H.x; m[1] = H.y; m[2] = H.z; m[3] = H.w;
         }
     }
-    // low bits for every width (r == 0 extracts zero-width fields)
-    const uint32_t bitpos = 4 * u * r;
-    const uint32_t X = __builtin_bswap32(lds_ld4(hp + B * nj + (bitpos >> 3))) << (bitpos & 7);
+    // low bits for every width (r == 0 extracts zero-width fields): value c's
+    // field starts at bit 32 - (bitpos & 7) - r(c+1) of the big-endian dword
+    const uint32_t X = __builtin_bswap32(Xr);
+    uint32_t pos = 32u - (bitpos & 4u);
 #pragma unroll
     for (int c = 0; c < 4; c++) {
-        const uint32_t mc = (m[c] << r) | __builtin_amdgcn_ubfe(X, 32 - r * (c + 1), r);
+        pos -= r;
+        const uint32_t mc = (m[c] << r) | __builtin_amdgcn_ubfe(X, pos, r);
         const uint32_t s = (uint32_t) __builtin_amdgcn_sbfe((int32_t) nib, 3 - c, 1);  // 0 or ~0
         acc[c] += (mc ^ s) - s;
     }
