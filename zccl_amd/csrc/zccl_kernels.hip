@@ -1020,13 +1020,13 @@ __device__ __forceinline__ void accumulate_slot(const uint8_t* bp, uint32_t w, u
         for (int c = 0; c < 4; c++)
             hs[c] = __builtin_amdgcn_ubfe(H0, 8 * c, 8 * B);
     } else if (B == 2) {
+        // widths of 16 and more: rare, a second read
         const uint2 H = lds_ld8(hu);
         hs[0] = H.x & 0xFFFFu;
         hs[1] = H.x >> 16;
         hs[2] = H.y & 0xFFFFu;
         hs[3] = H.y >> 16;
     } else {
-        // widths of 24 and more: rare, a second read
         const uint4 W = lds_ld16(hu);
         if (B == 3) {
             hs[0] = W.x & 0xFFFFFFu;
