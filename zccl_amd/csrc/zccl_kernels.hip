@@ -1114,7 +1114,8 @@ __device__ __forceinline__ void homsum_tile_slots(const Geom& g, const Tiling& t
     const bool on = lane < M::ACTIVE;
     const uint32_t jj = on ? lane / M::U : 0u, u = on ? lane - jj * M::U : 0u;
     const uint32_t u4 = 4 * u, uh = u >> 1, nsh = (u & 1) ? 0u : 4u;
-    const uint32_t last_len = x.nblk ? tile_block_len(g, x, x.nblk - 1) : 0u;
+    // (a block holds at most L values; the bound lets the compiler use 24-bit products)
+    const uint32_t last_len = x.nblk ? min(tile_block_len(g, x, x.nblk - 1), L) : 0u;
     // slot i's block is jb0 + i * BPS; nj[i] its values (0: no block), and
     // the lane's value count min(4, nj - 4u) is derived where needed
     const uint32_t jb0 = wv * 4 * M::BPS + jj;

@@ -15,14 +15,14 @@ namespace zk {
 __device__ __forceinline__ uint32_t lds_ld4(const uint8_t* p)
 {
     const uint32_t a = (uint32_t) (uintptr_t) p & 3u;
-    const uint32_t* q = (const uint32_t*) (p - a);
+    const uint32_t* q = (const uint32_t*) __builtin_align_down(p, 4);
     return __builtin_amdgcn_alignbyte(q[1], q[0], a);
 }
 
 __device__ __forceinline__ uint2 lds_ld8(const uint8_t* p)
 {
     const uint32_t a = (uint32_t) (uintptr_t) p & 3u;
-    const uint32_t* q = (const uint32_t*) (p - a);
+    const uint32_t* q = (const uint32_t*) __builtin_align_down(p, 4);
     const uint32_t d0 = q[0], d1 = q[1], d2 = q[2];
     return make_uint2(__builtin_amdgcn_alignbyte(d1, d0, a), __builtin_amdgcn_alignbyte(d2, d1, a));
 }
@@ -30,7 +30,7 @@ __device__ __forceinline__ uint2 lds_ld8(const uint8_t* p)
 __device__ __forceinline__ uint4 lds_ld16(const uint8_t* p)
 {
     const uint32_t a = (uint32_t) (uintptr_t) p & 3u;
-    const uint32_t* q = (const uint32_t*) (p - a);
+    const uint32_t* q = (const uint32_t*) __builtin_align_down(p, 4);
     const uint32_t d0 = q[0], d1 = q[1], d2 = q[2], d3 = q[3], d4 = q[4];
     return make_uint4(__builtin_amdgcn_alignbyte(d1, d0, a), __builtin_amdgcn_alignbyte(d2, d1, a),
         __builtin_amdgcn_alignbyte(d3, d2, a), __builtin_amdgcn_alignbyte(d4, d3, a));
