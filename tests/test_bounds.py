@@ -31,3 +31,16 @@ def test_slot_layout_block_encoder_matches_oracle(tmp_path):
     out = subprocess.run([exe], capture_output=True, text=True, timeout=600)
     assert out.returncode == 0, out.stdout + out.stderr
     assert out.stdout.startswith("ok ")
+
+
+def test_tiled_3d_work_split_covers_every_block_once(tmp_path):
+    """The tiled 3-D kernels' chunks (compress) and block columns (decoder),
+    replayed on the CPU through the same zccl_sz3d.h functions, cover every
+    block exactly once at its stream index, with tile and row indices inside
+    their LDS arrays (tests/sz3d_tiles_check.cpp)."""
+    exe = str(tmp_path / "sz3d_tiles_check")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "zccl_amd", "csrc"),
+                    os.path.join(ROOT, "tests", "sz3d_tiles_check.cpp"), "-o", exe], check=True)
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=600)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert out.stdout.startswith("ok ")

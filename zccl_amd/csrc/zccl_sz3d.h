@@ -181,6 +181,49 @@ ZK_HD uint32_t part_of_block(const Part3* parts, uint32_t T, uint32_t gb)
     return lo;
 }
 
+// Sub-volume of a tiled work item (first chunk u0, or first block column c0
+// with cols set), by binary search over the table.
+ZK_HD uint32_t part_of_unit(const Part3* parts, uint32_t T, uint32_t v, bool cols)
+{
+    uint32_t lo = 0, hi = T;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if ((cols ? parts[mid].c0 : parts[mid].u0) <= v)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// A chunk of the tiled kernels: block column (bx, by) of sub-volume p, cut to
+// tile_nz3(Bs) blocks along z; rel is the chunk's index inside p (column
+// major: rel = (bx * b2 + by) * chunks per column + ch).
+struct Chunk3 {
+    uint32_t bx, by, ch;   // block column and chunk along z
+    uint32_t ex, ey;       // extents of the column's blocks in x and y
+    uint32_t lz0, wz, nbz; // chunk's first local z, z values, blocks
+    uint32_t gcol;         // global index of the chunk's first block
+};
+
+ZK_HD Chunk3 chunk_of3(const Part3& p, uint32_t rel, uint32_t Bs)
+{
+    const uint32_t nz = tile_nz3(Bs), wzmax = nz * Bs;
+    Chunk3 c;
+    const uint32_t nch = (p.b3 + nz - 1) / nz;
+    const uint32_t col = rel / nch;
+    c.ch = rel - col * nch;
+    c.bx = col / p.b2;
+    c.by = col - c.bx * p.b2;
+    c.ex = p.sx - c.bx * Bs < Bs ? p.sx - c.bx * Bs : Bs;
+    c.ey = p.sy - c.by * Bs < Bs ? p.sy - c.by * Bs : Bs;
+    c.lz0 = c.ch * wzmax;
+    c.wz = p.sz - c.lz0 < wzmax ? p.sz - c.lz0 : wzmax;
+    c.nbz = (c.wz + Bs - 1) / Bs;
+    c.gcol = p.g0 + (c.bx * p.b2 + c.by) * p.b3 + c.ch * nz;
+    return c;
+}
+
 // Device workspace of one 3-D call, in order: 256 bytes of status words
 // (status[0] stream size, status[1] error bits), the per-block size words,
 // their exclusive scan (u64), the scan's per-group sums (+ the total) and the

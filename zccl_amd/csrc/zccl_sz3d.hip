@@ -560,19 +560,6 @@ struct Tile3 {
     static constexpr uint32_t PAY = ((BS * BS * BS + 7) / 8 + 4 * BS * BS * BS + 15) / 16 * 16 + 32;
 };
 
-__device__ __forceinline__ uint32_t part_of_unit(const Part3* parts, uint32_t T, uint32_t v, bool cols)
-{
-    uint32_t lo = 0, hi = T;
-    while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) / 2;
-        if ((cols ? parts[mid].c0 : parts[mid].u0) <= v)
-            lo = mid;
-        else
-            hi = mid;
-    }
-    return lo;
-}
-
 // Quantized values of a chunk and its halo into qt[(i * S1 + j) * SZ + k]:
 // tile (i, j, k) holds q at local (bx Bs + i - 1, by Bs + j - 1, lz0 + k - 1).
 // A wave takes RG rows at a time and issues all their loads before it
@@ -664,30 +651,10 @@ __device__ __forceinline__ uint32_t unit_resid(const int32_t* qt, uint32_t ex, u
     return n;
 }
 
-struct Chunk3 {
-    uint32_t bx, by, ch;   // block column and chunk along z
-    uint32_t ex, ey;       // extents of the column's blocks in x and y
-    uint32_t lz0, wz, nbz; // chunk's first local z, z values, blocks
-    uint32_t gcol;         // global index of the chunk's first block
-};
-
 template <uint32_t BS>
 __device__ __forceinline__ Chunk3 chunk_of(const Part3& p, uint32_t rel)
 {
-    using C = Tile3<BS>;
-    Chunk3 c;
-    const uint32_t nch = (p.b3 + C::NZ - 1) / C::NZ;
-    const uint32_t col = rel / nch;
-    c.ch = rel - col * nch;
-    c.bx = col / p.b2;
-    c.by = col - c.bx * p.b2;
-    c.ex = min(BS, p.sx - c.bx * BS);
-    c.ey = min(BS, p.sy - c.by * BS);
-    c.lz0 = c.ch * C::WZ;
-    c.wz = min(C::WZ, p.sz - c.lz0);
-    c.nbz = (c.wz + BS - 1) / BS;
-    c.gcol = p.g0 + (c.bx * p.b2 + c.by) * p.b3 + c.ch * C::NZ;
-    return c;
+    return chunk_of3(p, rel, BS);
 }
 
 // Width and payload size of every block of a chunk.
