@@ -969,26 +969,27 @@ __device__ __forceinline__ void homsum_tile(const Geom& g, const Tiling& tl, con
 // Slot layout: a wave slot is BPS whole blocks with one 4-value unit per
 // lane (lane = U * block-in-slot + unit), so no block straddles two waves
 // and the two units that share a sign byte or a byte of low bits sit in
-// neighbouring lanes.  Each of the 4 waves owns 4 consecutive slots, so a
-// tile holds G = 16 * BPS blocks; with IG = 16 this is the tile that
-// make_tiling(g, 256) cuts (TG = 64 / U groups), so the layout in HBM and the
-// lookback chains are those of the generic sum.
-template <uint32_t L>
+// neighbouring lanes.  Each of the W waves owns 4 consecutive slots, so a
+// tile holds G = 4 * W * BPS blocks; with IG = 16 this is the tile that
+// make_tiling(g, 64 W) cuts (TG = W * 16 / U groups), so the layout in HBM
+// and the lookback chains are those of the generic kernels.  The sum uses
+// 4 waves, compress 8.
+template <uint32_t L, uint32_t W = 4>
 struct SlotMap {
     static constexpr uint32_t U = (L + 3) / 4;
     static constexpr uint32_t BPS = 64 / U;
-    static constexpr uint32_t G = 16 * BPS;
+    static constexpr uint32_t G = 4 * W * BPS;
     static constexpr uint32_t SB = (L + 7) / 8;
     static constexpr uint32_t ACTIVE = BPS * U;
     static_assert(U >= 4 && U <= 36 && G <= kMaxTileBlocks, "slot layout needs 13 <= L <= 144");
 };
 
-template <uint32_t L>
+template <uint32_t L, uint32_t W = 4>
 struct SlotShared {
     unsigned long long bc[4];
-    uint32_t bmax[SlotMap<L>::G];
-    uint32_t boff[SlotMap<L>::G];
-    uint8_t width[SlotMap<L>::G];
+    uint32_t bmax[SlotMap<L, W>::G];
+    uint32_t boff[SlotMap<L, W>::G];
+    uint8_t width[SlotMap<L, W>::G];
 };
 
 // Decoder of the slot-layout sum: add the up-to-4 values of unit u of one
@@ -1092,6 +1093,59 @@ __device__ __forceinline__ void encode_unit_pair(uint8_t* buf, uint32_t bo, uint
     if (u == 0 && cnt)
         buf[bo] = (uint8_t) w;
     emit_unit(buf, bo + 1, w, nj, u, U, cnt, f, pnib, plv);
+}
+
+// Output side of the slot-layout kernels, once sh.bmax holds every block's
+// largest magnitude and d[] the lane's four units (two's complement): wave 0
+// computes widths (dwidth), block sizes and offsets with one wave scan and
+// publishes the tile's aggregate; every lane encodes its units into buf (one
+// writer per byte); finish_tile resolves the tile's offset and stores it.
+template <uint32_t L, uint32_t W = 4>
+__device__ __forceinline__ void slot_output(const Geom& g, const Tiling& tl, const Tile& x, uint64_t t,
+    SlotShared<L, W>& sh, uint32_t* buf, uint32_t jb0, const uint32_t (&nj)[4], uint32_t u, uint32_t last_len,
+    const uint32_t (&d)[4][4], uint32_t seed, uint8_t* out, uint64_t out_cap, uint8_t* index,
+    unsigned long long* size_out, unsigned long long* status, WsHeader* ws, uint32_t nchains)
+{
+    using M = SlotMap<L, W>;
+    constexpr uint32_t G = M::G;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t u4 = 4 * u;
+    __syncthreads();
+    if (tid < 64) {
+        // wave 0: widths, sizes and offsets of the tile's blocks with one wave scan
+        constexpr uint32_t CPT = (G + 63) / 64;
+        uint32_t loc[CPT];
+        uint32_t run = 0;
+#pragma unroll
+        for (uint32_t c = 0; c < CPT; c++) {
+            const uint32_t j = tid * CPT + c;
+            loc[c] = run;
+            if (j < x.nblk) {
+                const uint32_t w = dwidth(sh.bmax[j]);
+                sh.width[j] = (uint8_t) w;
+                run += block_bytes(w, j == x.nblk - 1 ? last_len : L);
+            }
+        }
+        const uint32_t incl = wave_incl_scan(run);
+#pragma unroll
+        for (uint32_t c = 0; c < CPT; c++) {
+            const uint32_t j = tid * CPT + c;
+            if (j < x.nblk)
+                sh.boff[j] = incl - run + loc[c];
+        }
+        if (tid == 63) {
+            sh.bc[0] = incl;
+            lookback_publish(status, t, 0, incl, 0);  // successors see the aggregate now
+        }
+    }
+    __syncthreads();
+    const uint32_t total = (uint32_t) sh.bc[0];
+#pragma unroll
+    for (uint32_t i = 0; i < 4; i++) {
+        const uint32_t j = jb0 + i * M::BPS, cnt = u4 < nj[i] ? min(4u, nj[i] - u4) : 0u;
+        encode_unit_pair<M::U>((uint8_t*) buf, sh.boff[j], cnt ? sh.width[j] : 0u, nj[i], u, cnt, d[i]);
+    }
+    finish_tile(g, x, t, sh, buf, total, status, ws, seed, out, out_cap, index, size_out, t == tl.ntiles - 1, nchains);
 }
 
 template <uint32_t L>
@@ -1199,43 +1253,82 @@ __device__ __forceinline__ void homsum_tile_slots(const Geom& g, const Tiling& t
         if (umax)
             atomicMax(&sh.bmax[jb0 + i * M::BPS], umax);
     }
-    __syncthreads();
-    if (tid < 64) {
-        // wave 0: widths, sizes and offsets of the tile's blocks with one wave scan
-        constexpr uint32_t CPT = (G + 63) / 64;
-        uint32_t loc[CPT];
-        uint32_t run = 0;
-#pragma unroll
-        for (uint32_t c = 0; c < CPT; c++) {
-            const uint32_t j = tid * CPT + c;
-            loc[c] = run;
-            if (j < x.nblk) {
-                const uint32_t w = dwidth(sh.bmax[j]);
-                sh.width[j] = (uint8_t) w;
-                run += block_bytes(w, j == x.nblk - 1 ? last_len : L);
-            }
-        }
-        const uint32_t incl = wave_incl_scan(run);
-#pragma unroll
-        for (uint32_t c = 0; c < CPT; c++) {
-            const uint32_t j = tid * CPT + c;
-            if (j < x.nblk)
-                sh.boff[j] = incl - run + loc[c];
-        }
-        if (tid == 63) {
-            sh.bc[0] = incl;
-            lookback_publish(status, t, 0, incl, 0);  // successors see the aggregate now
-        }
-    }
-    __syncthreads();
-    const uint32_t total = (uint32_t) sh.bc[0];
+    slot_output<L>(g, tl, x, t, sh, stage, jb0, nj, u, last_len, acc, seed, io.out, io.out_cap, io.index, io.size_out,
+        status, ws, nchains);
+}
+
+// Compress with the slot layout (36-value blocks, 8 waves): the tile's
+// values and the one before them are staged in LDS (direct-to-LDS loads);
+// each lane quantizes its four units (FP64 product, truncation,
+// ZCCL/src/ZCCL_float.c:563,577-604) and takes their deltas, the chain
+// continuing across blocks; block maxima by LDS atomics; then the output side
+// shared with the sum (one writer per encoded byte, no buffer clearing, no
+// LDS atomics in the encoder).
+template <uint32_t L, uint32_t W>
+__device__ __forceinline__ void compress_tile_slots(const Geom& g, const Tiling& tl, const OpParams& p,
+    const CompressIO& io, uint64_t t, unsigned long long* status, WsHeader* ws, uint32_t nchains)
+{
+    using M = SlotMap<L, W>;
+    constexpr uint32_t G = M::G;
+    __shared__ SlotShared<L, W> sh;
+    uint32_t* buf = zk_lds;
+    const Tile x = tile_of(g, tl, t);
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const bool on = lane < M::ACTIVE;
+    const uint32_t jj = on ? lane / M::U : 0u, u = on ? lane - jj * M::U : 0u;
+    const uint32_t u4 = 4 * u;
+    // (a block holds at most L values; the bound lets the compiler use 24-bit products)
+    const uint32_t last_len = x.nblk ? min(tile_block_len(g, x, x.nblk - 1), L) : 0u;
+    const uint32_t jb0 = wv * 4 * M::BPS + jj;
+    uint32_t nj[4];
 #pragma unroll
     for (uint32_t i = 0; i < 4; i++) {
-        const uint32_t j = jb0 + i * M::BPS, cnt = u4 < nj[i] ? min(4u, nj[i] - u4) : 0u;
-        encode_unit_pair<M::U>(stage8, sh.boff[j], cnt ? sh.width[j] : 0u, nj[i], u, cnt, acc[i]);
+        const uint32_t j = jb0 + i * M::BPS;
+        nj[i] = on && j < x.nblk ? (j == x.nblk - 1 ? last_len : L) : 0u;
     }
-    finish_tile(g, x, t, sh, stage, total, status, ws, seed, io.out, io.out_cap, io.index, io.size_out,
-        t == tl.ntiles - 1, nchains);
+    for (uint32_t j = tid; j < G; j += blockDim.x)
+        sh.bmax[j] = 0;
+    const uint32_t tile_end = x.nblk ? x.e0 + (x.nblk - 1) * g.L + last_len : x.e0;
+    const uint32_t shift = fetch_range((const uint8_t*) (io.x + x.e0 - 1), 4u * (tile_end - x.e0 + 1),
+        (const uint8_t*) io.x, (const uint8_t*) (io.x + g.n), (uint8_t*) buf);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    // value e of the tile at xs[e - (e0 - 1)]; every lane reads its units'
+    // positions whether or not they hold values (all inside the staged
+    // range's LDS allocation) and masks the rest
+    const float* xs = (const float*) ((const uint8_t*) buf + shift);
+    uint32_t d[4][4];
+#pragma unroll
+    for (uint32_t i = 0; i < 4; i++) {
+        const uint32_t cnt = u4 < nj[i] ? min(4u, nj[i] - u4) : 0u;
+        const float* xf = xs + 1 + (jb0 + i * M::BPS) * L + u4;
+        const float pv = xf[-1];
+        const float v[4] = {xf[0], xf[1], xf[2], xf[3]};
+        const uint32_t prev = cnt ? (uint32_t) quant(pv, p.inv) : 0u;
+#pragma unroll
+        for (uint32_t c = 0; c < 4; c++)
+            d[i][c] = c < cnt ? (uint32_t) quant(v[c], p.inv) : 0u;
+#pragma unroll
+        for (int c = 3; c >= 1; c--)
+            d[i][c] -= d[i][c - 1];
+        d[i][0] -= prev;
+        uint32_t umax = 0;
+#pragma unroll
+        for (uint32_t c = 0; c < 4; c++) {
+            if (c >= cnt)
+                d[i][c] = 0;
+            umax = max(umax, umag((int32_t) d[i][c]));
+        }
+        if (umax)
+            atomicMax(&sh.bmax[jb0 + i * M::BPS], umax);
+    }
+    uint32_t seed = 0;
+    if (x.lt == 0 && tid == 0)
+        seed = (uint32_t) quant(io.x[x.seg_a], p.inv);
+    // (slot_output's first barrier also ends every read of the staged values,
+    // so the encode may reuse the buffer)
+    slot_output<L, W>(g, tl, x, t, sh, buf, jb0, nj, u, last_len, d, seed, io.out, io.out_cap, io.index,
+        io.size_out, status, ws, nchains);
 }
 
 // ------------------------------------------------------------ decompress
@@ -1393,6 +1486,15 @@ __global__ void __launch_bounds__(kHomsumThreads, 6) k_homsum_slots(HomsumArgs a
     homsum_tile_slots<kSlotL>(a.job.g, a.job.tl, a.job.p, a.job.io, t, status, a.ws, 1);
 }
 
+__global__ void __launch_bounds__(kCompressThreads) k_compress_slots(CompressArgs a)
+{
+    uint64_t t;
+    unsigned long long* status;
+    if (!launch_status(a.ws, a.job.tl.ntiles, t, status))
+        return;
+    compress_tile_slots<kSlotL, kCompressThreads / 64>(a.job.g, a.job.tl, a.job.p, a.job.io, t, status, a.ws, 1);
+}
+
 __global__ void __launch_bounds__(kDecompressThreads) k_decompress(DecompressArgs a)
 {
     uint64_t t;
@@ -1483,6 +1585,22 @@ __global__ void __launch_bounds__(kCompressThreads) k_compress_batch(BatchArgs b
     job_of(b, tg, jidx, t);
     stage_desc(b.jobs, jidx, &sio);
     compress_tile(b.g, b.tl, b.p, sio, t, status + b.job_tile0[jidx] * kStatusStride, b.ws, b.total_chains);
+}
+
+__global__ void __launch_bounds__(kCompressThreads) k_compress_batch_slots(BatchArgs b)
+{
+    __shared__ CompressIO sio;
+    unsigned long long* status;
+    uint64_t tg;
+    if (!launch_status(b.ws, b.total_tiles, tg, status))
+        return;
+    uint32_t jidx;
+    uint64_t t;
+    job_of(b, tg, jidx, t);
+    stage_desc(b.jobs, jidx, &sio);
+    // a uniform batch's jobs have equal tile counts: job j's tiles start at j * ntiles
+    compress_tile_slots<kSlotL, kCompressThreads / 64>(b.g, b.tl, b.p, sio, t,
+        status + (uint64_t) jidx * b.tl.ntiles * kStatusStride, b.ws, b.total_chains);
 }
 
 __global__ void __launch_bounds__(kDecompressThreads) k_decompress_batch(BatchArgs b)
@@ -1707,10 +1825,22 @@ size_t operand_lds_bytes(const Tiling& tl, int k, uint32_t stage)
     return (size_t) (stage + 4ull * k * tl.G);
 }
 
+// The slot-layout compress: the same block length and index groups at the
+// compress workgroup size (8 waves).
+static bool slot_layout_compress(const Geom& g, const Tiling& tl)
+{
+    return g.L == kSlotL && g.IG == 16 && tl.threads == kCompressThreads &&
+        tl.G == SlotMap<kSlotL, kCompressThreads / 64>::G;
+}
+
 hipError_t launch_compress(const CompressArgs& a, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_compress, dim3((uint32_t) a.job.tl.ntiles), dim3(a.job.tl.threads),
-        compress_lds_bytes(a.job.tl), s, a);
+    if (slot_layout_compress(a.job.g, a.job.tl))
+        hipLaunchKernelGGL(k_compress_slots, dim3((uint32_t) a.job.tl.ntiles), dim3(a.job.tl.threads),
+            compress_lds_bytes(a.job.tl), s, a);
+    else
+        hipLaunchKernelGGL(k_compress, dim3((uint32_t) a.job.tl.ntiles), dim3(a.job.tl.threads),
+            compress_lds_bytes(a.job.tl), s, a);
     return hipGetLastError();
 }
 
@@ -1743,7 +1873,9 @@ hipError_t launch_batch(BatchKind kind, const BatchArgs& b, uint32_t threads, si
     dim3 grid((uint32_t) b.total_tiles), block(threads);
     switch (kind) {
     case BatchKind::Compress:
-        if (b.uniform)
+        if (b.uniform && slot_layout_compress(b.g, b.tl) && threads == kCompressThreads)
+            hipLaunchKernelGGL(k_compress_batch_slots, grid, block, lds, s, b);
+        else if (b.uniform)
             hipLaunchKernelGGL(k_compress_batch, grid, block, lds, s, b);
         else
             hipLaunchKernelGGL(k_compress_mixed, grid, block, lds, s, b);
