@@ -78,14 +78,16 @@ __device__ __forceinline__ void accumulate_unit(const uint8_t* bp, uint32_t w, u
             m[0] = H.x; m[1] = H.y; m[2] = H.z; m[3] = H.w;
         }
     }
-    // low bits for every width (r == 0 extracts zero-width fields): value c's
-    // field starts at bit 32 - (bitpos & 7) - r(c+1) of the big-endian dword
-    const uint32_t X = __builtin_bswap32(Xr);
-    uint32_t pos = 32u - (bitpos & 4u);
+    // low bits for every width: the unit's fields start at bit (bitpos & 7)
+    // of the big-endian dword; shifted to the top of lo, one 64-bit shift of
+    // {m[c], lo} by r gives value c's magnitude (high dword) and the next
+    // window (low dword), r == 0 included
+    uint32_t lo = __builtin_bswap32(Xr) << (bitpos & 4u);
 #pragma unroll
     for (int c = 0; c < 4; c++) {
-        pos -= r;
-        const uint32_t mc = (m[c] << r) | __builtin_amdgcn_ubfe(X, pos, r);
+        const uint64_t v = ((uint64_t) m[c] << 32 | lo) << r;
+        const uint32_t mc = (uint32_t) (v >> 32);
+        lo = (uint32_t) v;
         const uint32_t s = (uint32_t) __builtin_amdgcn_sbfe((int32_t) nib, 3 - c, 1);  // 0 or ~0
         acc[c] += (mc ^ s) - s;
     }
