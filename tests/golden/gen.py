@@ -100,6 +100,24 @@ def _outliers(n, rng):
     return x.astype(np.float32)
 
 
+def _ladder(n, rng):
+    """Width ladder for eb = 1: 36-value stretches s that alternate between 0
+    and +-m with m in [2^(j-1), 2^j), j = s mod 25 (j = 0: a flat stretch),
+    so deltas of magnitude m give every block width 0..24 (integers below
+    2^24, exact in FP32; widths 25..32 are the "wide" kind's)."""
+    x = np.zeros(n, dtype=np.float64)
+    for s, i in enumerate(range(0, n, 36)):
+        j = s % 25
+        k = min(36, n - i)
+        if j == 0:
+            continue
+        lo = 2 ** (j - 1)
+        m = rng.integers(lo, 2 * lo, k).astype(np.float64)
+        sg = np.where(rng.random(k) < 0.5, -1.0, 1.0)
+        x[i:i + k] = np.where(np.arange(k) & 1, sg * m, 0.0)
+    return x.astype(np.float32)
+
+
 _GEN = {
     "walk": _walk,
     "sines": _sines,
@@ -111,6 +129,7 @@ _GEN = {
     "special": _special,
     "denormal": _denormal,
     "outliers": _outliers,
+    "ladder": _ladder,
 }
 
 KINDS = tuple(_GEN)

@@ -93,6 +93,8 @@ CASES = [
     ("walk", 17, 1e-2, 36, 16),      # remainder of 1
     ("walk", 40, 1e-2, 36, 16),
     ("walk", 3, 1e-2, 36, 1),
+    ("ladder", 40_000, 1.0, 36, 4),  # every block width 0..24 (25..32: "wide")
+    ("ladder", 40_000, 1.0, 20, 3),
 ]
 
 
@@ -130,6 +132,24 @@ def test_fused_k_operand_sum(codec, k):
     codec.check_errors()
     ref = O.homsum([O.compress(x, eb, L, T) for x in xs], n, L, T)
     assert np.array_equal(h.to_host(), ref)
+
+
+@pytest.mark.parametrize("kind,L", [("ladder", 36), ("ladder", 20), ("wide", 36)])
+def test_fused_sum_every_width(codec, kind, L):
+    """Eight operands whose blocks take every width (0..24 for the ladder,
+    21..32 for the wide kind), so the decoder meets each split of a width
+    into high bytes and low bits, r = 0 included; bytes equal the oracle's."""
+    n, eb, T, k = 40_000, 1.0, 4, 8
+    say("homsum widths", kind, L)
+    xs = [gen.make(kind, n, 500 + r) for r in range(k)]
+    ss = [codec.compress(dev(x), eb, L, T) for x in xs]
+    h = codec.homsum(ss)
+    torch.cuda.synchronize()
+    codec.check_errors()
+    cs = [O.compress(x, eb, L, T) for x in xs]
+    w = set(O.parse(cs[0], n, L, T)[0].tolist())
+    assert w >= (set(range(25)) if kind == "ladder" else set(range(21, 33)))
+    assert np.array_equal(h.to_host(), O.homsum(cs, n, L, T))
 
 
 # ------------------------------------------------------------ foreign streams

@@ -282,6 +282,20 @@ def test_parse_offsets_follow_block_sizes():
                 assert bo[first] == so[seg] + 4
 
 
+def test_width_inputs_cover_every_width():
+    """The ladder and wide kinds, which the GPU parity tests use to reach
+    every decoder branch, together give blocks of every width 0..32, and the
+    oracle round-trips them exactly (integer inputs, eb = 1)."""
+    seen = set()
+    for kind, L, T in [("ladder", 36, 4), ("ladder", 20, 3), ("wide", 36, 4)]:
+        n = 40_000
+        x = gen.make(kind, n, 500)
+        s = O.compress(x, 1.0, L, T)
+        seen |= set(O.parse(s, n, L, T)[0].tolist())
+        assert np.array_equal(O.decode_q(s, n, L, T), O.quantize(x, 1.0))
+    assert seen == set(range(33))
+
+
 def test_tolerance_policy_and_reference_eb_violations():
     """|x - x'| <= eb + 2^-21 |x| holds everywhere |x/eb| < 2^31; the plain
     |x - x'| <= eb does not (the reference itself violates it, 1/eb in FP32
