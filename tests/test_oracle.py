@@ -125,7 +125,7 @@ def test_random_cases_against_compiled_reference():
         L = int(rng.choice([1, 7, 8, 32, 36, 128]))
         n = int(rng.integers(T, 4000))
         kind = gen.KINDS[trial % len(gen.KINDS)]
-        eb = 1.0 if kind == "wide" else (3e-39 if kind == "denormal" else
+        eb = 1.0 if kind in ("wide", "ladder") else (3e-39 if kind == "denormal" else
                                            float(rng.choice([1e-1, 1e-2, 1e-3, 1e-4])))
         x = gen.make(kind, n, 100 + trial)
         y = gen.perturb(x, 200 + trial)
