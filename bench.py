@@ -63,6 +63,8 @@ def parse_args():
     ap.add_argument("--no-3d", action="store_true", help="skip the 3-D codec measurement")
     ap.add_argument("--side3d", type=int, default=512, help="grid side of the 3-D codec's field")
     ap.add_argument("--bs3d", type=int, default=4, help="block side of the 3-D codec")
+    ap.add_argument("--layout", choices=("lane", "tiles"), default="lane",
+                    help="sum layout for 36-value blocks: one lane per block, or workgroup tiles")
     return ap.parse_args()
 
 
@@ -232,6 +234,8 @@ def main():
     from zccl_amd.allreduce import StagedAllreduce
 
     torch.cuda.set_device(0)
+    from zccl_amd._lib import lib as _zl
+    _zl().zccl_tune_layout(1, 0 if args.layout == "lane" else 1)
     if args.copy_calibration:
         # elementwise kernel reading 256 MiB and writing 256 MiB per launch
         a = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
@@ -354,10 +358,14 @@ def main():
     t_doc = timed(lambda: ar.reduce_scatter_doc(xs), 2)
     t_homo_rs = t_comp + ms_per_step / 1e3
 
-    # the uniform batch of 36-value blocks at 256 threads runs the slot-layout sum
-    from zccl_amd._lib import lib as _zl
-    homsum_kernel = ("zk::k_homsum_batch_slots" if L == 36 and _zl().zccl_tune_get_threads(1) == 256
-                     else "zk::k_homsum_batch")
+    # the uniform batch of 36-value blocks runs the lane-layout sum (or, with
+    # --layout tiles, the slot-layout sum at 256 threads)
+    if L == 36 and args.layout == "lane":
+        homsum_kernel = "zk::k_homsum_lane_batch"
+    elif L == 36 and _zl().zccl_tune_get_threads(1) == 256:
+        homsum_kernel = "zk::k_homsum_batch_slots"
+    else:
+        homsum_kernel = "zk::k_homsum_batch"
     roof = {
         "bound": "hbm",
         "achieved": round(achieved, 1),

@@ -211,6 +211,14 @@ int zccl_batch_bind(zccl_batch *batch, void *device_desc);
 int zccl_tune_threads(int op, int threads);
 int zccl_tune_get_threads(int op);
 
+/* Layout of the homomorphic sum (op 1) for 36-value blocks: 0 one lane per
+ * block, a wave per 64-block tile (the default); 1 workgroup tiles of the
+ * zccl_tune_threads size.  Affects launches and plans created afterwards
+ * and zccl_num_tiles() never shrinks below either layout's need; never the
+ * stream bytes.  No reference counterpart. */
+int zccl_tune_layout(int op, int layout);
+int zccl_tune_get_layout(int op);
+
 /* Block-case histogram of two indexed streams (see the _record add). */
 int zccl_case_count_dev(const void *index1, const void *index2, size_t nbEle, int blockSize,
     int nchunks, unsigned long long *d_counts4, void *stream);

@@ -40,7 +40,7 @@ EXPORTS = (
     "zccl_batch_tiles", "zccl_batch_launch", "zccl_batch_destroy",
     "zccl_batch_homsum_plan", "zccl_batch_compress_plan", "zccl_batch_decompress_plan",
     "zccl_batch_desc_bytes", "zccl_batch_desc_host", "zccl_batch_bind",
-    "zccl_tune_threads", "zccl_tune_get_threads",
+    "zccl_tune_threads", "zccl_tune_get_threads", "zccl_tune_layout", "zccl_tune_get_layout",
     "SZp_compress3D_fast_openmp", "SZp_decompress3D_fast_openmp",
     "zccl_sz3d_sizes", "zccl_sz3d_compress_dev", "zccl_sz3d_decompress_dev", "zccl_sz3d_status",
 )
@@ -121,6 +121,8 @@ def lib() -> ctypes.CDLL:
         "zccl_sz3d_status": ([vp, sz_p], i32),
         "zccl_tune_threads": ([i32, i32], i32),
         "zccl_tune_get_threads": ([i32], i32),
+        "zccl_tune_layout": ([i32, i32], i32),
+        "zccl_tune_get_layout": ([i32], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

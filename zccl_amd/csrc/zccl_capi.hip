@@ -124,6 +124,30 @@ extern "C" int zccl_tune_threads(int op, int threads)
 
 extern "C" int zccl_tune_get_threads(int op) { return op < 0 || op > 2 ? 0 : g_threads[op].load(); }
 
+// Layout of the homomorphic sum for 36-value blocks: 0 one lane per block
+// (zccl_lane.hip, the default), 1 workgroup tiles (slot layout).
+static std::atomic<int> g_layout[3] = {{1}, {0}, {1}};
+
+extern "C" int zccl_tune_layout(int op, int layout)
+{
+    if (op != 1 || layout < 0 || layout > 1)
+        return ZCCL_EINVAL;
+    g_layout[op].store(layout);
+    return ZCCL_OK;
+}
+
+extern "C" int zccl_tune_get_layout(int op) { return op < 0 || op > 2 ? -1 : g_layout[op].load(); }
+
+// Tiling of operation op for geometry g.
+static Tiling op_tiling(int op, const Geom& g)
+{
+    if (op == 1 && g_layout[1].load() == 0 && lane_layout_ok(g))
+        return make_lane_tiling(g);
+    return make_tiling(g, op_threads(op));
+}
+
+static bool is_lane_tiling(const Geom& g, const Tiling& tl) { return lane_layout_ok(g) && tl.threads == 64 && tl.G == kLaneTileBlocks; }
+
 // ------------------------------------------------------------ sizes
 
 static bool geom_of(size_t n, int L, int T, Geom* g)
@@ -150,9 +174,11 @@ extern "C" size_t zccl_num_tiles(size_t n, int L, int T)
     Geom g;
     if (!geom_of(n, L, T, &g))
         return 0;
-    uint64_t a = make_tiling(g, op_threads(0)).ntiles, b = make_tiling(g, op_threads(1)).ntiles,
-             c = make_tiling(g, op_threads(2)).ntiles;
-    return (size_t) std::max(a, std::max(b, c));
+    uint64_t a = op_tiling(0, g).ntiles, b = op_tiling(1, g).ntiles, c = op_tiling(2, g).ntiles;
+    uint64_t m = std::max(a, std::max(b, c));
+    if (lane_layout_ok(g))
+        m = std::max<uint64_t>(m, make_lane_tiling(g).ntiles);  // either layout of the sum
+    return (size_t) m;
 }
 
 extern "C" size_t zccl_num_blocks(size_t n, int L, int T)
@@ -282,7 +308,7 @@ extern "C" int zccl_homsum_dev(const unsigned char* const* ins, const void* cons
     HomsumArgs a;
     memset(&a, 0, sizeof(a));
     a.job.g = g;
-    a.job.tl = make_tiling(g, op_threads(1));
+    a.job.tl = op_tiling(1, g);
     int rc = check_ws(ws, ws_bytes, a.job.tl.ntiles);
     if (rc)
         return rc;
@@ -298,7 +324,10 @@ extern "C" int zccl_homsum_dev(const unsigned char* const* ins, const void* cons
     a.job.io.index = (uint8_t*) index;
     a.job.io.size_out = d_size;
     a.ws = (WsHeader*) ws;
-    HIP_TRY(launch_homsum(a, (hipStream_t) stream));
+    if (is_lane_tiling(g, a.job.tl))
+        HIP_TRY(launch_homsum_lane(a, (hipStream_t) stream));
+    else
+        HIP_TRY(launch_homsum(a, (hipStream_t) stream));
     return ZCCL_OK;
 }
 
@@ -320,6 +349,7 @@ struct zccl_batch {
     size_t tile0_off;                 // byte offset of the tile starts in `host`
     void* owned;                      // device copy owned by the plan (create()), else null
     bool bound;
+    bool lane;                        // homomorphic sum in the lane layout
 };
 
 namespace {
@@ -340,6 +370,7 @@ zccl_batch* batch_plan(BatchKind kind, const std::vector<Job>& jobs, uint32_t ch
     b->kind = kind;
     b->owned = nullptr;
     b->bound = false;
+    b->lane = kind == BatchKind::Homsum && nj > 0 && is_lane_tiling(jobs[0].g, jobs[0].tl);
     uint32_t threads = 64;
     size_t lds = 0;
     std::vector<unsigned long long> tile0(nj + 1, 0);
@@ -425,7 +456,7 @@ extern "C" zccl_batch* zccl_batch_homsum_plan(int njobs, const int* ks, const si
             *st = ZCCL_EINVAL;
             return nullptr;
         }
-        h.tl = make_tiling(h.g, op_threads(1));
+        h.tl = op_tiling(1, h.g);
         h.p = op_params(h.tl, 0.0, 0.0f, k);
         for (int r = 0; r < k; r++, pos++) {
             if (!ins[pos] || !indices[pos] || ins[pos] == outs[j]) {
@@ -558,7 +589,10 @@ extern "C" int zccl_batch_launch(const zccl_batch* b, void* ws, size_t ws_bytes,
         return rc;
     BatchArgs a = b->args;
     a.ws = (WsHeader*) ws;
-    HIP_TRY(launch_batch(b->kind, a, b->threads, b->lds, (hipStream_t) stream));
+    if (b->lane)
+        HIP_TRY(launch_homsum_lane_batch(a, (hipStream_t) stream));
+    else
+        HIP_TRY(launch_batch(b->kind, a, b->threads, b->lds, (hipStream_t) stream));
     return ZCCL_OK;
 }
 

@@ -154,6 +154,28 @@ ZK_HD Tiling make_tiling(const Geom& g, uint32_t target_threads)
     return t;
 }
 
+// Lane layout (zccl_lane.hip): one lane per block of the reference's default
+// length, one wave per tile of 64 blocks (4 index groups), kLaneWaves tiles
+// per workgroup.
+constexpr uint32_t kLaneL = 36;
+constexpr uint32_t kLaneTileBlocks = 64;
+constexpr uint32_t kLaneWaves = 4;
+
+ZK_HD bool lane_layout_ok(const Geom& g) { return g.L == kLaneL && g.IG == 16; }
+
+ZK_HD Tiling make_lane_tiling(const Geom& g)
+{
+    Tiling t;
+    t.TG = kLaneTileBlocks / g.IG;
+    t.G = kLaneTileBlocks;
+    t.threads = 64;
+    t.tilesT = (uint32_t) ceil_div(g.gT, t.TG);
+    t.tilesR = g.gR ? (uint32_t) ceil_div(g.gR, t.TG) : 0u;
+    t.ntiles = g.T * t.tilesT + t.tilesR;
+    t.tile_bytes_max = t.G * block_bytes(32, g.L);
+    return t;
+}
+
 struct Tile {
     uint32_t s;              // segment
     uint32_t nblk;           // blocks in this tile

@@ -158,6 +158,10 @@ hipError_t launch_homsum(const HomsumArgs& a, hipStream_t s);
 hipError_t launch_decompress(const DecompressArgs& a, hipStream_t s);
 hipError_t launch_batch(BatchKind kind, const BatchArgs& b, uint32_t threads, size_t lds, hipStream_t s);
 hipError_t launch_parse(const ParseArgs& a, hipStream_t s);
+// Lane layout (zccl_lane.hip): tilings from make_lane_tiling, 36-value blocks.
+size_t lane_lds_bytes();
+hipError_t launch_homsum_lane(const HomsumArgs& a, hipStream_t s);
+hipError_t launch_homsum_lane_batch(const BatchArgs& b, hipStream_t s);
 hipError_t launch_case_count(const uint8_t* w1, const uint8_t* w2, uint64_t nblocks,
     unsigned long long* counts, hipStream_t s);
 hipError_t launch_ws_init(WsHeader* ws, uint64_t capacity, hipStream_t s);

@@ -1,0 +1,776 @@
+// zccl_lane.hip -- block-per-lane kernels for 36-value blocks, the
+// reference's default block length (examples/compression_allreduce.c,
+// jobs/*.sh): the fused homomorphic sum.
+//
+// Layout.  A wave owns one tile of 64 consecutive blocks of one segment and
+// every lane owns one whole block.  For each operand a lane reads its block's
+// bytes straight from HBM into registers with range-checked buffer loads (a
+// read past the operand's stream returns zeros, so no read ever leaves the
+// stream), decodes its 36 deltas and adds them in int32.  No LDS staging and
+// no workgroup barrier sit on this path: the only per-operand cross-lane step
+// is one DPP scan of block sizes that places each lane's block.  The next
+// operand's loads are in flight while the current one is decoded.
+//
+// Each wave is a tile of the decoupled lookback; a workgroup of four waves
+// takes one ticket for four consecutive tiles.  The summed block is
+// re-encoded into the wave's own LDS buffer with OR-writes of dwords (the
+// buffer is zeroed first, and a lane ORs zeros wherever its dwords overlap a
+// neighbour's bytes, so overlapping writes need no ordering) and leaves with
+// 16-byte stores.
+//
+// Blocks of width 16 and more, and a segment's last (short) block, take a
+// per-unit path with the same arithmetic.
+//
+// Wire format and arithmetic (reference):
+//   block      width byte, sign bitmap MSB first, (w>>3) high bytes per
+//              value, MSB-first low (w&7) bits     ZCCL/src/ZCCL_TypeManager.c:12-91, 824-848
+//   sum        deltas add in int32, block re-encoded at the width of the
+//              largest magnitude                   ZCCL/src/ZCCLd_float.c:1099-1219
+//   width      (int)log2f((float)max)+1             ZCCL/src/ZCCLd_float.c:1205
+#include <hip/hip_runtime.h>
+
+#include "zccl_device.h"
+#include "zccl_internal.h"
+#include "zccl_kernels.h"
+
+namespace zk {
+
+// Dynamic LDS: one output buffer per wave, 16-byte aligned (the buffers are
+// zeroed with 16-byte stores).
+extern __shared__ __attribute__((aligned(16))) uint32_t zk_lds[];
+
+namespace {
+
+constexpr uint32_t L = kLaneL;          // values per block
+constexpr uint32_t SB = (L + 7) / 8;    // sign bytes of a whole block
+constexpr uint32_t HEADW = 11;          // dwords of bytes [0, 44): width, signs, 36 high bytes
+constexpr uint32_t NGRP = (L + 7) / 8;  // groups of (up to) 8 values in the low bits
+constexpr uint32_t kWorstBlock = 1 + SB + 4 * L;   // 150 bytes at width 32
+// Output buffer of one wave: a worst-case tile of 64 blocks plus the reach of
+// the OR-writes past a block's end (at most 48 bytes).
+constexpr uint32_t kOutBuf = kLaneTileBlocks * kWorstBlock + 64;
+static_assert(kOutBuf % 16 == 0, "wave buffers stay 16-byte aligned");
+
+static_assert(L % 4 == 0 && L > 32 && L <= 40, "the lane decoder is laid out for 36-value blocks");
+
+// Encoded size of a whole block at width w (w = 0..32): 1 + SB + 36 B +
+// ceil(36 r / 8) = 6 + ceil(9 w / 2).
+__device__ __forceinline__ uint32_t full_block_bytes(uint32_t w) { return w ? 6u + ((9u * w + 1u) >> 1) : 1u; }
+
+using View = __amdgpu_buffer_rsrc_t;
+
+// Range-checked view of [base, base + bytes) (base 4-byte aligned, bytes a
+// multiple of 4).  A buffer load returns zeros for every dword that does not
+// lie wholly inside the view, whatever its alignment: a dword load at an
+// unaligned offset that reaches past the view's last dword reads as zero even
+// where its first bytes are inside.
+__device__ __forceinline__ View make_view(const uint8_t* base, uint64_t bytes)
+{
+    const uint32_t n = bytes > 0xFFFFFFFCull ? 0xFFFFFFFCu : (uint32_t) bytes;
+    return __builtin_amdgcn_make_buffer_rsrc((void*) base, (short) 0, (int) n, 0x00020000);
+}
+
+__device__ __forceinline__ uint32_t bl4(View v, uint32_t off) { return __builtin_amdgcn_raw_buffer_load_b32(v, off, 0, 0); }
+
+__device__ __forceinline__ uint2 bl8(View v, uint32_t off)
+{
+    const auto x = __builtin_amdgcn_raw_buffer_load_b64(v, off, 0, 0);
+    return make_uint2(x[0], x[1]);
+}
+
+__device__ __forceinline__ uint3 bl12(View v, uint32_t off)
+{
+    const auto x = __builtin_amdgcn_raw_buffer_load_b96(v, off, 0, 0);
+    return make_uint3(x[0], x[1], x[2]);
+}
+
+__device__ __forceinline__ uint4 bl16(View v, uint32_t off)
+{
+    const auto x = __builtin_amdgcn_raw_buffer_load_b128(v, off, 0, 0);
+    return make_uint4(x[0], x[1], x[2], x[3]);
+}
+
+// 4 / 8 bytes from any offset through aligned dword loads only, so every
+// byte inside the view reads as itself (up to its very end).
+__device__ __forceinline__ uint32_t bl4a(View v, uint32_t off)
+{
+    const uint2 x = bl8(v, off & ~3u);
+    return __builtin_amdgcn_alignbyte(x.y, x.x, off);  // uses off & 3
+}
+
+__device__ __forceinline__ uint2 bl8a(View v, uint32_t off)
+{
+    const uint3 x = bl12(v, off & ~3u);
+    return make_uint2(__builtin_amdgcn_alignbyte(x.y, x.x, off), __builtin_amdgcn_alignbyte(x.z, x.y, off));
+}
+
+// Read-only data of the launch (descriptors, input indices) through the
+// scalar cache when the address is the same for the whole wave.
+template <typename T>
+__device__ __forceinline__ T ldc(const T* p)
+{
+    return *(const __attribute__((address_space(4))) T*) (uintptr_t) p;
+}
+
+// |d| of an int32 delta as unsigned (|INT_MIN| = 2^31).
+__device__ __forceinline__ uint32_t mag(uint32_t d) { return (uint32_t) max((int32_t) d, (int32_t) (0u - d)); }
+
+// One operand's block of this lane, loaded: bytes [0, 44) of the block (width
+// byte, sign bytes, the high bytes of a width below 16) and, for each group
+// of 8 values, the 8 bytes from the group's first byte of low bits as two
+// big-endian words.
+struct Block {
+    uint32_t h[HEADW];
+    uint32_t ge[2 * NGRP];
+    uint32_t w;   // index width (0..32)
+    uint32_t o;   // offset of the block in the operand's view
+    uint32_t lim; // length of the operand's view (the same for the wave)
+    bool fast;    // decode_fast serves this lane (else the general decoder, if w > 0)
+};
+
+// Add a whole block of width w < 16 to acc: magnitude = (high byte << r) |
+// low r bits, sign applied as (m ^ s) - s with s = 0 or ~0.  A zero width
+// adds zeros whatever the bytes.  ZCCL/src/ZCCL_TypeManager.c:423-517 (extract),
+// :906-950 (signs).
+__device__ __forceinline__ void decode_fast(const Block& q, uint32_t (&acc)[L])
+{
+    const uint32_t r = q.w & 7, hb = q.w & 8;  // hb: bits per high byte (0 or 8)
+    const uint32_t s0 = __builtin_amdgcn_alignbyte(q.h[1], q.h[0], 1);  // sign bytes 0..3, memory order
+    const uint32_t s1 = q.h[1] >> 8;                                   // sign byte 4 in bits 7..0
+#pragma unroll
+    for (uint32_t p = 0; p < NGRP; p++) {
+        // the group's low bits big-endian: value 8p+c's field is r bits from bit 64 - (c+1) r
+        const uint32_t e0 = q.ge[2 * p], e1 = q.ge[2 * p + 1];
+        uint32_t gh = e0;
+#pragma unroll
+        for (uint32_t c = 0; c < 8; c++) {
+            const uint32_t j = 8 * p + c;
+            if (j >= L)
+                break;
+            if (c == 4)
+                gh = (uint32_t) (((((uint64_t) e0 << 32) | e1) << (4 * r)) >> 32);
+            const uint32_t hbyte = __builtin_amdgcn_ubfe(q.h[(6 + j) >> 2], 8 * ((6 + j) & 3), hb);
+            // {high byte, window} << r: the magnitude in the upper dword, the
+            // next value's window in the lower one
+            const uint64_t v = (((uint64_t) hbyte << 32) | gh) << r;
+            const uint32_t m = (uint32_t) (v >> 32);
+            gh = (uint32_t) v;
+            const uint32_t sm = j < 32 ? (uint32_t) __builtin_amdgcn_sbfe((int32_t) s0, 8 * (j >> 3) + 7 - (j & 7), 1)
+                                       : (uint32_t) __builtin_amdgcn_sbfe((int32_t) s1, 7 - (j - 32), 1);
+            acc[j] += (m ^ sm) - sm;
+        }
+    }
+}
+
+// Add a block of any width (nj values, nj <= L) unit by unit; values past nj
+// add garbage that the caller clears.  Same fields as decode_fast; aligned
+// loads only (this path also serves blocks that end in the view's last
+// dword), each value's high bytes with a load of their own.  Branch-free and
+// one unit's loads at a time: this path is rare and must not raise the
+// kernel's register count.  Returns whether the block's width byte differs
+// from w.
+__device__ __forceinline__ bool decode_general(View v, uint32_t o, uint32_t w, uint32_t nj, uint32_t (&acc)[L])
+{
+    const uint32_t B = w >> 3, r = w & 7, nsb = (nj + 7) >> 3;
+    const uint32_t hmask = B >= 4 ? ~0u : (1u << (8 * B)) - 1u;
+    const uint32_t hp = o + 1 + nsb, lp = hp + B * nj;
+    const bool bad = (bl4a(v, o) & 0xFFu) != w;
+    const uint2 sg = bl8a(v, o + 1);
+#pragma unroll
+    for (uint32_t u = 0; u < L / 4; u++) {
+        __builtin_amdgcn_sched_barrier(0);
+        uint32_t H[4];
+#pragma unroll
+        for (uint32_t c = 0; c < 4; c++)
+            H[c] = bl4a(v, hp + (4 * u + c) * B);
+        const uint32_t X = bl4a(v, lp + ((4 * u * r) >> 3));
+        const uint32_t sw = (u >> 1) < 4 ? sg.x : sg.y;
+        const uint32_t nib = __builtin_amdgcn_ubfe(sw, 8 * ((u >> 1) & 3) + ((u & 1) ? 0 : 4), 4);
+        uint32_t lo = __builtin_bswap32(X) << ((4 * u * r) & 4u);
+#pragma unroll
+        for (uint32_t c = 0; c < 4; c++) {
+            const uint32_t hs = H[c] & hmask;
+            const uint64_t x = (((uint64_t) hs << 32) | lo) << r;
+            const uint32_t m = (uint32_t) (x >> 32);
+            lo = (uint32_t) x;
+            const uint32_t sm = (uint32_t) __builtin_amdgcn_sbfe((int32_t) nib, 3 - c, 1);
+            acc[4 * u + c] += (m ^ sm) - sm;
+        }
+    }
+    return bad;
+}
+
+// OR bytes into the wave's zeroed LDS buffer.
+__device__ __forceinline__ void lds_or(uint32_t* p, uint32_t v)
+{
+    __hip_atomic_fetch_or(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Byte-shift selector for v_perm: perm(x_i, x_{i-1}, sel) is the dword at
+// 4i of a byte array placed sh bytes into the aligned grid.
+__device__ __forceinline__ uint32_t shift_sel(uint32_t sh) { return 0x07060504u - sh * 0x01010101u; }
+
+// Encode a whole block of width 0 < w < 16 at byte o of the buffer: width
+// byte, sign bitmap MSB first, one high byte per value when w >= 8, then the
+// low r bits of every magnitude as an MSB-first stream (groups of 8 values
+// fill r whole bytes).  wide: some lane of the wave has w >= 8 (otherwise
+// the high bytes are known to be absent).
+// ZCCL/src/ZCCL_TypeManager.c:12-91 (fixed-length bits), :824-848 (signs).
+// s: signs of values 0..31 (value j at bit 31 - j), s2: signs of values
+// 32..35 (value 32 at bit 3); m: magnitudes.
+__device__ __forceinline__ void encode_fast(uint32_t* ob, uint32_t o, uint32_t w, uint32_t s, uint32_t s2,
+    const uint32_t (&m)[L], bool wide)
+{
+    const uint32_t r = w & 7, B = w >> 3;
+    const uint32_t sb = __builtin_bswap32(s);  // sign bytes 0..3 in memory order
+    // high bytes (zero when w < 8: then m < 2^r)
+    uint32_t H[L];
+#pragma unroll
+    for (uint32_t j = 0; j < L; j++)
+        H[j] = m[j] >> r;
+    uint32_t x[HEADW + 1];
+    x[0] = w | (sb << 8);
+    x[1] = (sb >> 24) | (s2 << (8 + 40 - L)) | (H[0] << 16) | (H[1] << 24);  // sign byte 4: value 32 at bit 7
+#pragma unroll
+    for (uint32_t i = 2; i < HEADW; i++) {
+        const uint32_t j = 4 * i - 6;
+        x[i] = H[j] | (j + 1 < L ? H[j + 1] << 8 : 0u) | (j + 2 < L ? H[j + 2] << 16 : 0u) |
+               (j + 3 < L ? H[j + 3] << 24 : 0u);
+    }
+    x[HEADW] = 0;
+    const uint32_t sel = shift_sel(o & 3);
+    uint32_t* p = ob + (o >> 2);
+    uint32_t prev = 0;
+    const uint32_t nd = wide ? HEADW + 1 : 3;
+#pragma unroll
+    for (uint32_t i = 0; i < HEADW + 1; i++) {
+        if (i < nd)
+            lds_or(p + i, __builtin_amdgcn_perm(x[i], prev, sel));
+        prev = x[i];
+    }
+    if (r) {
+        const uint32_t mask = (1u << r) - 1, lo0 = o + 1 + SB + B * L;
+#pragma unroll
+        for (uint32_t g = 0; g < NGRP; g++) {
+            uint32_t ua = 0, ub = 0;
+#pragma unroll
+            for (uint32_t c = 0; c < 4; c++) {
+                const uint32_t j = 8 * g + c;
+                ua = (ua << r) | (m[j] & mask);
+                if (j + 4 < L)
+                    ub = (ub << r) | (m[j + 4] & mask);
+            }
+            // 8 fields (the last group: 4 and zeros) = r bytes, top-aligned in Y
+            const uint64_t G = ((uint64_t) ua << (4 * r)) | ub;
+            const uint64_t Y = G << (64 - 8 * r);
+            const uint32_t z0 = __builtin_bswap32((uint32_t) (Y >> 32)), z1 = __builtin_bswap32((uint32_t) Y);
+            const uint32_t a = lo0 + g * r;
+            const uint32_t gs = shift_sel(a & 3);
+            uint32_t* q = ob + (a >> 2);
+            lds_or(q, __builtin_amdgcn_perm(z0, 0u, gs));
+            lds_or(q + 1, __builtin_amdgcn_perm(z1, z0, gs));
+            lds_or(q + 2, __builtin_amdgcn_perm(0u, z1, gs));
+        }
+    }
+}
+
+// OR the n <= 4 little-endian bytes of v at LDS byte address a (two dwords).
+__device__ __forceinline__ void or_bytes(uint32_t* ob, uint32_t a, uint32_t v)
+{
+    const uint64_t x = (uint64_t) v << (8 * (a & 3));
+    uint32_t* q = ob + (a >> 2);
+    lds_or(q, (uint32_t) x);
+    lds_or(q + 1, (uint32_t) (x >> 32));
+}
+
+// Encode a block of any width w > 0 with 0 < nj <= L values (values past nj
+// are zero) by OR-writes, value by value and without branches, for the rare
+// blocks encode_fast does not take (widths from 16, a segment's short last
+// block).  Writes of the zero values past nj go to the last value's bytes
+// or at most 30 bytes past the block.  Same format and arguments as
+// encode_fast.
+__device__ __forceinline__ void encode_general(uint32_t* ob, uint32_t o, uint32_t w, uint32_t nj, uint32_t s,
+    uint32_t s2, const uint32_t (&m)[L])
+{
+    const uint32_t B = w >> 3, r = w & 7, nsb = (nj + 7) >> 3;
+    const uint32_t hmask = B >= 4 ? ~0u : (1u << (8 * B)) - 1u;
+    const uint32_t fmask = (1u << r) - 1u;
+    const uint32_t hp = o + 1 + nsb, lo0 = hp + B * nj;
+    // width byte and sign bytes (the bits of values past nj are zero, and so
+    // are the sign bytes past nsb that these dwords also cover)
+    const uint32_t sb = __builtin_bswap32(s);
+    const uint32_t x0 = w | (sb << 8), x1 = (sb >> 24) | (s2 << (8 + 40 - L));
+    const uint32_t sel = shift_sel(o & 3);
+    uint32_t* p = ob + (o >> 2);
+    lds_or(p, __builtin_amdgcn_perm(x0, 0u, sel));
+    lds_or(p + 1, __builtin_amdgcn_perm(x1, x0, sel));
+    lds_or(p + 2, __builtin_amdgcn_perm(0u, x1, sel));
+#pragma unroll
+    for (uint32_t j = 0; j < L; j++) {
+        __builtin_amdgcn_sched_barrier(0);  // value by value: few registers live
+        // high bytes: B bytes of m >> r at hp + j B (m >> r < 2^32, so width 32 has 4)
+        or_bytes(ob, hp + min(j, nj - 1) * B, (m[j] >> r) & hmask);
+        // low bits: r bits at bit j r of the MSB-first stream (a 16-bit window)
+        const uint32_t bit = j * r;
+        const uint32_t be = (m[j] & fmask) << (16 - r - (bit & 7));
+        or_bytes(ob, lo0 + (bit >> 3), ((be >> 8) & 0xFFu) | ((be & 0xFFu) << 8));
+    }
+}
+
+// Write bytes [0, nbytes) of the wave's LDS buffer to dst with 16-byte stores
+// on dst's 16-byte grid; the partial first and last chunk go out byte by
+// byte, so bytes outside [dst, dst + nbytes) are never written.
+__device__ __forceinline__ void store_wave(const uint32_t* ob, uint8_t* dst, uint32_t nbytes)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    const uintptr_t d0 = (uintptr_t) dst, d1 = d0 + nbytes;
+    const uintptr_t c0 = d0 & ~(uintptr_t) 15;
+    const uint32_t nch = nbytes ? (uint32_t) ((d1 - c0 + 15) >> 4) : 0u;
+    for (uint32_t c = lane; c < nch; c += 64) {
+        const uintptr_t ca = c0 + 16ull * c;
+        if (ca >= d0 && ca + 16 <= d1) {
+            const uint32_t o = (uint32_t) (ca - d0), wi = o >> 2, sh = o & 3;
+            const uint32_t a = ob[wi], b = ob[wi + 1], cc = ob[wi + 2], dd = ob[wi + 3], e = ob[wi + 4];
+            *(uint4*) ca = make_uint4(__builtin_amdgcn_alignbyte(b, a, sh), __builtin_amdgcn_alignbyte(cc, b, sh),
+                __builtin_amdgcn_alignbyte(dd, cc, sh), __builtin_amdgcn_alignbyte(e, dd, sh));
+        } else {
+            const uintptr_t s = ca > d0 ? ca : d0, e = ca + 16 < d1 ? ca + 16 : d1;
+            for (uintptr_t p = s; p < e; p++)
+                *(uint8_t*) p = ((const uint8_t*) ob)[p - d0];
+        }
+    }
+}
+
+// The operands of one sum: in device memory (read through the scalar cache)
+// or in the kernel arguments.
+struct SumIO {
+    const HomsumIO* io;
+    __device__ const uint8_t* bytes(int r) const { return ldc(&io->in[r].bytes); }
+    __device__ const uint8_t* index(int r) const { return ldc(&io->in[r].index); }
+};
+
+struct ArgIO {
+    const HomsumIO& io;
+    __device__ const uint8_t* bytes(int r) const { return io.in[r].bytes; }
+    __device__ const uint8_t* index(int r) const { return io.in[r].index; }
+};
+
+// Staging area of one operand's tile bytes: half of the wave's output buffer
+// (the buffer is free until the encode), less the reach of the block reads
+// past the staged bytes (up to 47 bytes).
+constexpr uint32_t kStageHalf = kOutBuf / 2;
+constexpr uint32_t kStageCap = kStageHalf - 64;
+static_assert(kStageHalf % 16 == 0, "stage halves stay 16-byte aligned");
+
+// One tile of a fused k-operand sum, executed by one wave.  obuf: the wave's
+// LDS buffer (kOutBuf bytes, 16-byte aligned).
+template <typename Src>
+__device__ __forceinline__ void homsum_tile_lane(const Geom& g, const Tiling& tl, int k, const Src& src,
+    uint8_t* out, uint64_t out_cap, uint8_t* oindex, unsigned long long* size_out, uint64_t t,
+    unsigned long long* status, WsHeader* ws, uint32_t nchains, uint32_t* obuf)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    const Tile x = tile_of(g, tl, (uint32_t) t);
+    const bool has = lane < x.nblk;
+    const uint32_t nj = has ? min(tile_block_len(g, x, lane), L) : 0u;
+    bool bad_byte = false, bad_width = false, bad_over = false;
+
+    // ---- operand table: lane r < k holds operand r's stream, index, tile
+    // offset and stream size, read once per tile; each operand's values come
+    // out of it with readlane, so no operand waits on a scalar load chain
+    const uint8_t* tb_bytes = nullptr;
+    const uint8_t* tb_index = nullptr;
+    uint64_t tb_goff = 0, tb_size = 0;
+    if (lane < (uint32_t) k) {
+        tb_bytes = src.bytes((int) lane);
+        tb_index = src.index((int) lane);
+        const IndexView iv = index_view(g, tb_index);
+        tb_goff = iv.group_off[x.grp0];
+        tb_size = iv.meta[0];
+    }
+    // (readlane returns int: widen through uint32_t, never by sign extension)
+    auto rl64 = [](uint64_t v, int r) -> uint64_t {
+        const uint32_t hi = (uint32_t) __builtin_amdgcn_readlane((uint32_t) (v >> 32), r);
+        const uint32_t lo = (uint32_t) __builtin_amdgcn_readlane((uint32_t) v, r);
+        return ((uint64_t) hi << 32) | lo;
+    };
+    struct Op {
+        const uint8_t* bytes;
+        const uint8_t* index;
+        uint64_t goff, size;
+    };
+    auto op_of = [&](int r) {
+        Op o;
+        o.bytes = (const uint8_t*) rl64((uint64_t) tb_bytes, r);
+        o.index = (const uint8_t*) rl64((uint64_t) tb_index, r);
+        o.goff = rl64(tb_goff, r);
+        o.size = rl64(tb_size, r);
+        return o;
+    };
+    auto widths = [&](const Op& o) -> uint32_t { return has ? (uint32_t) o.index[x.g0 + lane] : 0u; };
+
+    // this lane's block of an operand: width, size, offset in the operand's
+    // view of the tile bytes (one scan of the block sizes), and whether the
+    // tile lies inside its stream.  The view starts at the 4-byte aligned
+    // address at or below the tile's bytes and ends with the stream's last
+    // dword (lim: its length).
+    struct Loc {
+        View v;
+        uint32_t w, bs, o, lim, tot;
+        bool skip;
+    };
+    auto locate = [&](const Op& p, uint32_t wq) {
+        Loc c;
+        c.w = wq;
+        if (c.w > 32) {
+            bad_width = true;
+            c.w = 0;
+        }
+        c.bs = has ? (nj == L ? full_block_bytes(c.w) : block_bytes(c.w, nj)) : 0u;
+        const uint32_t incl = wave_scan_dpp(c.bs);
+        c.tot = __builtin_amdgcn_readlane(incl, 63);
+        c.skip = p.goff > p.size || c.tot > p.size - p.goff;
+        const uint8_t* base = p.bytes + p.goff;
+        const uint32_t delta = (uint32_t) ((uintptr_t) base & 3u);
+        const uint64_t len = c.skip ? 0ull : align_up(delta + (p.size - p.goff), 4);
+        c.lim = len > 0xFFFFFFFCull ? 0xFFFFFFFCu : (uint32_t) len;
+        c.v = make_view(base - delta, len);
+        c.o = delta + incl - c.bs;
+        c.tot += delta;  // bytes of the view that hold the tile
+        return c;
+    };
+    // move an operand's tile bytes (view bytes [0, tot)) into LDS half `half`
+    // with 16-byte direct-to-LDS buffer loads, 64 lanes per instruction; the
+    // view's range check zero-fills past the stream.  False when they do not
+    // fit (then the blocks are read with per-lane loads).
+    auto stage = [&](const Loc& c, uint32_t half) -> bool {
+        const uint32_t nch = (c.tot + 15) >> 4;
+        if (c.skip || 16 * nch > kStageCap)
+            return false;
+        uint8_t* st = (uint8_t*) obuf + half * kStageHalf;
+        for (uint32_t run = 0; run * 64 < nch; run++) {
+            const uint32_t ch = run * 64 + lane;
+            if (ch < nch)
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(c.v, (__attribute__((address_space(3))) void*) (st + run * 1024),
+                    16, 16 * ch, 0, 0, 0);
+        }
+        return true;
+    };
+    // this lane's block from the staged bytes (aligned dword reads: a
+    // misaligned LDS read is legal but slow)
+    auto read_staged = [&](const Loc& c, uint32_t half, Block& q) {
+        const uint32_t* st = obuf + half * (kStageHalf / 4);
+        const uint32_t* p = st + (c.o >> 2);
+        uint32_t D[HEADW + 1];
+#pragma unroll
+        for (uint32_t i = 0; i < HEADW + 1; i++)
+            D[i] = p[i];
+#pragma unroll
+        for (uint32_t i = 0; i < HEADW; i++)
+            q.h[i] = __builtin_amdgcn_alignbyte(D[i + 1], D[i], c.o);  // uses o & 3
+        const uint32_t r = c.w & 7, lo = c.o + 1 + SB + (c.w >> 3) * L;
+#pragma unroll
+        for (uint32_t gp = 0; gp < NGRP; gp++) {
+            const uint32_t a = lo + gp * r;
+            const uint32_t* pg = st + (a >> 2);
+            const uint32_t E0 = pg[0], E1 = pg[1], E2 = pg[2];
+            const uint32_t sel = 0x00010203u + (a & 3u) * 0x01010101u;  // big-endian dword at byte a & 3
+            q.ge[2 * gp] = __builtin_amdgcn_perm(E1, E0, sel);
+            q.ge[2 * gp + 1] = __builtin_amdgcn_perm(E2, E1, sel);
+        }
+    };
+    // the same from the view, per lane (tiles too large to stage)
+    auto read_direct = [&](const Loc& c, Block& q) {
+        const uint4 h0 = bl16(c.v, c.o), h1 = bl16(c.v, c.o + 16);
+        const uint3 h2 = bl12(c.v, c.o + 32);
+        q.h[0] = h0.x; q.h[1] = h0.y; q.h[2] = h0.z; q.h[3] = h0.w;
+        q.h[4] = h1.x; q.h[5] = h1.y; q.h[6] = h1.z; q.h[7] = h1.w;
+        q.h[8] = h2.x; q.h[9] = h2.y; q.h[10] = h2.z;
+        const uint32_t r = c.w & 7, lo = c.o + 1 + SB + (c.w >> 3) * L;
+#pragma unroll
+        for (uint32_t gp = 0; gp < NGRP; gp++) {
+            const uint2 e = bl8(c.v, lo + gp * r);
+            q.ge[2 * gp] = __builtin_bswap32(e.x);
+            q.ge[2 * gp + 1] = __builtin_bswap32(e.y);
+        }
+    };
+
+    uint32_t acc[L];
+#pragma unroll
+    for (uint32_t j = 0; j < L; j++)
+        acc[j] = 0;
+    // bit r: this lane's block of operand r needs the general decoder, which
+    // runs after the pipelined loop (fewer registers live there)
+    uint32_t gen = 0;
+
+    // ---- operand pipeline: operand r+1's tile bytes move into the other LDS
+    // half while operand r is decoded, operand r+2's widths are on their way
+    {
+        Op on = op_of(0);
+        Loc cn = locate(on, widths(on));
+        bool sn = stage(cn, 0);
+        uint32_t wq2 = 0;
+        Op o2;
+        if (k > 1) {
+            o2 = op_of(1);
+            wq2 = widths(o2);
+        }
+        for (int r = 0; r < k; r++) {
+            const Loc c = cn;
+            const bool staged = sn;
+            __builtin_amdgcn_s_waitcnt(0);  // operand r's bytes are in LDS, operand r+1's widths in registers
+            Block q;
+            q.w = c.w;
+            q.o = c.o;
+            q.lim = c.lim;
+            if (staged)
+                read_staged(c, (uint32_t) r & 1u, q);
+            else
+                read_direct(c, q);
+            // (phase fences keep the scheduler from interleaving the block
+            // reads, the next operand's staging and the decode, which would
+            // hold all their values at once)
+            __builtin_amdgcn_sched_barrier(0);
+            if (r + 1 < k) {
+                cn = locate(o2, wq2);
+                sn = stage(cn, (uint32_t) (r + 1) & 1u);
+            }
+            if (r + 2 < k) {
+                o2 = op_of(r + 2);
+                wq2 = widths(o2);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+            // decode_fast reads a block correctly unless the per-lane loads
+            // read it and it ends in the view's last three bytes (a dword
+            // straddling the end reads as zero); such a block, a wide one or
+            // a short one goes to the general decoder.  Zero-width and empty
+            // lanes add nothing either way.  Branch-free: a lane that does
+            // not use the fast decoder runs it at width 0, which adds zeros.
+            bad_over |= c.skip;
+            const bool fast = !has || q.w == 0 || (nj == L && q.w < 16 && (staged || q.o + c.bs + 3 <= q.lim));
+            const bool use = fast && !c.skip;
+            if (!fast && !c.skip)
+                gen |= 1u << r;
+            // (a zero-width block in the view's last three bytes is not
+            // checked when read per lane: its first dword reads as zero)
+            bad_byte |= use && has && (q.w || staged || q.o + 4 <= q.lim) && (q.h[0] & 0xFFu) != q.w;
+            q.w = use ? q.w : 0u;
+            decode_fast(q, acc);
+        }
+    }
+    if (__any(gen)) {
+        for (int r = 0; r < k; r++) {
+            if (!__any((gen >> r) & 1u))
+                continue;
+            const Op o = op_of(r);
+            const Loc c = locate(o, widths(o));
+            if ((gen >> r) & 1u)
+                bad_byte |= decode_general(c.v, c.o, c.w, nj, acc);
+        }
+    }
+    if (__any(bad_width) && lane == 0)
+        raise_error(ws, ZK_ERR_WIDTH);
+    if (__any(bad_byte) && lane == 0)
+        raise_error(ws, ZK_ERR_MISMATCH);
+    if (__any(bad_over) && lane == 0)
+        raise_error(ws, ZK_ERR_OVERRUN);
+
+    // seed of the segment (its first tile): the operands' seeds add in int32
+    uint32_t seed = 0;
+    if (x.lt == 0) {
+        uint32_t sv = 0;
+        if (lane < (uint32_t) k) {
+            const IndexView iv = index_view(g, tb_index);
+            const uint64_t so = iv.seed_off[x.s];
+            if (so + 4 <= tb_size)
+                sv = load_u32_bytes(tb_bytes + so);
+            else
+                raise_error(ws, ZK_ERR_OVERRUN);
+        }
+        seed = (uint32_t) wave_sum64(sv);
+    }
+
+    // ---- output: widths, sizes, offsets
+    if (nj < L) {
+#pragma unroll
+        for (uint32_t j = 0; j < L; j++)
+            if (j >= nj)
+                acc[j] = 0;
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    // signs, then the magnitudes in place of the deltas
+    uint32_t s = 0, s2 = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < 32; j++)
+        s = __builtin_amdgcn_alignbit(s, acc[j], 31);  // (s << 1) | sign of value j
+#pragma unroll
+    for (uint32_t j = 32; j < L; j++)
+        s2 = __builtin_amdgcn_alignbit(s2, acc[j], 31);
+    // (phase fences: without them the scheduler computes the magnitudes
+    // before the sign words and keeps both arrays live)
+    __builtin_amdgcn_sched_barrier(0);
+    uint32_t mx = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < L; j++) {
+        acc[j] = mag(acc[j]);
+        mx = max(mx, acc[j]);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    const uint32_t w = has ? dwidth(mx) : 0u;
+    const uint32_t bs = has ? (nj == L ? full_block_bytes(w) : block_bytes(w, nj)) : 0u;
+    const uint32_t incl = wave_scan_dpp(bs);
+    const uint32_t excl = incl - bs;
+    const uint32_t total = __builtin_amdgcn_readlane(incl, 63);
+    if (lane == 0)
+        lookback_publish(status, t, 0, total, 0);  // successors see the aggregate now
+    const bool fast = nj == L && w < 16;
+    const bool wide = __any(has && w >= 8);
+
+    // ---- encode into the LDS buffer and store
+    {
+        const uint32_t nz = (total + 64 + 15) / 16;
+        for (uint32_t i = lane; i < nz; i += 64)
+            reinterpret_cast<uint4*>(obuf)[i] = make_uint4(0, 0, 0, 0);
+    }
+    // (two separate branches rather than if/else: the compiler would hoist
+    // what the two encoders share above an if/else and keep it live)
+    if (has && w && fast)
+        encode_fast(obuf, excl, w, s, s2, acc, wide);
+    if (has && w && !fast)
+        encode_general(obuf, excl, w, nj, s, s2, acc);
+    // the tile's stream offset: after its encoding, so predecessors had longer to publish
+    const unsigned long long ex = lookback(status, t, 0, total, 0, ws, true);
+    const uint64_t P = 8ull * g.T + 4ull * (x.s + 1) + ex;
+    const bool fits = P + total <= out_cap;
+    if (!fits && lane == 0)
+        raise_error(ws, ZK_ERR_NOSPACE);
+    if (fits)
+        store_wave(obuf, out + P, total);
+
+    // ---- output index, seed and chunk header
+    IndexView ox = index_view(g, oindex);
+    if (has)
+        ox.widths[x.g0 + lane] = (uint8_t) w;
+    if ((lane & 15) == 0 && lane / 16 < x.ngrp)
+        ox.group_off[x.grp0 + lane / 16] = P + excl;
+    if (lane == 0) {
+        if (x.lt == 0) {
+            const uint64_t so = P - 4;
+            if (so + 4 <= out_cap) {
+                store_bytes(out + so, &seed, 4);
+                if (x.s < g.T) {
+                    const unsigned long long h = so - 8ull * g.T;
+                    store_bytes(out + 8ull * x.s, &h, 8);
+                }
+            }
+            ox.seed_off[x.s] = so;
+        }
+        if (t == tl.ntiles - 1) {
+            // the last tile ends the stream: if it fits, every tile fits
+            const unsigned long long sz = fits ? P + total : 0ull;
+            ox.meta[0] = sz;
+            if (size_out)
+                *size_out = sz;
+            chain_finished(ws, nchains);
+        }
+    }
+}
+
+}  // namespace
+
+// One fused sum (kernel arguments hold its geometry and pointers).
+__global__ void __launch_bounds__(64 * kLaneWaves, 4) k_homsum_lane(HomsumArgs a)
+{
+    uint64_t t0;
+    unsigned long long* status;
+    if (!launch_status(a.ws, a.job.tl.ntiles, t0, status, kLaneWaves))
+        return;
+    const uint32_t wv = threadIdx.x >> 6;
+    const uint64_t t = t0 + wv;
+    if (t >= a.job.tl.ntiles)
+        return;
+    const ArgIO src{a.job.io};
+    homsum_tile_lane(a.job.g, a.job.tl, a.job.p.k, src, a.job.io.out, a.job.io.out_cap, a.job.io.index,
+        a.job.io.size_out, t, status, a.ws, 1, zk_lds + wv * (kOutBuf / 4));
+}
+
+// Uniform batch: shared geometry, per-job pointers in device memory; tiles of
+// the jobs interleave (tile b is tile b / njobs of job b % njobs).
+__global__ void __launch_bounds__(64 * kLaneWaves, 4) k_homsum_lane_batch(BatchArgs b)
+{
+    uint64_t t0;
+    unsigned long long* status;
+    if (!launch_status(b.ws, b.total_tiles, t0, status, kLaneWaves))
+        return;
+    const uint32_t wv = threadIdx.x >> 6;
+    const uint64_t tg = t0 + wv;
+    if (tg >= b.total_tiles)
+        return;
+    uint32_t jidx, t;
+    if (b.interleave) {
+        jidx = (uint32_t) tg % b.njobs;
+        t = (uint32_t) tg / b.njobs;
+    } else {
+        jidx = 0;
+        t = (uint32_t) tg;
+    }
+    jidx = __builtin_amdgcn_readfirstlane(jidx);
+    t = __builtin_amdgcn_readfirstlane(t);
+    const HomsumIO* io = (const HomsumIO*) b.jobs + jidx;
+    const SumIO src{io};
+    homsum_tile_lane(b.g, b.tl, b.p.k, src, ldc(&io->out), ldc(&io->out_cap), ldc(&io->index),
+        ldc(&io->size_out), t, status + (uint64_t) jidx * b.tl.ntiles * kStatusStride, b.ws, b.total_chains,
+        zk_lds + wv * (kOutBuf / 4));
+}
+
+// Mixed batch: every job carries its own geometry; job j owns launch tiles
+// [job_tile0[j], job_tile0[j+1]).
+__global__ void __launch_bounds__(64 * kLaneWaves, 4) k_homsum_lane_mixed(BatchArgs b)
+{
+    uint64_t t0;
+    unsigned long long* status;
+    if (!launch_status(b.ws, b.total_tiles, t0, status, kLaneWaves))
+        return;
+    const uint32_t wv = threadIdx.x >> 6;
+    const uint64_t tg = t0 + wv;
+    if (tg >= b.total_tiles)
+        return;
+    uint32_t lo = 0, hi = b.njobs;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (ldc(&b.job_tile0[mid]) <= tg)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    const uint32_t jidx = __builtin_amdgcn_readfirstlane(lo);
+    const uint64_t tile0 = ldc(&b.job_tile0[jidx]);
+    const uint32_t t = __builtin_amdgcn_readfirstlane((uint32_t) (tg - tile0));
+    const HomsumJob* job = (const HomsumJob*) b.jobs + jidx;
+    const Geom g = job->g;
+    const Tiling tl = job->tl;
+    const SumIO src{&job->io};
+    homsum_tile_lane(g, tl, ldc(&job->p.k), src, ldc(&job->io.out), ldc(&job->io.out_cap), ldc(&job->io.index),
+        ldc(&job->io.size_out), t, status + tile0 * kStatusStride, b.ws, b.total_chains, zk_lds + wv * (kOutBuf / 4));
+}
+
+size_t lane_lds_bytes() { return (size_t) kLaneWaves * kOutBuf; }
+
+hipError_t launch_homsum_lane(const HomsumArgs& a, hipStream_t s)
+{
+    const uint32_t groups = (uint32_t) ceil_div(a.job.tl.ntiles, kLaneWaves);
+    hipLaunchKernelGGL(k_homsum_lane, dim3(groups), dim3(64 * kLaneWaves), lane_lds_bytes(), s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_homsum_lane_batch(const BatchArgs& b, hipStream_t s)
+{
+    const uint32_t groups = (uint32_t) ceil_div(b.total_tiles, kLaneWaves);
+    if (b.uniform)
+        hipLaunchKernelGGL(k_homsum_lane_batch, dim3(groups), dim3(64 * kLaneWaves), lane_lds_bytes(), s, b);
+    else
+        hipLaunchKernelGGL(k_homsum_lane_mixed, dim3(groups), dim3(64 * kLaneWaves), lane_lds_bytes(), s, b);
+    return hipGetLastError();
+}
+
+}  // namespace zk
