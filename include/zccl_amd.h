@@ -150,6 +150,18 @@ int zccl_homsum_dev(const unsigned char *const *ins, const void *const *indices,
     size_t nbEle, int blockSize, int nchunks, unsigned char *out, size_t out_cap, void *index,
     unsigned long long *d_size, void *workspace, size_t workspace_bytes, void *stream);
 
+/* No counterpart in the reference: zccl_homsum_dev for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops).  It takes no
+ * workspace and allocates nothing: the lookback state lives in a fixed 1 MiB
+ * device array inside the library, one per GPU, so geometries needing more
+ * than 65 520 tiles (about 1.5e8 values at blockSize 36) return
+ * ZCCL_EWORKSPACE.  Calls that overlap on one GPU race on that array.
+ * Kernel-side errors (workspace_errors bits) are not reported; a sum that did
+ * not fit out_cap leaves size 0 in the index and *d_size. */
+int zccl_homsum_dev_noalloc(const unsigned char *const *ins, const void *const *indices, int k,
+    size_t nbEle, int blockSize, int nchunks, unsigned char *out, size_t out_cap, void *index,
+    unsigned long long *d_size, void *stream);
+
 /* Rebuild the index of a device stream that has none (e.g. produced by the
  * CPU library); reads at most in_cap bytes of `in`. */
 int zccl_parse_dev(const unsigned char *in, size_t in_cap, size_t nbEle, int blockSize,

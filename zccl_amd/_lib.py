@@ -34,7 +34,7 @@ EXPORTS = (
     "zccl_worst_case_bytes", "zccl_index_bytes", "zccl_num_tiles", "zccl_num_blocks",
     "zccl_index_size_offset", "zccl_workspace_bytes",
     "zccl_workspace_init", "zccl_workspace_errors",
-    "zccl_compress_dev", "zccl_decompress_dev", "zccl_homsum_dev", "zccl_parse_dev",
+    "zccl_compress_dev", "zccl_decompress_dev", "zccl_homsum_dev", "zccl_homsum_dev_noalloc", "zccl_parse_dev",
     "zccl_case_count_dev",
     "zccl_batch_homsum_create", "zccl_batch_compress_create", "zccl_batch_decompress_create",
     "zccl_batch_tiles", "zccl_batch_launch", "zccl_batch_destroy",
@@ -99,6 +99,7 @@ def lib() -> ctypes.CDLL:
         "zccl_compress_dev": ([vp, sz, f32, i32, i32, vp, sz, vp, vp, vp, sz, vp], i32),
         "zccl_decompress_dev": ([vp, vp, sz, f32, i32, i32, vp, vp, sz, vp], i32),
         "zccl_homsum_dev": ([vp, vp, i32, sz, i32, i32, vp, sz, vp, vp, vp, sz, vp], i32),
+        "zccl_homsum_dev_noalloc": ([vp, vp, i32, sz, i32, i32, vp, sz, vp, vp, vp], i32),
         "zccl_parse_dev": ([vp, sz, sz, i32, i32, vp, vp, vp, sz, vp], i32),
         "zccl_case_count_dev": ([vp, vp, sz, i32, i32, vp, vp], i32),
         "zccl_batch_homsum_create": ([i32, i32, vp, i32, i32, vp, vp, vp, vp, vp, i32_p], vp),

@@ -331,6 +331,62 @@ extern "C" int zccl_homsum_dev(const unsigned char* const* ins, const void* cons
     return ZCCL_OK;
 }
 
+// ------------------------------------------------------------ allocation-free sum
+//
+// zccl_homsum_dev without a caller workspace, for callers that cannot
+// allocate (graph capture, serving loops).  The lookback workspace is this
+// file-scope device array: each GPU holds its own copy until the process
+// ends.  Every call on a GPU shares that copy, so calls that overlap on one
+// GPU race; each call re-initialises the part it uses with kernels (no
+// hipMemsetAsync), so a call never depends on what an earlier one left.
+
+static constexpr size_t kScratchBytes = 1u << 20;  // serves launches of up to 65 520 tiles
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+static std::mutex g_scratch_mu;
+static std::vector<void*> g_scratch_ptr;  // device address of `scratch`, per device (null: not yet looked up)
+
+// Device address of this GPU's `scratch` (looked up once per device, with that
+// device current); null when the runtime cannot provide it.
+static void* scratch_ptr()
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) {
+        (void) hipGetLastError();
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if ((int) g_scratch_ptr.size() <= dev)
+        g_scratch_ptr.resize(dev + 1, nullptr);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+            (void) hipGetLastError();
+            return nullptr;
+        }
+        g_scratch_ptr[dev] = p;
+    }
+    return g_scratch_ptr[dev];
+}
+
+extern "C" int zccl_homsum_dev_noalloc(const unsigned char* const* ins, const void* const* indices, int k,
+    size_t n, int L, int T, unsigned char* out, size_t out_cap, void* index, unsigned long long* d_size,
+    void* stream)
+{
+    Geom g;
+    if (!geom_of(n, L, T, &g))
+        return ZCCL_EINVAL;
+    const uint64_t tiles = op_tiling(1, g).ntiles;
+    const uint64_t need = workspace_bytes(tiles);
+    if (need > kScratchBytes)
+        return ZCCL_EWORKSPACE;  // beyond the fixed scratch: use zccl_homsum_dev with a workspace
+    void* ws = scratch_ptr();
+    if (!ws)
+        return ZCCL_EDEVICE;
+    HIP_TRY(launch_ws_init((WsHeader*) ws, tiles, (hipStream_t) stream));
+    return zccl_homsum_dev(ins, indices, k, n, L, T, out, out_cap, index, d_size, ws, (size_t) need, stream);
+}
+
 // ------------------------------------------------------------ batch plans
 //
 // A plan is built on the host: jobs are validated and their device
