@@ -93,6 +93,8 @@ def lib():
         L.zo_homsum.restype = ctypes.c_int64
         L.zo_parse.argtypes = [vp, u64, u64, i32, i32, vp, vp, vp, vp]
         L.zo_parse.restype = i32
+        L.zo_doc_chain.argtypes = [vp, i32, u64, f32, i32, i32, vp, vp]
+        L.zo_doc_chain.restype = i32
         f64 = ctypes.c_double
         L.zo_dims_create.argtypes = [i32, vp]
         L.zo_dims_create.restype = None
@@ -211,6 +213,19 @@ def homsum(streams: Sequence[np.ndarray], n: int, L: int, T: int) -> np.ndarray:
     out = np.empty(cap, dtype=np.uint8)
     m = _check(lib().zo_homsum(ptrs, sizes, k, n, L, T, _ptr(out), cap), "homsum")
     return out[:m].copy()
+
+
+def doc_chain(xs: Sequence[np.ndarray], eb: float, L: int, T: int):
+    """The reference's DOC ring reduction of one segment (ranks' copies in ring
+    order): returns (last rank's FP32 partial sum, value after the compressed
+    allgather)."""
+    xs = [np.ascontiguousarray(x, dtype=np.float32) for x in xs]
+    n = xs[0].size
+    ptrs = (ctypes.c_void_p * len(xs))(*[x.ctypes.data for x in xs])
+    acc = np.empty(n, dtype=np.float32)
+    fin = np.empty(n, dtype=np.float32)
+    _check(lib().zo_doc_chain(ptrs, len(xs), n, ctypes.c_float(eb), L, T, _ptr(acc), _ptr(fin)), "doc_chain")
+    return acc, fin
 
 
 def parse(stream: np.ndarray, n: int, L: int, T: int):

@@ -516,6 +516,45 @@ int zo_decompress(const uint8_t *in, uint64_t in_size, uint64_t n, float eb, int
     return rc;
 }
 
+/* The decompress-operate-compress (DOC) reduction of one segment, the
+ * reference's non-homomorphic ring MPI_Allreduce_ZCCL_RI2_mt_oa_record
+ * (ZCCL/src/ZCCL_ring.c:979-1108): the first rank's values travel
+ * compressed (:1049-1055), each receiver decompresses them (:1066-1067) and
+ * adds its own values in FP32 (MPI_Reduce_local, MPI_SUM, :1068-1069) before
+ * compressing the partial sum for the next rank.  xs[0..P-1] are the P ranks'
+ * copies of the segment in ring order (xs[0] starts the chain); acc receives
+ * the last rank's FP32 partial sum.  final, if not null, receives what every
+ * rank holds after the compressed allgather (:1094-1105, :814-978): the
+ * decompression of acc's compression. */
+int zo_doc_chain(const float *const *xs, int P, uint64_t n, float eb, int L, int T, float *acc, float *final)
+{
+    zo_geom g;
+    if (P < 1 || zo_geometry(n, L, T, &g) != ZO_OK)
+        return ZO_EINVAL;
+    const uint64_t cap = zo_worst_case_bytes(n, L, T);
+    uint8_t *buf = (uint8_t *) malloc((size_t) cap);
+    float *dec = (float *) malloc((size_t) n * sizeof(float));
+    int rc = ZO_OK;
+    memcpy(acc, xs[0], (size_t) n * sizeof(float));
+    for (int m = 1; m < P && rc == ZO_OK; m++) {
+        int64_t sz = zo_compress(acc, n, eb, L, T, buf, cap);
+        if (sz < 0) {
+            rc = (int) sz;
+            break;
+        }
+        rc = zo_decompress(buf, (uint64_t) sz, n, eb, L, T, dec);
+        for (uint64_t i = 0; rc == ZO_OK && i < n; i++)
+            acc[i] = dec[i] + xs[m][i];
+    }
+    if (rc == ZO_OK && final) {
+        int64_t sz = zo_compress(acc, n, eb, L, T, buf, cap);
+        rc = sz < 0 ? (int) sz : zo_decompress(buf, (uint64_t) sz, n, eb, L, T, final);
+    }
+    free(dec);
+    free(buf);
+    return rc;
+}
+
 /* k-operand homomorphic sum.  The reference adds two streams block by block
  * (ZCCL/src/ZCCLd_float.c:1072 seeds, :1099-1219 blocks: both zero -> zero
  * block; one zero -> copy the other's bytes; else decode, add in int32,

@@ -72,6 +72,11 @@ int zo_decompress(const uint8_t *in, uint64_t in_size, uint64_t n, float eb, int
 int64_t zo_homsum(const uint8_t *const *ins, const uint64_t *in_sizes, int k, uint64_t n, int L,
     int T, uint8_t *out, uint64_t cap);
 
+/* Decompress-operate-compress reduction of one segment over P ranks
+ * (xs in ring order): acc = the last rank's FP32 partial sum; final (may be
+ * null) = what every rank holds after the compressed allgather. */
+int zo_doc_chain(const float *const *xs, int P, uint64_t n, float eb, int L, int T, float *acc, float *final);
+
 /* Parse a stream into its block index: widths[nblocks], byte offset of every
  * block's width byte (block_off[nblocks]) and of every segment's seed
  * (seed_off[S]); *total receives the stream size implied by the parse. */

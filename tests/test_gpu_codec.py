@@ -286,11 +286,47 @@ def test_errors_are_reported_not_faulted(codec, capi):
     # n < T
     with pytest.raises(Exception):
         codec.compress(dev(x[:3]), eb, L, T)
-    # drop-in status
-    capi.zccl_set_num_chunks(8)
-    dec = np.zeros(4, dtype=np.float32)
-    capi.ZCCL_float_decompress_openmp_threadblock_arg(dec.ctypes.data, 4, eb, L, bad.ctypes.data)
-    assert capi.zccl_last_error() != 0
+    # drop-in status: the same corrupt stream (chunk 0 broken, last chunk
+    # intact, so the host-side sizing succeeds) reaches the parse and reports
+    capi.zccl_set_num_chunks(T)
+    try:
+        dec = np.zeros(n, dtype=np.float32)
+        capi.ZCCL_float_decompress_openmp_threadblock_arg(dec.ctypes.data, n, eb, L, bad.ctypes.data)
+        assert capi.zccl_last_error() == -3  # ZCCL_ECORRUPT
+    finally:
+        capi.zccl_set_num_chunks(0)
+
+
+def test_failed_parse_leaves_no_stream_size(codec):
+    """A parse that fails on the last chunk must not leave an old stream size
+    in the index (later kernels read up to it): the index is pre-filled with
+    0xFF, the last chunk's first width byte is corrupted, and the size word and
+    the optional device size must read 0 afterwards, with ECORRUPT reported."""
+    from zccl_amd._lib import lib
+
+    n, eb, L, T = 50_000, 1e-3, 36, 4
+    say("failed parse")
+    x = gen.make("walk", n, 3)
+    cx = O.compress(x, eb, L, T)
+    w, bo, so, tot = O.parse(cx, n, L, T)
+    bad = cx.copy()
+    nb_chunk = len(bo) // T
+    bad[int(bo[(T - 1) * nb_chunk])] = 77  # first block of the last chunk
+    data = dev(bad)
+    index = torch.full((codec.index_bytes(n, L, T),), 0xFF, dtype=torch.uint8, device="cuda")
+    d_size = torch.full((1,), -1, dtype=torch.int64, device="cuda")
+    ws = codec.Workspace(1024)
+    rc = lib().zccl_parse_dev(ctypes.c_void_p(data.data_ptr()), data.numel(), n, L, T,
+                              ctypes.c_void_p(index.data_ptr()), ctypes.c_void_p(d_size.data_ptr()),
+                              ws.ptr, ws.nbytes, codec._stream_handle())
+    assert rc == 0
+    torch.cuda.synchronize()
+    off = int(lib().zccl_index_size_offset(n, L, T))
+    size_word = int(index[off: off + 8].cpu().numpy().view(np.uint64)[0])
+    assert size_word == 0
+    assert int(d_size.item()) == 0
+    with pytest.raises(Exception):
+        codec.check_errors(ws)
 
 
 @pytest.mark.parametrize("L", [36, 32])
@@ -519,3 +555,47 @@ def test_staged_allreduce_forms_agree(codec):
         t = truth[ar.displs[j]: ar.displs[j] + ar.cnts[j]]
         err = (seg.double() - t).abs().max().item()
         assert err <= (P - 1) * (eb + 1e-4), (j, err)
+
+
+def test_doc_ring_bit_exact(codec):
+    """The decompress-operate-compress ring (ZCCL/src/ZCCL_ring.c:979-1108) on
+    the GPU equals the oracle's zo_doc_chain bit for bit for every segment:
+    each rank's FP32 partial sum, and the value after the compressed
+    allgather.  The oracle is pinned against the compiled reference
+    (tests/test_oracle.py::test_doc_chain_equals_compiled_reference_ring)."""
+    from zccl_amd.allreduce import StagedAllreduce, ring_order
+
+    P, count, L, T, eb = 4, 4 * 60_001 + 3, 36, 8, 1e-3
+    say("doc ring", P, count)
+    hx = [gen.make("grf", count, 1300 + r) for r in range(P)]
+    xs = [dev(x) for x in hx]
+    ar = StagedAllreduce(P=P, count=count, eb=eb, L=L, T=T)
+    doc = ar.reduce_scatter_doc(xs)
+    torch.cuda.synchronize()
+    codec.check_errors()
+    for j, seg in enumerate(doc):
+        a, b = ar.displs[j], ar.displs[j] + ar.cnts[j]
+        acc, fin = O.doc_chain([hx[r][a:b] for r in ring_order(P, j)], eb, L, T)
+        assert np.array_equal(bits(seg.cpu().numpy()), bits(acc)), j
+        g = codec.decompress(codec.compress(seg, eb, L, T), eb)
+        assert np.array_equal(bits(g.cpu().numpy()), bits(fin)), j
+
+
+def test_two_operand_doc_bit_exact(codec):
+    """The two-operand DOC workflow timed by examples/testfloat_homocompress_
+    fastmode1_arg.c:211-243: decompress both streams, add in FP32 (add_vectors,
+    :40-46), compress the sum -- equal to the oracle's composition bit for bit,
+    next to the homomorphic add of the same streams."""
+    n, L, T, eb = 300_007, 36, 8, 1e-3
+    say("two-operand doc", n)
+    x, y = gen.make("grf", n, 1400), gen.make("grf", n, 1401)
+    sx, sy = codec.compress(dev(x), eb, L, T), codec.compress(dev(y), eb, L, T)
+    dsum = codec.decompress(sx, eb) + codec.decompress(sy, eb)
+    doc = codec.compress(dsum, eb, L, T)
+    hom = codec.homsum([sx, sy])
+    torch.cuda.synchronize()
+    codec.check_errors()
+    cx, cy = O.compress(x, eb, L, T), O.compress(y, eb, L, T)
+    odsum = O.decompress(cx, n, eb, L, T) + O.decompress(cy, n, eb, L, T)
+    assert np.array_equal(doc.to_host(), O.compress(odsum, eb, L, T))
+    assert np.array_equal(hom.to_host(), O.homsum([cx, cy], n, L, T))

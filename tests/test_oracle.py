@@ -159,6 +159,43 @@ def test_reference_ring_chain_equals_fused_sum():
 
 
 @needs_ref
+def test_doc_chain_equals_compiled_reference_ring():
+    """The DOC ring (ZCCL/src/ZCCL_ring.c:979-1108) rebuilt from the compiled
+    reference's own compress / decompress calls and FP32 adds equals the
+    oracle's zo_doc_chain bit for bit, before and after the allgather."""
+    n, eb, L, T, P = 20_003, 1e-2, 36, 4, 5
+    xs = [gen.make("walk", n, 70)] + [gen.perturb(gen.make("walk", n, 70), 71 + r) for r in range(P - 1)]
+    acc = xs[0].copy()
+    for m in range(1, P):
+        acc = O.ref_decompress(O.ref_compress(acc, eb, L, T), n, eb, L, T) + xs[m]
+    fin = O.ref_decompress(O.ref_compress(acc, eb, L, T), n, eb, L, T)
+    oacc, ofin = O.doc_chain(xs, eb, L, T)
+    assert np.array_equal(bits(oacc), bits(acc))
+    assert np.array_equal(bits(ofin), bits(fin))
+
+
+def test_doc_chain_closed_forms():
+    """One rank: the partial sum is the input itself and the allgathered value
+    its round trip; any P: the error against the exact sum stays within one
+    round trip's bound per compression plus FP32 rounding per add."""
+    n, eb, L, T = 30_001, 1e-3, 36, 8
+    x = gen.make("grf", n, 80)
+    acc, fin = O.doc_chain([x], eb, L, T)
+    assert np.array_equal(bits(acc), bits(x))
+    assert np.array_equal(bits(fin), bits(O.decompress(O.compress(x, eb, L, T), n, eb, L, T)))
+    P = 6
+    xs = [gen.perturb(x, 81 + r) for r in range(P)]
+    acc, fin = O.doc_chain(xs, eb, L, T)
+    exact = np.sum(np.stack([v.astype(np.float64) for v in xs]), axis=0)
+    mag = np.sum(np.stack([np.abs(v.astype(np.float64)) for v in xs]), axis=0)
+    assert np.all(np.abs(acc - exact) <= (P - 1) * eb + 2.0 ** -20 * P * mag)
+    assert np.all(np.abs(fin - exact) <= P * eb + 2.0 ** -20 * (P + 1) * mag)
+    # a mistake in the chain order or a skipped add shows as a larger error
+    wrong = acc - xs[-1]
+    assert np.max(np.abs(wrong - exact)) > 10 * eb
+
+
+@needs_ref
 def test_reference_add_accumulates_size():
     """ZCCL/src/ZCCLd_float.c:1027,1655 add to *final_cmpSize instead of assigning."""
     x = gen.make("walk", 3000, 5)

@@ -131,7 +131,10 @@ class StagedAllreduce:
         if batched:
             from . import batch
 
-            key = tuple(v.data_ptr() for v in views) + tuple(s.data.data_ptr() for s in sums)
+            # (the index pointers too: a stream re-indexed over the same bytes
+            # needs a new plan)
+            key = (tuple(v.data_ptr() for v in views) + tuple(s.data.data_ptr() for s in sums) +
+                   tuple(s.index.data_ptr() for s in sums))
             if getattr(self, "_dplan_key", None) != key:
                 self._dplan = batch.decompress_plan(list(sums), [self.eb] * self.P, views)
                 self._dplan_key = key

@@ -195,17 +195,6 @@ extern "C" size_t zccl_index_size_offset(size_t n, int L, int T)
 
 extern "C" size_t zccl_workspace_bytes(size_t max_tiles) { return (size_t) workspace_bytes(max_tiles); }
 
-// Worst-case bytes of segment s (seed included).
-static uint64_t worst_segment(const Geom& g, uint32_t s)
-{
-    uint64_t len = s < g.T ? g.tb : g.R;
-    if (len < 2)
-        return 4;
-    uint64_t nfull = (len - 1) / g.L;
-    uint32_t tail = (uint32_t) ((len - 1) % g.L);
-    return 4 + nfull * block_bytes(32, g.L) + (tail ? block_bytes(32, tail) : 0);
-}
-
 // ------------------------------------------------------------ device API
 
 static int check_ws(void* ws, size_t ws_bytes, uint64_t tiles)
@@ -802,30 +791,59 @@ int finish(Ctx& c)
 
 unsigned long long* small_slot(Ctx& c, int i) { return (unsigned long long*) c.small.p + i; }
 
-// Make stream bytes available on the device and build their index.  For a
-// host stream at most `cap` bytes are copied; for a device stream the
-// allocation bounds the read.
+// Exact size of a host stream: its last header entry locates chunk T-1,
+// whose blocks (and the remainder segment's) are walked by their width bytes.
+// Nothing past the bytes the walk proves to be in the stream is read, and the
+// walk never leaves the geometry's worst-case size; 0 when the header or a
+// width byte is malformed.  ZCCL/src/ZCCLd_float.c:597-745 (the same walk).
+uint64_t host_stream_bytes(const unsigned char* in, const Geom& g)
+{
+    const uint64_t worst = worst_case_bytes(g);
+    uint64_t h = 0;
+    memcpy(&h, in + 8ull * (g.T - 1), 8);
+    if (h > worst)
+        return 0;
+    uint64_t pos = 8ull * g.T + h;
+    const uint32_t nseg = g.R ? 2u : 1u;
+    for (uint32_t sidx = 0; sidx < nseg; sidx++) {
+        const uint32_t s = g.T - 1 + sidx;
+        const uint64_t nb = s < g.T ? g.nbT : g.nbR;
+        const uint64_t len = s < g.T ? g.tb : g.R;
+        pos += 4;  // seed
+        for (uint64_t jb = 0; jb < nb; jb++) {
+            if (pos >= worst)
+                return 0;
+            const uint32_t w = in[pos];
+            if (w > 32)
+                return 0;
+            const uint64_t left = len - 1 - jb * g.L;
+            pos += block_bytes(w, (uint32_t) (left < g.L ? left : g.L));
+        }
+    }
+    return pos <= worst ? pos : 0;
+}
+
+// Make stream bytes available on the device and build their index.  A host
+// stream is copied exactly: cap bytes when its size is stated (cap_exact),
+// else the size its own header and width bytes prove.  For a device stream
+// the allocation bounds the read.
 int stage_stream(Ctx& c, const Geom& g, const unsigned char* bytes, size_t cap, bool cap_exact,
     int slot, const unsigned char** dptr)
 {
     size_t avail = 0;
-    bool dev = device_ptr(bytes, &avail);
-    uint64_t hdr_last = 0;
-    if (8ull * g.T > (dev ? avail : SIZE_MAX))
-        return ZCCL_ECORRUPT;
-    {
-        uint64_t h = 0;
-        if (dev)
-            HIP_TRY(hipMemcpy(&h, bytes + 8ull * (g.T - 1), 8, hipMemcpyDeviceToHost));
-        else
-            memcpy(&h, bytes + 8ull * (g.T - 1), 8);
-        hdr_last = 8ull * g.T + h;
+    const bool dev = device_ptr(bytes, &avail);
+    if (dev) {
+        if (8ull * g.T > avail)
+            return ZCCL_ECORRUPT;
+        cap = cap_exact ? std::min(cap, avail) : avail;
+    } else if (cap_exact) {
+        if (cap < 8ull * g.T)
+            return ZCCL_ECORRUPT;
+    } else {
+        cap = (size_t) host_stream_bytes(bytes, g);
+        if (!cap)
+            return ZCCL_ECORRUPT;
     }
-    uint64_t bound = hdr_last + worst_segment(g, g.T - 1) + (g.R ? worst_segment(g, g.T) : 0);
-    if (!cap_exact)
-        cap = (size_t) std::min<uint64_t>(bound, std::max<uint64_t>(4ull * g.n, hdr_last + 8));
-    if (dev)
-        cap = std::min(cap, avail);
     const unsigned char* d = bytes;
     if (!dev) {
         int rc = ensure(c.in[slot], cap + 16);
@@ -1292,12 +1310,16 @@ int dropin_compress3(const float* x, unsigned char* out, size_t d1, size_t d2, s
 
 // Bytes of a host stream: the last sub-volume's offset from the header, then
 // its width bytes give its payload size (ZCCL/src/ZCCLd_float.c:2946-2951).
-uint64_t host_stream_bytes3(const unsigned char* in, const Geom3& g, const std::vector<Part3>& parts)
+uint64_t host_stream_bytes3(const unsigned char* in, const Geom3& g, const std::vector<Part3>& parts,
+    uint64_t worst)
 {
     const Part3& p = parts[g.T - 1];
     uint64_t off;
     memcpy(&off, in + 8ull * (g.T - 1), 8);
-    const uint64_t w0 = 8ull * g.T + off, nb = (uint64_t) p.b1 * p.b2 * p.b3;
+    const uint64_t nb = (uint64_t) p.b1 * p.b2 * p.b3;
+    if (off > worst || 8ull * g.T + off + nb > worst)
+        return 0;  // malformed header: the walk would leave the worst-case stream
+    const uint64_t w0 = 8ull * g.T + off;
     uint64_t tot = w0 + nb, lb = 0;
     for (uint32_t bx = 0; bx < p.b1; bx++)
         for (uint32_t by = 0; by < p.b2; by++)
@@ -1307,7 +1329,7 @@ uint64_t host_stream_bytes3(const unsigned char* in, const Geom3& g, const std::
                 const uint32_t w = in[w0 + lb];
                 tot += payload3(w > 32 ? 32 : w, ex * ey * ez);
             }
-    return tot;
+    return tot <= worst ? tot : 0;
 }
 
 int dropin_decompress3(float* y, const unsigned char* in, size_t d1, size_t d2, size_t d3, int Bs, double eb)
@@ -1332,7 +1354,12 @@ int dropin_decompress3(float* y, const unsigned char* in, size_t d1, size_t d2, 
     const unsigned char* din = in;
     uint64_t in_bytes = avail_i;
     if (!idev) {
-        in_bytes = host_stream_bytes3(in, g, parts);
+        size_t worst = 0, wsb = 0;
+        if ((rc = zccl_sz3d_sizes(d1, d2, d3, Bs, T, &worst, &wsb)))
+            return rc;
+        in_bytes = host_stream_bytes3(in, g, parts, worst);
+        if (!in_bytes)
+            return ZCCL_ECORRUPT;
         if ((rc = ensure(c->out, in_bytes)))
             return rc;
         HIP_TRY(hipMemcpyAsync(c->out.p, in, in_bytes, hipMemcpyHostToDevice, c->stream));

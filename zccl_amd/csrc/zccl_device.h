@@ -116,20 +116,36 @@ __device__ __forceinline__ void raise_error(WsHeader* ws, unsigned long long bit
     atomicOr(&ws->errors, bit);
 }
 
-// Decoupled lookback, executed by wave 0 of the workgroup.  Tiles t0..t form
+// Decoupled lookback, executed by one wave of the tile.  Tiles t0..t form
 // one scan (status entry of tile i at status[i * kStatusStride]); tile t0
 // starts from `init`.  Returns init + the aggregates of tiles t0..t-1 (mod
-// 2^62; callers mask further) and publishes tile t's inclusive value.  The
-// nearest predecessor is polled alone first; a 64-wide window is read only
-// when it has published just its aggregate.  Every wait is bounded: past
-// kSpinLimit polls the tile records ZK_ERR_SPIN and proceeds, so the grid
-// always drains.
+// 2^62; callers mask further) and publishes tile t's inclusive value.
+// Predecessors are polled 64 at a time (one per lane), and a window waits
+// only for those up to the nearest inclusive value.  A waiting wave backs off
+// (s_sleep doubling up to 64 x 64 cycles): polls are uncached device-scope
+// loads, and thousands of waves polling at full rate take memory bandwidth
+// from the tiles they wait for.  Every wait is bounded: past kSpinLimit
+// polls the tile records ZK_ERR_SPIN and proceeds, so the grid always drains.
 // First half of the lookback, callable early by any one thread: the tile's
 // aggregate (the chain's first tile: its inclusive value) becomes visible.
 __device__ __forceinline__ void lookback_publish(unsigned long long* status, uint64_t t, uint64_t t0,
     unsigned long long agg, unsigned long long init)
 {
     st_publish(&status[t * kStatusStride], t == t0 ? kFlagInc | ((init + agg) & kValMask) : kFlagAgg | (agg & kValMask));
+}
+
+// Sleep of the backoff step b (s_sleep takes an immediate: 64 cycles per unit).
+__device__ __forceinline__ void backoff_sleep(uint32_t b)
+{
+    switch (b) {
+    case 0: __builtin_amdgcn_s_sleep(1); break;
+    case 1: __builtin_amdgcn_s_sleep(2); break;
+    case 2: __builtin_amdgcn_s_sleep(4); break;
+    case 3: __builtin_amdgcn_s_sleep(8); break;
+    case 4: __builtin_amdgcn_s_sleep(16); break;
+    case 5: __builtin_amdgcn_s_sleep(32); break;
+    default: __builtin_amdgcn_s_sleep(64); break;
+    }
 }
 
 // `published`: lookback_publish already ran for this tile.
@@ -146,23 +162,28 @@ __device__ __forceinline__ unsigned long long lookback(unsigned long long* statu
     uint32_t spins = 0;
     bool timed_out = false;
     while (true) {
-        const int width = end == (int64_t) t ? 1 : 64;
-        int64_t idx = end - 1 - lane;
-        bool act = lane < width && idx >= (int64_t) t0;
+        const int64_t idx = end - 1 - lane;
+        const bool act = idx >= (int64_t) t0;
         unsigned long long st = 0;
+        uint32_t b = 0;
         while (true) {
             if (act)
                 st = st_poll(&status[(uint64_t) idx * kStatusStride]);
-            bool waiting = act && (st >> 62) == 0;
-            if (!__any(waiting))
+            // only the predecessors up to the nearest inclusive one matter: a
+            // slow tile further back must not hold this one
+            const unsigned long long zm = __ballot(act && (st >> 62) == 0);
+            const unsigned long long im = __ballot(act && (st >> 62) == 2);
+            const unsigned long long need = im ? ((im & (~im + 1)) << 1) - 1 : ~0ull;
+            if (!(zm & need))
                 break;
             if (++spins > kSpinLimit) {
-                if (waiting)
+                if (act && (st >> 62) == 0)
                     st = kFlagInc;
                 timed_out = true;
                 break;
             }
-            __builtin_amdgcn_s_sleep(1);
+            backoff_sleep(b);
+            b += b < 6 ? 1u : 0u;
         }
         unsigned long long pm = __ballot(act && (st >> 62) == 2);
         unsigned long long val = act ? (st & kValMask) : 0ull;
@@ -174,7 +195,7 @@ __device__ __forceinline__ unsigned long long lookback(unsigned long long* statu
             break;
         }
         excl += wave_sum64(val);
-        end -= width;
+        end -= 64;
     }
     if (timed_out && lane == 0)
         raise_error(ws, ZK_ERR_SPIN);

@@ -1665,6 +1665,12 @@ hipError_t launch_batch(BatchKind kind, const BatchArgs& b, uint32_t threads, si
 
 hipError_t launch_parse(const ParseArgs& a, hipStream_t s)
 {
+    // the stream size starts at 0 (a fill kernel, safe under graph capture):
+    // only a parse that reaches the stream's end writes it, so a failed parse
+    // leaves no stale size for later kernels to read up to
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, (unsigned long long*) (a.index + a.g.ix_meta), 1ull, 0ull);
+    if (a.size_out)
+        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, a.size_out, 1ull, 0ull);
     hipLaunchKernelGGL(k_parse, dim3(a.g.T), dim3(64), 0, s, a);
     return hipGetLastError();
 }
