@@ -120,36 +120,85 @@ def test_random_against_oracle(codec, kind, n, eb, L, T):
     assert np.array_equal(bits(dh.cpu().numpy()), bits(O.decompress(ch, n, eb, L, T)))
 
 
+# the two forms of the blockSize-36 sum: two-pass (stage area) and
+# single-pass (decoupled lookback)
+FORMS = [None, False]
+FORM_IDS = ["two_pass", "lookback"]
+
+
+@pytest.mark.parametrize("stage", FORMS, ids=FORM_IDS)
 @pytest.mark.parametrize("k", [1, 3, 8, 17, 32, 33, 40])
-def test_fused_k_operand_sum(codec, k):
+def test_fused_k_operand_sum(codec, k, stage):
     n, eb, L, T = 200_001, 1e-3, 36, 16
-    say("homsum k", k)
+    say("homsum k", k, FORM_IDS[FORMS.index(stage)])
     base = gen.make("grf", n, 3)
     xs = [gen.perturb(base, 100 + r) for r in range(k)]
     ss = [codec.compress(dev(x), eb, L, T) for x in xs]
-    h = codec.homsum(ss)
+    h = codec.homsum(ss, stage=stage)
     torch.cuda.synchronize()
     codec.check_errors()
     ref = O.homsum([O.compress(x, eb, L, T) for x in xs], n, L, T)
     assert np.array_equal(h.to_host(), ref)
 
 
+@pytest.mark.parametrize("stage", FORMS, ids=FORM_IDS)
 @pytest.mark.parametrize("kind,L", [("ladder", 36), ("ladder", 20), ("wide", 36)])
-def test_fused_sum_every_width(codec, kind, L):
+def test_fused_sum_every_width(codec, kind, L, stage):
     """Eight operands whose blocks take every width (0..24 for the ladder,
     21..32 for the wide kind), so the decoder meets each split of a width
-    into high bytes and low bits, r = 0 included; bytes equal the oracle's."""
+    into high bytes and low bits, r = 0 included; bytes equal the oracle's.
+    The wide sums have tiles too large for a wave's LDS buffer, which are
+    encoded half a wave at a time."""
     n, eb, T, k = 40_000, 1.0, 4, 8
-    say("homsum widths", kind, L)
+    say("homsum widths", kind, L, FORM_IDS[FORMS.index(stage)])
     xs = [gen.make(kind, n, 500 + r) for r in range(k)]
     ss = [codec.compress(dev(x), eb, L, T) for x in xs]
-    h = codec.homsum(ss)
+    h = codec.homsum(ss, stage=stage)
     torch.cuda.synchronize()
     codec.check_errors()
     cs = [O.compress(x, eb, L, T) for x in xs]
     w = set(O.parse(cs[0], n, L, T)[0].tolist())
     assert w >= (set(range(25)) if kind == "ladder" else set(range(21, 33)))
     assert np.array_equal(h.to_host(), O.homsum(cs, n, L, T))
+
+
+@pytest.mark.parametrize("stage", FORMS, ids=FORM_IDS)
+def test_fused_sum_tiles_wider_than_lds(codec, stage):
+    """Integer data of magnitude up to 2^26 at eb = 1: every block is 28 to
+    31 bits wide, so a 64-block tile of the sum holds more than a wave's
+    8 KB LDS buffer and is encoded half a wave at a time."""
+    n, eb, L, T, k = 50_000, 1.0, 36, 4, 8
+    say("homsum tiles wider than LDS", FORM_IDS[FORMS.index(stage)])
+    rng = np.random.default_rng(77)
+    xs = [rng.integers(-2 ** 26, 2 ** 26, n).astype(np.float32) for _ in range(k)]
+    ss = [codec.compress(dev(x), eb, L, T) for x in xs]
+    h = codec.homsum(ss, stage=stage)
+    torch.cuda.synchronize()
+    codec.check_errors()
+    cs = [O.compress(x, eb, L, T) for x in xs]
+    want = O.homsum(cs, n, L, T)
+    w = O.parse(want, n, L, T)[0].astype(np.int64)
+    bb = np.where(w == 0, 1, 6 + (9 * w + 1) // 2)
+    assert bb[:64].sum() > 8128  # the first tile of chunk 0 is too wide for the buffer
+    assert np.array_equal(h.to_host(), want)
+
+
+@pytest.mark.parametrize("stage", FORMS, ids=FORM_IDS)
+def test_sum_output_too_small_is_reported(codec, stage):
+    """A sum whose output does not fit out_cap raises the capacity error and
+    leaves size 0, in both forms."""
+    n, eb, L, T = 120_000, 1e-3, 36, 8
+    say("homsum nospace", FORM_IDS[FORMS.index(stage)])
+    x = gen.make("grf", n, 41)
+    sx = codec.compress(dev(x), eb, L, T)
+    torch.cuda.synchronize()
+    codec.check_errors()
+    small = codec.empty_stream(n, L, T, capacity=sx.size // 2)
+    codec.homsum([sx, sx], out=small, stage=stage)
+    torch.cuda.synchronize()
+    with pytest.raises(Exception):
+        codec.check_errors()
+    assert small.size == 0
 
 
 # ------------------------------------------------------------ foreign streams
@@ -372,6 +421,12 @@ def test_batch_plans_uniform_and_mixed(codec):
         sy = [codec.compress(dev(y), e, L, T) for y, e in zip(ys, ebs)]
         sums = [codec.empty_stream(n, L, T) for n in ns]
         hp = batch.homsum_plan([[a, b] for a, b in zip(outs, sy)], sums)
+        # single-pass form first, then the two-pass form (uniform batches
+        # only) over the same outputs
+        hp.launch(stage=False)
+        torch.cuda.synchronize()
+        codec.check_errors()
+        single = [s.to_host() for s in sums]
         hp.launch()
         decs = [torch.empty(n, dtype=torch.float32, device="cuda") for n in ns]
         dp = batch.decompress_plan(sums, ebs, decs)
@@ -384,6 +439,7 @@ def test_batch_plans_uniform_and_mixed(codec):
             assert np.array_equal(outs[i].to_host(), cx), (sizes, i)
             ch = O.homsum([cx, cy], n, L, T)
             assert np.array_equal(sums[i].to_host(), ch), (sizes, i)
+            assert np.array_equal(single[i], ch), (sizes, i)
             assert np.array_equal(bits(decs[i].cpu().numpy()), bits(O.decompress(ch, n, ebs[i], L, T)))
 
 

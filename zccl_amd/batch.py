@@ -18,11 +18,12 @@ import torch
 
 from . import _lib
 from ._lib import check, lib
-from .codec import DeviceStream, Workspace, _stream_handle, default_workspace
+from .codec import STAGED_L, DeviceStream, Workspace, _stream_handle, default_stage, default_workspace
 
 
 class BatchPlan:
-    def __init__(self, handle: int, keep: list, outs: Optional[List[DeviceStream]] = None):
+    def __init__(self, handle: int, keep: list, outs: Optional[List[DeviceStream]] = None,
+                 staged: bool = False):
         if not handle:
             raise _lib.ZcclError(-1, "batch plan")
         self.handle = ctypes.c_void_p(handle)
@@ -36,14 +37,26 @@ class BatchPlan:
         check(L.zccl_batch_bind(self.handle, ctypes.c_void_p(self.desc.data_ptr())), "batch bind")
         self._keep = keep
         self.outs = outs or []
+        # a homomorphic-sum batch of one geometry and operand count at
+        # blockSize 36: the two-pass form applies
+        self.staged = staged
 
-    def launch(self, ws: Optional[Workspace] = None) -> None:
+    def launch(self, ws: Optional[Workspace] = None, stage=None) -> None:
+        """stage: a StageArea for the two-pass sum (None: the default one,
+        False: the single-pass form); ignored by other batches."""
         w = ws if ws is not None else default_workspace(self.tiles)
         if w.max_tiles < self.tiles:
             raise _lib.ZcclError(-6, f"workspace holds {w.max_tiles} tiles, batch needs {self.tiles}")
         for o in self.outs:
             o.invalidate()
-        check(lib().zccl_batch_launch(self.handle, w.ptr, w.nbytes, _stream_handle()), "batch launch")
+        if self.staged and stage is not False:
+            st = stage if stage is not None else default_stage(self.tiles)
+            if st.max_tiles < self.tiles:
+                raise _lib.ZcclError(-6, f"stage area holds {st.max_tiles} tiles, batch needs {self.tiles}")
+            check(lib().zccl_batch_launch_staged(self.handle, w.ptr, w.nbytes, st.ptr, st.nbytes, _stream_handle()),
+                  "batch launch")
+        else:
+            check(lib().zccl_batch_launch(self.handle, w.ptr, w.nbytes, _stream_handle()), "batch launch")
 
     def close(self) -> None:
         if self.handle is not None:
@@ -84,7 +97,8 @@ def homsum_plan(jobs: Sequence[Sequence[DeviceStream]], outs: Sequence[DeviceStr
     st = ctypes.c_int(0)
     h = lib().zccl_batch_homsum_plan(nj, ks, n, L, T, ins, ixs, od, ocap, oix, ctypes.byref(st))
     check(st.value, "batch homsum plan")
-    return BatchPlan(h, [list(jobs), list(outs)], list(outs))
+    staged = L == STAGED_L and len({(o.n, len(ops)) for o, ops in zip(outs, jobs)}) == 1
+    return BatchPlan(h, [list(jobs), list(outs)], list(outs), staged=staged)
 
 
 def compress_plan(xs: Sequence[torch.Tensor], ebs: Sequence[float], outs: Sequence[DeviceStream]) -> BatchPlan:

@@ -92,6 +92,38 @@ def default_workspace(tiles: int) -> Workspace:
     return ws
 
 
+class StageArea:
+    """Stage area of the two-pass homomorphic sum (blockSize 36): a record
+    and a 9600-byte slot per tile, for sums of up to ``max_tiles`` tiles.
+    Like a workspace, it serves one launch at a time."""
+
+    def __init__(self, max_tiles: int, device: Optional[torch.device] = None):
+        self.max_tiles = max(int(max_tiles), 1)
+        self.nbytes = int(lib().zccl_sum_stage_bytes(self.max_tiles))
+        self.buf = torch.empty(self.nbytes, dtype=torch.uint8, device=device or "cuda")
+
+    @property
+    def ptr(self) -> ctypes.c_void_p:
+        return _ptr(self.buf)
+
+
+_DEFAULT_STAGE: Dict[Tuple[int, int], StageArea] = {}
+
+
+def default_stage(tiles: int) -> StageArea:
+    dev = torch.cuda.current_device()
+    key = (dev, torch.cuda.current_stream().cuda_stream)
+    st = _DEFAULT_STAGE.get(key)
+    if st is None or st.max_tiles < tiles:
+        st = StageArea(tiles)
+        _DEFAULT_STAGE[key] = st
+    return st
+
+
+# blockSize of the two-pass sum
+STAGED_L = 36
+
+
 @dataclass
 class DeviceStream:
     """A compressed stream resident in HBM together with its index.
@@ -177,10 +209,12 @@ def decompress(s: DeviceStream, eb: float, out: Optional[torch.Tensor] = None,
 
 
 def homsum(streams: Sequence[DeviceStream], out: Optional[DeviceStream] = None,
-           ws: Optional[Workspace] = None) -> DeviceStream:
+           ws: Optional[Workspace] = None, stage=None) -> DeviceStream:
     """Fused homomorphic sum of k streams.  Up to 32 operands go through one
     launch; more are summed in groups (the encoding is canonical, so grouping
-    does not change the bytes)."""
+    does not change the bytes).  For blockSize 36 the two-pass form runs with
+    ``stage`` (a StageArea; None: the per-stream default one); stage=False
+    runs the single-pass lookback form."""
     if not streams:
         raise ValueError("need at least one stream")
     n, L, T = streams[0].n, streams[0].L, streams[0].T
@@ -188,22 +222,34 @@ def homsum(streams: Sequence[DeviceStream], out: Optional[DeviceStream] = None,
         if (s.n, s.L, s.T) != (n, L, T):
             raise ValueError("operands differ in (n, L, T)")
     w = _ws_for(n, L, T, ws)
+    st = None
+    if L == STAGED_L and stage is not False:
+        tiles = num_tiles(n, L, T)
+        st = stage if stage is not None else default_stage(tiles)
+        if st.max_tiles < tiles:
+            raise _lib.ZcclError(-6, f"stage area holds {st.max_tiles} tiles, sum needs {tiles}")
     ops = list(streams)
     while len(ops) > MAX_OPERANDS:
-        part = _homsum_launch(ops[:MAX_OPERANDS], None, w)
+        part = _homsum_launch(ops[:MAX_OPERANDS], None, w, st)
         ops = [part] + ops[MAX_OPERANDS:]
-    return _homsum_launch(ops, out, w)
+    return _homsum_launch(ops, out, w, st)
 
 
-def _homsum_launch(ops: Sequence[DeviceStream], out: Optional[DeviceStream], w: Workspace) -> DeviceStream:
+def _homsum_launch(ops: Sequence[DeviceStream], out: Optional[DeviceStream], w: Workspace,
+                   st: Optional[StageArea] = None) -> DeviceStream:
     n, L, T = ops[0].n, ops[0].L, ops[0].T
     o = out if out is not None else empty_stream(n, L, T, device=ops[0].data.device)
     k = len(ops)
     ins = (ctypes.c_void_p * k)(*[s.data.data_ptr() for s in ops])
     ixs = (ctypes.c_void_p * k)(*[s.index.data_ptr() for s in ops])
     o.invalidate()
-    check(lib().zccl_homsum_dev(ins, ixs, k, n, L, T, _ptr(o.data), o.data.numel(), _ptr(o.index),
-                                None, w.ptr, w.nbytes, _stream_handle()), "homsum")
+    if st is not None:
+        check(lib().zccl_homsum_dev_staged(ins, ixs, k, n, L, T, _ptr(o.data), o.data.numel(), _ptr(o.index),
+                                           None, w.ptr, w.nbytes, st.ptr, st.nbytes, _stream_handle()),
+              "homsum")
+    else:
+        check(lib().zccl_homsum_dev(ins, ixs, k, n, L, T, _ptr(o.data), o.data.numel(), _ptr(o.index),
+                                    None, w.ptr, w.nbytes, _stream_handle()), "homsum")
     return o
 
 

@@ -287,33 +287,69 @@ extern "C" int zccl_decompress_dev(const unsigned char* in, const void* index, s
     return ZCCL_OK;
 }
 
-extern "C" int zccl_homsum_dev(const unsigned char* const* ins, const void* const* indices, int k,
-    size_t n, int L, int T, unsigned char* out, size_t out_cap, void* index,
-    unsigned long long* d_size, void* ws, size_t ws_bytes, void* stream)
+static int homsum_args(const unsigned char* const* ins, const void* const* indices, int k, size_t n, int L,
+    int T, unsigned char* out, size_t out_cap, void* index, unsigned long long* d_size, void* ws, size_t ws_bytes,
+    HomsumArgs* a)
 {
     Geom g;
     if (!ins || !indices || k < 1 || k > kMaxOperands || !out || !index || !geom_of(n, L, T, &g))
         return ZCCL_EINVAL;
-    HomsumArgs a;
-    memset(&a, 0, sizeof(a));
-    a.job.g = g;
-    a.job.tl = op_tiling(1, g);
-    int rc = check_ws(ws, ws_bytes, a.job.tl.ntiles);
+    memset(a, 0, sizeof(*a));
+    a->job.g = g;
+    a->job.tl = op_tiling(1, g);
+    int rc = check_ws(ws, ws_bytes, a->job.tl.ntiles);
     if (rc)
         return rc;
-    a.job.p = op_params(a.job.tl, 0.0, 0.0f, k);
+    a->job.p = op_params(a->job.tl, 0.0, 0.0f, k);
     for (int r = 0; r < k; r++) {
         if (!ins[r] || !indices[r] || ins[r] == out)
             return ZCCL_EINVAL;
-        a.job.io.in[r].bytes = ins[r];
-        a.job.io.in[r].index = (const uint8_t*) indices[r];
+        a->job.io.in[r].bytes = ins[r];
+        a->job.io.in[r].index = (const uint8_t*) indices[r];
     }
-    a.job.io.out = out;
-    a.job.io.out_cap = out_cap;
-    a.job.io.index = (uint8_t*) index;
-    a.job.io.size_out = d_size;
-    a.ws = (WsHeader*) ws;
-    if (is_lane_tiling(g, a.job.tl))
+    a->job.io.out = out;
+    a->job.io.out_cap = out_cap;
+    a->job.io.index = (uint8_t*) index;
+    a->job.io.size_out = d_size;
+    a->ws = (WsHeader*) ws;
+    return ZCCL_OK;
+}
+
+extern "C" int zccl_homsum_dev(const unsigned char* const* ins, const void* const* indices, int k,
+    size_t n, int L, int T, unsigned char* out, size_t out_cap, void* index,
+    unsigned long long* d_size, void* ws, size_t ws_bytes, void* stream)
+{
+    HomsumArgs a;
+    int rc = homsum_args(ins, indices, k, n, L, T, out, out_cap, index, d_size, ws, ws_bytes, &a);
+    if (rc)
+        return rc;
+    if (is_lane_tiling(a.job.g, a.job.tl))
+        HIP_TRY(launch_homsum_lane(a, (hipStream_t) stream));
+    else
+        HIP_TRY(launch_homsum(a, (hipStream_t) stream));
+    return ZCCL_OK;
+}
+
+extern "C" size_t zccl_sum_stage_bytes(size_t max_tiles) { return (size_t) stage_area_bytes(max_tiles); }
+
+// A stage area serves the two-pass form when it holds every tile of the sum
+// and is 16-byte aligned.
+static bool stage_ok(const void* stage, size_t stage_bytes, uint64_t tiles)
+{
+    return stage && ((uintptr_t) stage & 15) == 0 && stage_bytes >= stage_area_bytes(tiles);
+}
+
+extern "C" int zccl_homsum_dev_staged(const unsigned char* const* ins, const void* const* indices, int k,
+    size_t n, int L, int T, unsigned char* out, size_t out_cap, void* index,
+    unsigned long long* d_size, void* ws, size_t ws_bytes, void* stage, size_t stage_bytes, void* stream)
+{
+    HomsumArgs a;
+    int rc = homsum_args(ins, indices, k, n, L, T, out, out_cap, index, d_size, ws, ws_bytes, &a);
+    if (rc)
+        return rc;
+    if (is_lane_tiling(a.job.g, a.job.tl) && stage_ok(stage, stage_bytes, a.job.tl.ntiles))
+        HIP_TRY(launch_homsum_lane_staged(a, (uint8_t*) stage, (hipStream_t) stream));
+    else if (is_lane_tiling(a.job.g, a.job.tl))
         HIP_TRY(launch_homsum_lane(a, (hipStream_t) stream));
     else
         HIP_TRY(launch_homsum(a, (hipStream_t) stream));
@@ -638,6 +674,22 @@ extern "C" int zccl_batch_launch(const zccl_batch* b, void* ws, size_t ws_bytes,
         HIP_TRY(launch_homsum_lane_batch(a, (hipStream_t) stream));
     else
         HIP_TRY(launch_batch(b->kind, a, b->threads, b->lds, (hipStream_t) stream));
+    return ZCCL_OK;
+}
+
+extern "C" int zccl_batch_launch_staged(const zccl_batch* b, void* ws, size_t ws_bytes, void* stage,
+    size_t stage_bytes, void* stream)
+{
+    if (!b || !b->bound)
+        return ZCCL_EINVAL;
+    if (!(b->lane && b->args.uniform && stage_ok(stage, stage_bytes, b->args.total_tiles)))
+        return zccl_batch_launch(b, ws, ws_bytes, stream);
+    int rc = check_ws(ws, ws_bytes, b->args.total_tiles);
+    if (rc)
+        return rc;
+    BatchArgs a = b->args;
+    a.ws = (WsHeader*) ws;
+    HIP_TRY(launch_homsum_lane_batch_staged(a, (uint8_t*) stage, (hipStream_t) stream));
     return ZCCL_OK;
 }
 

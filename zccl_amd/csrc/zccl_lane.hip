@@ -46,9 +46,13 @@ constexpr uint32_t SB = (L + 7) / 8;    // sign bytes of a whole block
 constexpr uint32_t HEADW = 11;          // dwords of bytes [0, 44): width, signs, 36 high bytes
 constexpr uint32_t NGRP = (L + 7) / 8;  // groups of (up to) 8 values in the low bits
 constexpr uint32_t kWorstBlock = 1 + SB + 4 * L;   // 150 bytes at width 32
-// Output buffer of one wave: a worst-case tile of 64 blocks plus the reach of
-// the OR-writes past a block's end (at most 48 bytes).
-constexpr uint32_t kOutBuf = kLaneTileBlocks * kWorstBlock + 64;
+// Output buffer of one wave: 8 KB, so that five workgroups of four waves
+// share a CU's 160 KB of LDS.  A tile's bytes plus the reach of the
+// OR-writes past a block's end (at most 48 bytes) fit unless the tile is
+// wide (the worst tile is 64 blocks of kWorstBlock bytes); such a tile is
+// encoded half a wave at a time, and a half always fits.
+constexpr uint32_t kOutBuf = 8192;
+static_assert(kLaneTileBlocks / 2 * kWorstBlock + 64 <= kOutBuf, "half a worst-case tile fits a wave buffer");
 static_assert(kOutBuf % 16 == 0, "wave buffers stay 16-byte aligned");
 
 static_assert(L % 4 == 0 && L > 32 && L <= 40, "the lane decoder is laid out for 36-value blocks");
@@ -129,14 +133,30 @@ struct Block {
 };
 
 // Add a whole block of width w < 16 to acc: magnitude = (high byte << r) |
-// low r bits, sign applied as (m ^ s) - s with s = 0 or ~0.  A zero width
-// adds zeros whatever the bytes.  ZCCL/src/ZCCL_TypeManager.c:423-517 (extract),
-// :906-950 (signs).
+// low r bits, times the value's sign as +1 or -1.  A zero width adds zeros
+// whatever the bytes.  wide: some lane of the wave has w >= 8 (otherwise no
+// lane has high bytes and they are not extracted).
+// ZCCL/src/ZCCL_TypeManager.c:423-517 (extract), :906-950 (signs).
+// a * b + c for a, b of 24 bits (signed) and any c, in one instruction.
+__device__ __forceinline__ uint32_t mad_i24(uint32_t a, uint32_t b, uint32_t c)
+{
+    uint32_t d;
+    asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "v"(c));
+    return d;
+}
+
+template <bool wide>
 __device__ __forceinline__ void decode_fast(const Block& q, uint32_t (&acc)[L])
 {
     const uint32_t r = q.w & 7, hb = q.w & 8;  // hb: bits per high byte (0 or 8)
     const uint32_t s0 = __builtin_amdgcn_alignbyte(q.h[1], q.h[0], 1);  // sign bytes 0..3, memory order
     const uint32_t s1 = q.h[1] >> 8;                                   // sign byte 4 in bits 7..0
+    // Sign words: a value's sign bit moved to an odd bit with a one below it,
+    // so the 2-bit field there reads -1 (sign set) or +1 as a signed number.
+    // so0/so1 hold the values whose bit is odd in s0/s1, se0/se1 (shifted up
+    // one) the values whose bit is even.
+    const uint32_t so0 = (s0 & 0xAAAAAAAAu) | 0x55555555u, se0 = ((s0 << 1) & 0xAAAAAAAAu) | 0x55555555u;
+    const uint32_t so1 = (s1 & 0xAAu) | 0x55u, se1 = ((s1 << 1) & 0xAAu) | 0x55u;
 #pragma unroll
     for (uint32_t p = 0; p < NGRP; p++) {
         // the group's low bits big-endian: value 8p+c's field is r bits from bit 64 - (c+1) r
@@ -149,15 +169,16 @@ __device__ __forceinline__ void decode_fast(const Block& q, uint32_t (&acc)[L])
                 break;
             if (c == 4)
                 gh = (uint32_t) (((((uint64_t) e0 << 32) | e1) << (4 * r)) >> 32);
-            const uint32_t hbyte = __builtin_amdgcn_ubfe(q.h[(6 + j) >> 2], 8 * ((6 + j) & 3), hb);
-            // {high byte, window} << r: the magnitude in the upper dword, the
-            // next value's window in the lower one
+            const uint32_t hbyte = wide ? __builtin_amdgcn_ubfe(q.h[(6 + j) >> 2], 8 * ((6 + j) & 3), hb) : 0u;
+            // {high byte, window} << r: the magnitude (< 2^15) in the upper
+            // dword, the next value's window in the lower one
             const uint64_t v = (((uint64_t) hbyte << 32) | gh) << r;
             const uint32_t m = (uint32_t) (v >> 32);
             gh = (uint32_t) v;
-            const uint32_t sm = j < 32 ? (uint32_t) __builtin_amdgcn_sbfe((int32_t) s0, 8 * (j >> 3) + 7 - (j & 7), 1)
-                                       : (uint32_t) __builtin_amdgcn_sbfe((int32_t) s1, 7 - (j - 32), 1);
-            acc[j] += (m ^ sm) - sm;
+            const uint32_t bit = j < 32 ? 8 * (j >> 3) + 7 - (j & 7) : 7 - (j - 32);
+            const uint32_t sw = j < 32 ? ((bit & 1) ? so0 : se0) : ((bit & 1) ? so1 : se1);
+            const int32_t f = __builtin_amdgcn_sbfe((int32_t) sw, bit & ~1u, 2);
+            acc[j] = mad_i24(m, (uint32_t) f, acc[j]);
         }
     }
 }
@@ -364,10 +385,14 @@ static_assert(kStageHalf % 16 == 0, "stage halves stay 16-byte aligned");
 
 // One tile of a fused k-operand sum, executed by one wave.  obuf: the wave's
 // LDS buffer (kOutBuf bytes, 16-byte aligned).
-template <typename Src>
+// Staged (pass one of the two-pass sum): no lookback; the tile's bytes go to
+// slot sidx of the stage area `area` of R tiles, its sizes and seed to record sidx,
+// and only the widths to the output index.
+template <bool Staged, typename Src>
 __device__ __forceinline__ void homsum_tile_lane(const Geom& g, const Tiling& tl, int k, const Src& src,
     uint8_t* out, uint64_t out_cap, uint8_t* oindex, unsigned long long* size_out, uint64_t t,
-    unsigned long long* status, WsHeader* ws, uint32_t nchains, uint32_t* obuf)
+    unsigned long long* status, WsHeader* ws, uint32_t nchains, uint32_t* obuf, uint8_t* area = nullptr,
+    uint64_t R = 0, uint64_t sidx = 0)
 {
     const uint32_t lane = threadIdx.x & 63;
     const Tile x = tile_of(g, tl, (uint32_t) t);
@@ -555,7 +580,7 @@ __device__ __forceinline__ void homsum_tile_lane(const Geom& g, const Tiling& tl
             // checked when read per lane: its first dword reads as zero)
             bad_byte |= use && has && (q.w || staged || q.o + 4 <= q.lim) && (q.h[0] & 0xFFu) != q.w;
             q.w = use ? q.w : 0u;
-            decode_fast(q, acc);
+            decode_fast<true>(q, acc);
         }
     }
     if (__any(gen)) {
@@ -621,31 +646,97 @@ __device__ __forceinline__ void homsum_tile_lane(const Geom& g, const Tiling& tl
     const uint32_t incl = wave_scan_dpp(bs);
     const uint32_t excl = incl - bs;
     const uint32_t total = __builtin_amdgcn_readlane(incl, 63);
-    if (lane == 0)
+    if (!Staged && lane == 0)
         lookback_publish(status, t, 0, total, 0);  // successors see the aggregate now
     const bool fast = nj == L && w < 16;
     const bool wide = __any(has && w >= 8);
 
+    if constexpr (Staged) {
+        uint8_t* slot = area + sizeof(StageRec) * R + kStageSlot * sidx;
+        if (total + 64 <= kOutBuf) {
+            const uint32_t nz = (total + 64 + 15) / 16;
+            for (uint32_t i = lane; i < nz; i += 64)
+                reinterpret_cast<uint4*>(obuf)[i] = make_uint4(0, 0, 0, 0);
+            if (has && w && fast)
+                encode_fast(obuf, excl, w, s, s2, acc, wide);
+            if (has && w && !fast)
+                encode_general(obuf, excl, w, nj, s, s2, acc);
+            // whole 16-byte chunks: the slot is the tile's own (total <= kStageSlot, a multiple of 16)
+            for (uint32_t i = lane; i < (total + 15) / 16; i += 64)
+                reinterpret_cast<uint4*>(slot)[i] = reinterpret_cast<const uint4*>(obuf)[i];
+        } else {
+            // too wide for the buffer: half a wave at a time, value by value
+            const uint32_t mid = __builtin_amdgcn_readlane(excl, 32);
+            for (uint32_t i = lane; i < (mid + 64 + 15) / 16; i += 64)
+                reinterpret_cast<uint4*>(obuf)[i] = make_uint4(0, 0, 0, 0);
+            if (lane < 32 && has && w)
+                encode_general(obuf, excl, w, nj, s, s2, acc);
+            store_wave(obuf, slot, mid);
+            for (uint32_t i = lane; i < (total - mid + 64 + 15) / 16; i += 64)
+                reinterpret_cast<uint4*>(obuf)[i] = make_uint4(0, 0, 0, 0);
+            if (lane >= 32 && has && w)
+                encode_general(obuf, excl - mid, w, nj, s, s2, acc);
+            store_wave(obuf, slot + mid, total - mid);
+        }
+        if (has)
+            index_view(g, oindex).widths[x.g0 + lane] = (uint8_t) w;
+        const uint32_t e16 = __builtin_amdgcn_readlane(excl, 16), e32 = __builtin_amdgcn_readlane(excl, 32),
+                       e48 = __builtin_amdgcn_readlane(excl, 48);
+        if (lane == 0) {
+            StageRec rec;
+            rec.total = total;
+            rec.e16 = (uint16_t) e16;
+            rec.e32 = (uint16_t) e32;
+            rec.e48 = (uint16_t) e48;
+            rec.pad = 0;
+            rec.seed = seed;
+            reinterpret_cast<StageRec*>(area)[sidx] = rec;
+        }
+        return;
+    }
+
     // ---- encode into the LDS buffer and store
-    {
+    const uint64_t hdr = 8ull * g.T + 4ull * (x.s + 1);
+    unsigned long long ex;
+    if (total + 64 <= kOutBuf) {
         const uint32_t nz = (total + 64 + 15) / 16;
         for (uint32_t i = lane; i < nz; i += 64)
             reinterpret_cast<uint4*>(obuf)[i] = make_uint4(0, 0, 0, 0);
+        // (two separate branches rather than if/else: the compiler would hoist
+        // what the two encoders share above an if/else and keep it live)
+        if (has && w && fast)
+            encode_fast(obuf, excl, w, s, s2, acc, wide);
+        if (has && w && !fast)
+            encode_general(obuf, excl, w, nj, s, s2, acc);
+        // the tile's stream offset: after its encoding, so predecessors had longer to publish
+        ex = lookback(status, t, 0, total, 0, ws, true);
+        if (hdr + ex + total <= out_cap)
+            store_wave(obuf, out + hdr + ex, total);
+    } else {
+        // a tile too wide for the buffer (its blocks average over 127 bytes):
+        // half a wave at a time, each half encoded value by value after the
+        // offset is known.  Straight-line: a loop here would keep the
+        // accumulators in scratch memory.
+        ex = lookback(status, t, 0, total, 0, ws, true);
+        const uint32_t mid = __builtin_amdgcn_readlane(excl, 32);  // lane 32's block offset
+        const bool ok = hdr + ex + total <= out_cap;
+        for (uint32_t i = lane; i < (mid + 64 + 15) / 16; i += 64)
+            reinterpret_cast<uint4*>(obuf)[i] = make_uint4(0, 0, 0, 0);
+        if (lane < 32 && has && w)
+            encode_general(obuf, excl, w, nj, s, s2, acc);
+        if (ok)
+            store_wave(obuf, out + hdr + ex, mid);
+        for (uint32_t i = lane; i < (total - mid + 64 + 15) / 16; i += 64)
+            reinterpret_cast<uint4*>(obuf)[i] = make_uint4(0, 0, 0, 0);
+        if (lane >= 32 && has && w)
+            encode_general(obuf, excl - mid, w, nj, s, s2, acc);
+        if (ok)
+            store_wave(obuf, out + hdr + ex + mid, total - mid);
     }
-    // (two separate branches rather than if/else: the compiler would hoist
-    // what the two encoders share above an if/else and keep it live)
-    if (has && w && fast)
-        encode_fast(obuf, excl, w, s, s2, acc, wide);
-    if (has && w && !fast)
-        encode_general(obuf, excl, w, nj, s, s2, acc);
-    // the tile's stream offset: after its encoding, so predecessors had longer to publish
-    const unsigned long long ex = lookback(status, t, 0, total, 0, ws, true);
-    const uint64_t P = 8ull * g.T + 4ull * (x.s + 1) + ex;
+    const uint64_t P = hdr + ex;
     const bool fits = P + total <= out_cap;
     if (!fits && lane == 0)
         raise_error(ws, ZK_ERR_NOSPACE);
-    if (fits)
-        store_wave(obuf, out + P, total);
 
     // ---- output index, seed and chunk header
     IndexView ox = index_view(g, oindex);
@@ -690,13 +781,13 @@ __global__ void __launch_bounds__(64 * kLaneWaves, 4) k_homsum_lane(HomsumArgs a
     if (t >= a.job.tl.ntiles)
         return;
     const ArgIO src{a.job.io};
-    homsum_tile_lane(a.job.g, a.job.tl, a.job.p.k, src, a.job.io.out, a.job.io.out_cap, a.job.io.index,
+    homsum_tile_lane<false>(a.job.g, a.job.tl, a.job.p.k, src, a.job.io.out, a.job.io.out_cap, a.job.io.index,
         a.job.io.size_out, t, status, a.ws, 1, zk_lds + wv * (kOutBuf / 4));
 }
 
 // Uniform batch: shared geometry, per-job pointers in device memory; tiles of
 // the jobs interleave (tile b is tile b / njobs of job b % njobs).
-__global__ void __launch_bounds__(64 * kLaneWaves, 4) k_homsum_lane_batch(BatchArgs b)
+__global__ void __launch_bounds__(64 * kLaneWaves, 5) k_homsum_lane_batch(BatchArgs b)
 {
     uint64_t t0;
     unsigned long long* status;
@@ -718,7 +809,7 @@ __global__ void __launch_bounds__(64 * kLaneWaves, 4) k_homsum_lane_batch(BatchA
     t = __builtin_amdgcn_readfirstlane(t);
     const HomsumIO* io = (const HomsumIO*) b.jobs + jidx;
     const SumIO src{io};
-    homsum_tile_lane(b.g, b.tl, b.p.k, src, ldc(&io->out), ldc(&io->out_cap), ldc(&io->index),
+    homsum_tile_lane<false>(b.g, b.tl, b.p.k, src, ldc(&io->out), ldc(&io->out_cap), ldc(&io->index),
         ldc(&io->size_out), t, status + (uint64_t) jidx * b.tl.ntiles * kStatusStride, b.ws, b.total_chains,
         zk_lds + wv * (kOutBuf / 4));
 }
@@ -750,8 +841,171 @@ __global__ void __launch_bounds__(64 * kLaneWaves, 4) k_homsum_lane_mixed(BatchA
     const Geom g = job->g;
     const Tiling tl = job->tl;
     const SumIO src{&job->io};
-    homsum_tile_lane(g, tl, ldc(&job->p.k), src, ldc(&job->io.out), ldc(&job->io.out_cap), ldc(&job->io.index),
+    homsum_tile_lane<false>(g, tl, ldc(&job->p.k), src, ldc(&job->io.out), ldc(&job->io.out_cap), ldc(&job->io.index),
         ldc(&job->io.size_out), t, status + tile0 * kStatusStride, b.ws, b.total_chains, zk_lds + wv * (kOutBuf / 4));
+}
+
+// ---------------------------------------------------------------- two-pass sum
+
+// Pass one of a single sum: tile t of workgroup-wave t / kLaneWaves, slot t.
+__global__ void __launch_bounds__(64 * kLaneWaves, 4) k_homsum_lane_staged(HomsumArgs a, uint8_t* stage)
+{
+    const uint32_t wv = threadIdx.x >> 6;
+    const uint64_t t = (uint64_t) blockIdx.x * kLaneWaves + wv;
+    if (t >= a.job.tl.ntiles)
+        return;
+    const ArgIO src{a.job.io};
+    homsum_tile_lane<true>(a.job.g, a.job.tl, a.job.p.k, src, a.job.io.out, a.job.io.out_cap, a.job.io.index,
+        a.job.io.size_out, t, nullptr, a.ws, 1, zk_lds + wv * (kOutBuf / 4), stage, a.job.tl.ntiles, t);
+}
+
+// Pass one of a uniform batch: launch tile b is tile b / njobs of job
+// b % njobs (jobs side by side), slot job * ntiles + tile.
+__global__ void __launch_bounds__(64 * kLaneWaves, 5) k_homsum_lane_batch_staged(BatchArgs b, uint8_t* stage)
+{
+    const uint32_t wv = threadIdx.x >> 6;
+    const uint64_t tg = (uint64_t) blockIdx.x * kLaneWaves + wv;
+    if (tg >= b.total_tiles)
+        return;
+    const uint32_t jidx = __builtin_amdgcn_readfirstlane((uint32_t) (tg % b.njobs));
+    const uint32_t t = __builtin_amdgcn_readfirstlane((uint32_t) (tg / b.njobs));
+    const HomsumIO* io = (const HomsumIO*) b.jobs + jidx;
+    const SumIO src{io};
+    homsum_tile_lane<true>(b.g, b.tl, b.p.k, src, ldc(&io->out), ldc(&io->out_cap), ldc(&io->index),
+        ldc(&io->size_out), t, nullptr, b.ws, 1, zk_lds + wv * (kOutBuf / 4), stage, b.total_tiles,
+        (uint64_t) jidx * b.tl.ntiles + t);
+}
+
+// Copy n bytes from src (16-byte aligned, readable up to src + n rounded up
+// to 4) to dst, one wave: 16-byte stores on dst's 16-byte grid, the partial
+// first and last chunk byte by byte, so no byte outside [dst, dst + n) is
+// written.  The source is read through a range-checked view.
+__device__ __forceinline__ void copy_wave(uint8_t* dst, const uint8_t* src, uint32_t n)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    const View v = make_view(src, kStageSlot);
+    const uintptr_t d0 = (uintptr_t) dst, d1 = d0 + n;
+    const uintptr_t c0 = d0 & ~(uintptr_t) 15;
+    const uint32_t nch = n ? (uint32_t) ((d1 - c0 + 15) >> 4) : 0u;
+    for (uint32_t c = lane; c < nch; c += 64) {
+        const uintptr_t ca = c0 + 16ull * c;
+        if (ca >= d0 && ca + 16 <= d1) {
+            *(uint4*) ca = bl16(v, (uint32_t) (ca - d0));
+        } else {
+            const uintptr_t s = ca > d0 ? ca : d0, e = ca + 16 < d1 ? ca + 16 : d1;
+            for (uintptr_t p = s; p < e; p++)
+                *(uint8_t*) p = src[p - d0];
+        }
+    }
+}
+
+// One place job: the sum's geometry and output, its lookback chain over
+// groups, and the stage slots of its tiles.
+struct PlaceJob {
+    const Geom* g;
+    const Tiling* tl;
+    uint8_t* out;
+    uint64_t out_cap;
+    uint8_t* oindex;
+    unsigned long long* size_out;
+    unsigned long long* status;  // the sum's group chain
+    uint32_t gi;                 // group index in the sum
+    uint64_t sidx0;              // slot of the sum's tile 0
+};
+
+// Pass two, one workgroup of four waves per group of kPlaceTiles tiles of
+// one sum: wave 0 scans the group's tile sizes, finds the group's offset by
+// decoupled lookback over the sum's groups (every size is known when the
+// group starts, so the wait is short) and each wave then places every
+// fourth tile: its bytes, output index group offsets, and for a segment's
+// first tile the seed and chunk header, for the sum's last tile the size.
+__device__ __forceinline__ void place_group(const PlaceJob& J, const uint8_t* stage, uint64_t R, WsHeader* ws,
+    uint32_t nchains)
+{
+    __shared__ unsigned long long s_pos[kPlaceTiles];
+    const uint32_t wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const Geom& g = *J.g;
+    const Tiling& tl = *J.tl;
+    const uint32_t gpj = (uint32_t) place_groups(tl.ntiles);
+    const uint32_t tb = J.gi * kPlaceTiles;
+    const uint32_t nt = min(kPlaceTiles, tl.ntiles - tb);
+    const StageRec* recs = reinterpret_cast<const StageRec*>(stage);
+    const uint8_t* slots = stage + sizeof(StageRec) * R;
+    if (wv == 0) {
+        const uint32_t tot = lane < nt ? recs[J.sidx0 + tb + lane].total : 0u;
+        const uint32_t incl = wave_scan_dpp(tot);
+        const unsigned long long agg = (uint32_t) __builtin_amdgcn_readlane(incl, 63);
+        const unsigned long long ex = lookback(J.status, J.gi, 0, agg, 0, ws);
+        if (lane == 0 && J.gi == gpj - 1)
+            chain_finished(ws, nchains);
+        if (lane < nt) {
+            const Tile x = tile_of(g, tl, tb + lane);
+            s_pos[lane] = 8ull * g.T + 4ull * (x.s + 1) + ex + (incl - tot);
+        }
+    }
+    __syncthreads();
+    IndexView ox = index_view(g, J.oindex);
+    for (uint32_t i = wv; i < nt; i += kLaneWaves) {
+        const uint32_t t = tb + i;
+        const StageRec rec = recs[J.sidx0 + t];
+        const uint64_t P = s_pos[i];
+        const Tile x = tile_of(g, tl, t);
+        const bool fits = P + rec.total <= J.out_cap;
+        if (!fits && lane == 0)
+            raise_error(ws, ZK_ERR_NOSPACE);
+        if (fits)
+            copy_wave(J.out + P, slots + kStageSlot * (J.sidx0 + t), rec.total);
+        if (lane < x.ngrp) {
+            const uint32_t e = lane == 0 ? 0u : lane == 1 ? rec.e16 : lane == 2 ? rec.e32 : rec.e48;
+            ox.group_off[x.grp0 + lane] = P + e;
+        }
+        if (lane == 0) {
+            if (x.lt == 0) {
+                const uint64_t so = P - 4;
+                if (so + 4 <= J.out_cap) {
+                    store_bytes(J.out + so, &rec.seed, 4);
+                    if (x.s < g.T) {
+                        const unsigned long long h = so - 8ull * g.T;
+                        store_bytes(J.out + 8ull * x.s, &h, 8);
+                    }
+                }
+                ox.seed_off[x.s] = so;
+            }
+            if (t == tl.ntiles - 1) {
+                // the last tile ends the stream: if it fits, every tile fits
+                const unsigned long long sz = fits ? P + rec.total : 0ull;
+                ox.meta[0] = sz;
+                if (J.size_out)
+                    *J.size_out = sz;
+            }
+        }
+    }
+}
+
+__global__ void __launch_bounds__(64 * kLaneWaves) k_place_lane(HomsumArgs a, const uint8_t* stage)
+{
+    uint64_t gg;
+    unsigned long long* status;
+    if (!launch_status(a.ws, place_groups(a.job.tl.ntiles), gg, status, 1))
+        return;
+    const PlaceJob J{&a.job.g, &a.job.tl, a.job.io.out, a.job.io.out_cap, a.job.io.index, a.job.io.size_out, status,
+        (uint32_t) gg, 0};
+    place_group(J, stage, a.job.tl.ntiles, a.ws, 1);
+}
+
+// Uniform batch: group ticket b is group b / njobs of job b % njobs.
+__global__ void __launch_bounds__(64 * kLaneWaves) k_place_lane_batch(BatchArgs b, const uint8_t* stage)
+{
+    const uint64_t gpj = place_groups(b.tl.ntiles);
+    uint64_t gg;
+    unsigned long long* status;
+    if (!launch_status(b.ws, gpj * b.njobs, gg, status, 1))
+        return;
+    const uint32_t jidx = (uint32_t) (gg % b.njobs);
+    const HomsumIO* io = (const HomsumIO*) b.jobs + jidx;
+    const PlaceJob J{&b.g, &b.tl, ldc(&io->out), ldc(&io->out_cap), ldc(&io->index), ldc(&io->size_out),
+        status + jidx * gpj * kStatusStride, (uint32_t) (gg / b.njobs), (uint64_t) jidx * b.tl.ntiles};
+    place_group(J, stage, b.total_tiles, b.ws, b.njobs);
 }
 
 size_t lane_lds_bytes() { return (size_t) kLaneWaves * kOutBuf; }
@@ -770,6 +1024,33 @@ hipError_t launch_homsum_lane_batch(const BatchArgs& b, hipStream_t s)
         hipLaunchKernelGGL(k_homsum_lane_batch, dim3(groups), dim3(64 * kLaneWaves), lane_lds_bytes(), s, b);
     else
         hipLaunchKernelGGL(k_homsum_lane_mixed, dim3(groups), dim3(64 * kLaneWaves), lane_lds_bytes(), s, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_homsum_lane_staged(const HomsumArgs& a, uint8_t* stage, hipStream_t s)
+{
+    const uint64_t tiles = a.job.tl.ntiles;
+    hipLaunchKernelGGL(k_homsum_lane_staged, dim3((uint32_t) ceil_div(tiles, kLaneWaves)), dim3(64 * kLaneWaves),
+        lane_lds_bytes(), s, a, stage);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        return e;
+    hipLaunchKernelGGL(k_place_lane, dim3((uint32_t) place_groups(tiles)), dim3(64 * kLaneWaves), 0, s, a,
+        (const uint8_t*) stage);
+    return hipGetLastError();
+}
+
+hipError_t launch_homsum_lane_batch_staged(const BatchArgs& b, uint8_t* stage, hipStream_t s)
+{
+    if (!b.uniform)
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_homsum_lane_batch_staged, dim3((uint32_t) ceil_div(b.total_tiles, kLaneWaves)),
+        dim3(64 * kLaneWaves), lane_lds_bytes(), s, b, stage);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        return e;
+    hipLaunchKernelGGL(k_place_lane_batch, dim3((uint32_t) (place_groups(b.tl.ntiles) * b.njobs)),
+        dim3(64 * kLaneWaves), 0, s, b, (const uint8_t*) stage);
     return hipGetLastError();
 }
 
