@@ -164,9 +164,11 @@ hipError_t launch_homsum_lane(const HomsumArgs& a, hipStream_t s);
 hipError_t launch_homsum_lane_batch(const BatchArgs& b, hipStream_t s);
 
 // Two-pass lane sum.  Pass one sums and encodes every tile into a slot of a
-// stage area with no wait on other tiles; pass two scans the tile sizes and
-// places the bytes (with the index entries that depend on stream offsets).
-// A stage area for R tiles holds R records, then R slots.
+// stage area with no wait on other tiles; pass two scans the tile sizes (one
+// workgroup per sum) and places the bytes (one wave per tile, with the index
+// entries that depend on stream offsets).
+// A stage area for R tiles holds R stream offsets (u64), R records, then R
+// slots.
 struct StageRec {
     uint32_t total;          // encoded bytes of the tile
     uint16_t e16, e32, e48;  // offsets of the tile's blocks 16, 32 and 48 inside the tile
@@ -175,14 +177,10 @@ struct StageRec {
 };
 static_assert(sizeof(StageRec) == 16, "stage records are 16 bytes");
 constexpr uint64_t kStageSlot = 9600;  // 64 blocks of at most 150 bytes
-constexpr uint32_t kPlaceTiles = 64;   // tiles per workgroup of pass two
 
-ZK_HD uint64_t stage_area_bytes(uint64_t tiles) { return tiles * (sizeof(StageRec) + kStageSlot); }
-// Lookback status entries pass two uses for a sum of `tiles` tiles.
-ZK_HD uint64_t place_groups(uint64_t tiles) { return ceil_div(tiles, kPlaceTiles); }
+ZK_HD uint64_t stage_area_bytes(uint64_t tiles) { return tiles * (8 + sizeof(StageRec) + kStageSlot); }
 
-// Single sum (a.job) and uniform batch; the workspace serves pass two's
-// lookback over groups of kPlaceTiles tiles.
+// Single sum (a.job) and uniform batch.
 hipError_t launch_homsum_lane_staged(const HomsumArgs& a, uint8_t* stage, hipStream_t s);
 hipError_t launch_homsum_lane_batch_staged(const BatchArgs& b, uint8_t* stage, hipStream_t s);
 hipError_t launch_case_count(const uint8_t* w1, const uint8_t* w2, uint64_t nblocks,

@@ -652,7 +652,7 @@ __device__ __forceinline__ void homsum_tile_lane(const Geom& g, const Tiling& tl
     const bool wide = __any(has && w >= 8);
 
     if constexpr (Staged) {
-        uint8_t* slot = area + sizeof(StageRec) * R + kStageSlot * sidx;
+        uint8_t* slot = area + (8 + sizeof(StageRec)) * R + kStageSlot * sidx;
         if (total + 64 <= kOutBuf) {
             const uint32_t nz = (total + 64 + 15) / 16;
             for (uint32_t i = lane; i < nz; i += 64)
@@ -690,7 +690,7 @@ __device__ __forceinline__ void homsum_tile_lane(const Geom& g, const Tiling& tl
             rec.e48 = (uint16_t) e48;
             rec.pad = 0;
             rec.seed = seed;
-            reinterpret_cast<StageRec*>(area)[sidx] = rec;
+            reinterpret_cast<StageRec*>(area + 8 * R)[sidx] = rec;
         }
         return;
     }
@@ -876,10 +876,10 @@ __global__ void __launch_bounds__(64 * kLaneWaves, 5) k_homsum_lane_batch_staged
         (uint64_t) jidx * b.tl.ntiles + t);
 }
 
-// Copy n bytes from src (16-byte aligned, readable up to src + n rounded up
-// to 4) to dst, one wave: 16-byte stores on dst's 16-byte grid, the partial
-// first and last chunk byte by byte, so no byte outside [dst, dst + n) is
-// written.  The source is read through a range-checked view.
+// Copy n bytes from src (16-byte aligned, a stage slot) to dst, one wave:
+// 16-byte stores on dst's 16-byte grid, the partial first and last chunk
+// byte by byte, so no byte outside [dst, dst + n) is written.  The source is
+// read through a range-checked view, four chunks per lane in flight.
 __device__ __forceinline__ void copy_wave(uint8_t* dst, const uint8_t* src, uint32_t n)
 {
     const uint32_t lane = threadIdx.x & 63;
@@ -887,125 +887,160 @@ __device__ __forceinline__ void copy_wave(uint8_t* dst, const uint8_t* src, uint
     const uintptr_t d0 = (uintptr_t) dst, d1 = d0 + n;
     const uintptr_t c0 = d0 & ~(uintptr_t) 15;
     const uint32_t nch = n ? (uint32_t) ((d1 - c0 + 15) >> 4) : 0u;
-    for (uint32_t c = lane; c < nch; c += 64) {
-        const uintptr_t ca = c0 + 16ull * c;
-        if (ca >= d0 && ca + 16 <= d1) {
-            *(uint4*) ca = bl16(v, (uint32_t) (ca - d0));
-        } else {
-            const uintptr_t s = ca > d0 ? ca : d0, e = ca + 16 < d1 ? ca + 16 : d1;
-            for (uintptr_t p = s; p < e; p++)
-                *(uint8_t*) p = src[p - d0];
+    for (uint32_t base = 0; base < nch; base += 256) {
+        uint4 x[4];
+#pragma unroll
+        for (uint32_t u = 0; u < 4; u++) {
+            const uintptr_t ca = c0 + 16ull * (base + 64 * u + lane);
+            x[u] = make_uint4(0, 0, 0, 0);
+            if (ca >= d0 && ca + 16 <= d1)
+                x[u] = bl16(v, (uint32_t) (ca - d0));
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < 4; u++) {
+            const uint32_t c = base + 64 * u + lane;
+            const uintptr_t ca = c0 + 16ull * c;
+            if (c >= nch)
+                continue;
+            if (ca >= d0 && ca + 16 <= d1) {
+                *(uint4*) ca = x[u];
+            } else {
+                const uintptr_t s = ca > d0 ? ca : d0, e = ca + 16 < d1 ? ca + 16 : d1;
+                for (uintptr_t p = s; p < e; p++)
+                    *(uint8_t*) p = src[p - d0];
+            }
         }
     }
 }
 
-// One place job: the sum's geometry and output, its lookback chain over
-// groups, and the stage slots of its tiles.
-struct PlaceJob {
-    const Geom* g;
-    const Tiling* tl;
-    uint8_t* out;
-    uint64_t out_cap;
-    uint8_t* oindex;
-    unsigned long long* size_out;
-    unsigned long long* status;  // the sum's group chain
-    uint32_t gi;                 // group index in the sum
-    uint64_t sidx0;              // slot of the sum's tile 0
-};
+// Stream offset of every tile of one sum (pass two, first kernel): one
+// workgroup of 1024 threads per sum; thread i adds the sizes of tiles
+// [8i, 8i + 8) of each round of 8192 tiles, then a workgroup scan.  A tile's
+// offset counts the stream header, one seed per segment started and the
+// sizes of all earlier tiles.
+constexpr uint32_t kScanThreads = 1024;
+constexpr uint32_t kScanPer = 8;
 
-// Pass two, one workgroup of four waves per group of kPlaceTiles tiles of
-// one sum: wave 0 scans the group's tile sizes, finds the group's offset by
-// decoupled lookback over the sum's groups (every size is known when the
-// group starts, so the wait is short) and each wave then places every
-// fourth tile: its bytes, output index group offsets, and for a segment's
-// first tile the seed and chunk header, for the sum's last tile the size.
-__device__ __forceinline__ void place_group(const PlaceJob& J, const uint8_t* stage, uint64_t R, WsHeader* ws,
-    uint32_t nchains)
+__device__ __forceinline__ void scan_sum(const Geom& g, const Tiling& tl, const uint8_t* stage, uint64_t R,
+    uint64_t sidx0)
 {
-    __shared__ unsigned long long s_pos[kPlaceTiles];
-    const uint32_t wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const Geom& g = *J.g;
-    const Tiling& tl = *J.tl;
-    const uint32_t gpj = (uint32_t) place_groups(tl.ntiles);
-    const uint32_t tb = J.gi * kPlaceTiles;
-    const uint32_t nt = min(kPlaceTiles, tl.ntiles - tb);
-    const StageRec* recs = reinterpret_cast<const StageRec*>(stage);
-    const uint8_t* slots = stage + sizeof(StageRec) * R;
-    if (wv == 0) {
-        const uint32_t tot = lane < nt ? recs[J.sidx0 + tb + lane].total : 0u;
-        const uint32_t incl = wave_scan_dpp(tot);
-        const unsigned long long agg = (uint32_t) __builtin_amdgcn_readlane(incl, 63);
-        const unsigned long long ex = lookback(J.status, J.gi, 0, agg, 0, ws);
-        if (lane == 0 && J.gi == gpj - 1)
-            chain_finished(ws, nchains);
-        if (lane < nt) {
-            const Tile x = tile_of(g, tl, tb + lane);
-            s_pos[lane] = 8ull * g.T + 4ull * (x.s + 1) + ex + (incl - tot);
+    __shared__ unsigned long long s_w[kScanThreads / 64];
+    __shared__ unsigned long long s_run;
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    unsigned long long* pos = (unsigned long long*) stage;
+    const StageRec* recs = reinterpret_cast<const StageRec*>(stage + 8 * R);
+    if (tid == 0)
+        s_run = 0;
+    for (uint32_t base = 0; base < tl.ntiles; base += kScanThreads * kScanPer) {
+        uint32_t tot[kScanPer];
+        uint64_t sum = 0;
+#pragma unroll
+        for (uint32_t u = 0; u < kScanPer; u++) {
+            const uint32_t t = base + tid * kScanPer + u;
+            tot[u] = t < tl.ntiles ? recs[sidx0 + t].total : 0u;
         }
+#pragma unroll
+        for (uint32_t u = 0; u < kScanPer; u++)
+            sum += tot[u];
+        // workgroup exclusive scan of the per-thread sums (values < 2^32 per
+        // wave: 64 x 8 tiles of at most 9600 bytes)
+        const uint32_t incl = wave_scan_dpp((uint32_t) sum);
+        if (lane == 63)
+            s_w[wv] = incl;
+        __syncthreads();
+        unsigned long long before = s_run;
+        for (uint32_t i = 0; i < wv; i++)
+            before += s_w[i];
+        uint64_t at = before + (incl - (uint32_t) sum);
+#pragma unroll
+        for (uint32_t u = 0; u < kScanPer; u++) {
+            const uint32_t t = base + tid * kScanPer + u;
+            if (t < tl.ntiles) {
+                const Tile x = tile_of(g, tl, t);
+                pos[sidx0 + t] = 8ull * g.T + 4ull * (x.s + 1) + at;
+            }
+            at += tot[u];
+        }
+        __syncthreads();
+        if (tid == kScanThreads - 1)
+            s_run = at;
+        __syncthreads();
     }
-    __syncthreads();
-    IndexView ox = index_view(g, J.oindex);
-    for (uint32_t i = wv; i < nt; i += kLaneWaves) {
-        const uint32_t t = tb + i;
-        const StageRec rec = recs[J.sidx0 + t];
-        const uint64_t P = s_pos[i];
-        const Tile x = tile_of(g, tl, t);
-        const bool fits = P + rec.total <= J.out_cap;
-        if (!fits && lane == 0)
-            raise_error(ws, ZK_ERR_NOSPACE);
-        if (fits)
-            copy_wave(J.out + P, slots + kStageSlot * (J.sidx0 + t), rec.total);
-        if (lane < x.ngrp) {
-            const uint32_t e = lane == 0 ? 0u : lane == 1 ? rec.e16 : lane == 2 ? rec.e32 : rec.e48;
-            ox.group_off[x.grp0 + lane] = P + e;
-        }
-        if (lane == 0) {
-            if (x.lt == 0) {
-                const uint64_t so = P - 4;
-                if (so + 4 <= J.out_cap) {
-                    store_bytes(J.out + so, &rec.seed, 4);
-                    if (x.s < g.T) {
-                        const unsigned long long h = so - 8ull * g.T;
-                        store_bytes(J.out + 8ull * x.s, &h, 8);
-                    }
+}
+
+// One tile's placement (pass two, second kernel), one wave: its bytes, the
+// output index group offsets, for a segment's first tile the seed and chunk
+// header, for the sum's last tile the stream size.
+__device__ __forceinline__ void place_tile(const Geom& g, const Tiling& tl, uint32_t t, uint8_t* out,
+    uint64_t out_cap, uint8_t* oindex, unsigned long long* size_out, const uint8_t* stage, uint64_t R,
+    uint64_t sidx, WsHeader* ws)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    const StageRec rec = reinterpret_cast<const StageRec*>(stage + 8 * R)[sidx];
+    const uint64_t P = ((const unsigned long long*) stage)[sidx];
+    const Tile x = tile_of(g, tl, t);
+    const bool fits = P + rec.total <= out_cap;
+    if (!fits && lane == 0)
+        raise_error(ws, ZK_ERR_NOSPACE);
+    if (fits)
+        copy_wave(out + P, stage + (8 + sizeof(StageRec)) * R + kStageSlot * sidx, rec.total);
+    IndexView ox = index_view(g, oindex);
+    if (lane < x.ngrp) {
+        const uint32_t e = lane == 0 ? 0u : lane == 1 ? rec.e16 : lane == 2 ? rec.e32 : rec.e48;
+        ox.group_off[x.grp0 + lane] = P + e;
+    }
+    if (lane == 0) {
+        if (x.lt == 0) {
+            const uint64_t so = P - 4;
+            if (so + 4 <= out_cap) {
+                store_bytes(out + so, &rec.seed, 4);
+                if (x.s < g.T) {
+                    const unsigned long long h = so - 8ull * g.T;
+                    store_bytes(out + 8ull * x.s, &h, 8);
                 }
-                ox.seed_off[x.s] = so;
             }
-            if (t == tl.ntiles - 1) {
-                // the last tile ends the stream: if it fits, every tile fits
-                const unsigned long long sz = fits ? P + rec.total : 0ull;
-                ox.meta[0] = sz;
-                if (J.size_out)
-                    *J.size_out = sz;
-            }
+            ox.seed_off[x.s] = so;
+        }
+        if (t == tl.ntiles - 1) {
+            // the last tile ends the stream: if it fits, every tile fits
+            const unsigned long long sz = fits ? P + rec.total : 0ull;
+            ox.meta[0] = sz;
+            if (size_out)
+                *size_out = sz;
         }
     }
+}
+
+__global__ void __launch_bounds__(kScanThreads) k_scan_lane(HomsumArgs a, const uint8_t* stage)
+{
+    scan_sum(a.job.g, a.job.tl, stage, a.job.tl.ntiles, 0);
+}
+
+__global__ void __launch_bounds__(kScanThreads) k_scan_lane_batch(BatchArgs b, const uint8_t* stage)
+{
+    scan_sum(b.g, b.tl, stage, b.total_tiles, (uint64_t) blockIdx.x * b.tl.ntiles);
 }
 
 __global__ void __launch_bounds__(64 * kLaneWaves) k_place_lane(HomsumArgs a, const uint8_t* stage)
 {
-    uint64_t gg;
-    unsigned long long* status;
-    if (!launch_status(a.ws, place_groups(a.job.tl.ntiles), gg, status, 1))
+    const uint64_t t = (uint64_t) blockIdx.x * kLaneWaves + (threadIdx.x >> 6);
+    if (t >= a.job.tl.ntiles)
         return;
-    const PlaceJob J{&a.job.g, &a.job.tl, a.job.io.out, a.job.io.out_cap, a.job.io.index, a.job.io.size_out, status,
-        (uint32_t) gg, 0};
-    place_group(J, stage, a.job.tl.ntiles, a.ws, 1);
+    place_tile(a.job.g, a.job.tl, (uint32_t) t, a.job.io.out, a.job.io.out_cap, a.job.io.index, a.job.io.size_out,
+        stage, a.job.tl.ntiles, t, a.ws);
 }
 
-// Uniform batch: group ticket b is group b / njobs of job b % njobs.
+// Uniform batch: launch tile b is tile b / njobs of job b % njobs.
 __global__ void __launch_bounds__(64 * kLaneWaves) k_place_lane_batch(BatchArgs b, const uint8_t* stage)
 {
-    const uint64_t gpj = place_groups(b.tl.ntiles);
-    uint64_t gg;
-    unsigned long long* status;
-    if (!launch_status(b.ws, gpj * b.njobs, gg, status, 1))
+    const uint64_t tg = (uint64_t) blockIdx.x * kLaneWaves + (threadIdx.x >> 6);
+    if (tg >= b.total_tiles)
         return;
-    const uint32_t jidx = (uint32_t) (gg % b.njobs);
+    const uint32_t jidx = __builtin_amdgcn_readfirstlane((uint32_t) (tg % b.njobs));
+    const uint32_t t = __builtin_amdgcn_readfirstlane((uint32_t) (tg / b.njobs));
     const HomsumIO* io = (const HomsumIO*) b.jobs + jidx;
-    const PlaceJob J{&b.g, &b.tl, ldc(&io->out), ldc(&io->out_cap), ldc(&io->index), ldc(&io->size_out),
-        status + jidx * gpj * kStatusStride, (uint32_t) (gg / b.njobs), (uint64_t) jidx * b.tl.ntiles};
-    place_group(J, stage, b.total_tiles, b.ws, b.njobs);
+    place_tile(b.g, b.tl, t, ldc(&io->out), ldc(&io->out_cap), ldc(&io->index), ldc(&io->size_out), stage,
+        b.total_tiles, (uint64_t) jidx * b.tl.ntiles + t, b.ws);
 }
 
 size_t lane_lds_bytes() { return (size_t) kLaneWaves * kOutBuf; }
@@ -1035,7 +1070,8 @@ hipError_t launch_homsum_lane_staged(const HomsumArgs& a, uint8_t* stage, hipStr
     hipError_t e = hipGetLastError();
     if (e != hipSuccess)
         return e;
-    hipLaunchKernelGGL(k_place_lane, dim3((uint32_t) place_groups(tiles)), dim3(64 * kLaneWaves), 0, s, a,
+    hipLaunchKernelGGL(k_scan_lane, dim3(1), dim3(kScanThreads), 0, s, a, (const uint8_t*) stage);
+    hipLaunchKernelGGL(k_place_lane, dim3((uint32_t) ceil_div(tiles, kLaneWaves)), dim3(64 * kLaneWaves), 0, s, a,
         (const uint8_t*) stage);
     return hipGetLastError();
 }
@@ -1049,7 +1085,8 @@ hipError_t launch_homsum_lane_batch_staged(const BatchArgs& b, uint8_t* stage, h
     hipError_t e = hipGetLastError();
     if (e != hipSuccess)
         return e;
-    hipLaunchKernelGGL(k_place_lane_batch, dim3((uint32_t) (place_groups(b.tl.ntiles) * b.njobs)),
+    hipLaunchKernelGGL(k_scan_lane_batch, dim3(b.njobs), dim3(kScanThreads), 0, s, b, (const uint8_t*) stage);
+    hipLaunchKernelGGL(k_place_lane_batch, dim3((uint32_t) ceil_div(b.total_tiles, kLaneWaves)),
         dim3(64 * kLaneWaves), 0, s, b, (const uint8_t*) stage);
     return hipGetLastError();
 }
