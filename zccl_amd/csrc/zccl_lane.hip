@@ -155,8 +155,11 @@ __device__ __forceinline__ void decode_fast(const Block& q, uint32_t (&acc)[L])
     // so the 2-bit field there reads -1 (sign set) or +1 as a signed number.
     // so0/so1 hold the values whose bit is odd in s0/s1, se0/se1 (shifted up
     // one) the values whose bit is even.
-    const uint32_t so0 = (s0 & 0xAAAAAAAAu) | 0x55555555u, se0 = ((s0 << 1) & 0xAAAAAAAAu) | 0x55555555u;
-    const uint32_t so1 = (s1 & 0xAAu) | 0x55u, se1 = ((s1 << 1) & 0xAAu) | 0x55u;
+    uint32_t so0 = (s0 & 0xAAAAAAAAu) | 0x55555555u, se0 = ((s0 << 1) & 0xAAAAAAAAu) | 0x55555555u;
+    uint32_t so1 = (s1 & 0xAAu) | 0x55u, se1 = ((s1 << 1) & 0xAAu) | 0x55u;
+    // (opaque to the optimiser, which would otherwise extract the sign bit
+    // alone and OR in the one: two instructions per value instead of one)
+    asm volatile("" : "+v"(so0), "+v"(se0), "+v"(so1), "+v"(se1));
 #pragma unroll
     for (uint32_t p = 0; p < NGRP; p++) {
         // the group's low bits big-endian: value 8p+c's field is r bits from bit 64 - (c+1) r
@@ -167,8 +170,8 @@ __device__ __forceinline__ void decode_fast(const Block& q, uint32_t (&acc)[L])
             const uint32_t j = 8 * p + c;
             if (j >= L)
                 break;
-            if (c == 4)
-                gh = (uint32_t) (((((uint64_t) e0 << 32) | e1) << (4 * r)) >> 32);
+            if (c == 4)  // the window of value 4: bits 4r.. of e0:e1 (any value when r = 0)
+                gh = __builtin_amdgcn_alignbit(e0, e1, 32 - 4 * r);
             const uint32_t hbyte = wide ? __builtin_amdgcn_ubfe(q.h[(6 + j) >> 2], 8 * ((6 + j) & 3), hb) : 0u;
             // {high byte, window} << r: the magnitude (< 2^15) in the upper
             // dword, the next value's window in the lower one
