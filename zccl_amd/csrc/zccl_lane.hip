@@ -59,7 +59,25 @@ static_assert(L % 4 == 0 && L > 32 && L <= 40, "the lane decoder is laid out for
 
 // Encoded size of a whole block at width w (w = 0..32): 1 + SB + 36 B +
 // ceil(36 r / 8) = 6 + ceil(9 w / 2).
-__device__ __forceinline__ uint32_t full_block_bytes(uint32_t w) { return w ? 6u + ((9u * w + 1u) >> 1) : 1u; }
+// a * b + c for a, b below 2^24: one full-rate instruction (the compiler
+// picks a quarter-rate 32-bit multiply where it cannot prove the range).
+__device__ __forceinline__ uint32_t mad_u24(uint32_t a, uint32_t b, uint32_t c)
+{
+    uint32_t d;
+    asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "v"(c));
+    return d;
+}
+
+__device__ __forceinline__ uint32_t full_block_bytes(uint32_t w) { return w ? 6u + (mad_u24(w, 9u, 1u) >> 1) : 1u; }
+
+// block_bytes for n <= L values at width w <= 32.
+__device__ __forceinline__ uint32_t lane_block_bytes(uint32_t w, uint32_t n)
+{
+    return w ? 1u + ((n + 7u) >> 3) + mad_u24(w >> 3, n, (mad_u24(w & 7u, n, 7u) >> 3)) : 1u;
+}
+
+// (w >> 3) * L, the high bytes of a block, for w <= 32.
+__device__ __forceinline__ uint32_t high_bytes(uint32_t w) { return mad_u24(w >> 3, L, 0u); }
 
 using View = __amdgpu_buffer_rsrc_t;
 
@@ -454,7 +472,7 @@ __device__ __forceinline__ void homsum_tile_lane(const Geom& g, const Tiling& tl
             bad_width = true;
             c.w = 0;
         }
-        c.bs = has ? (nj == L ? full_block_bytes(c.w) : block_bytes(c.w, nj)) : 0u;
+        c.bs = has ? (nj == L ? full_block_bytes(c.w) : lane_block_bytes(c.w, nj)) : 0u;
         const uint32_t incl = wave_scan_dpp(c.bs);
         c.tot = __builtin_amdgcn_readlane(incl, 63);
         c.skip = p.goff > p.size || c.tot > p.size - p.goff;
@@ -496,15 +514,15 @@ __device__ __forceinline__ void homsum_tile_lane(const Geom& g, const Tiling& tl
 #pragma unroll
         for (uint32_t i = 0; i < HEADW; i++)
             q.h[i] = __builtin_amdgcn_alignbyte(D[i + 1], D[i], c.o);  // uses o & 3
-        const uint32_t r = c.w & 7, lo = c.o + 1 + SB + (c.w >> 3) * L;
+        const uint32_t r = c.w & 7, lo = c.o + 1 + SB + high_bytes(c.w);
+        uint32_t a = lo;
 #pragma unroll
-        for (uint32_t gp = 0; gp < NGRP; gp++) {
-            const uint32_t a = lo + gp * r;
+        for (uint32_t gp = 0; gp < NGRP; gp++, a += r) {
             const uint32_t* pg = st + (a >> 2);
             const uint32_t E0 = pg[0], E1 = pg[1], E2 = pg[2];
-            const uint32_t sel = 0x00010203u + (a & 3u) * 0x01010101u;  // big-endian dword at byte a & 3
-            q.ge[2 * gp] = __builtin_amdgcn_perm(E1, E0, sel);
-            q.ge[2 * gp + 1] = __builtin_amdgcn_perm(E2, E1, sel);
+            // the big-endian dwords at byte a & 3 (alignbyte uses a & 3)
+            q.ge[2 * gp] = __builtin_bswap32(__builtin_amdgcn_alignbyte(E1, E0, a));
+            q.ge[2 * gp + 1] = __builtin_bswap32(__builtin_amdgcn_alignbyte(E2, E1, a));
         }
     };
     // the same from the view, per lane (tiles too large to stage)
@@ -514,7 +532,7 @@ __device__ __forceinline__ void homsum_tile_lane(const Geom& g, const Tiling& tl
         q.h[0] = h0.x; q.h[1] = h0.y; q.h[2] = h0.z; q.h[3] = h0.w;
         q.h[4] = h1.x; q.h[5] = h1.y; q.h[6] = h1.z; q.h[7] = h1.w;
         q.h[8] = h2.x; q.h[9] = h2.y; q.h[10] = h2.z;
-        const uint32_t r = c.w & 7, lo = c.o + 1 + SB + (c.w >> 3) * L;
+        const uint32_t r = c.w & 7, lo = c.o + 1 + SB + high_bytes(c.w);
 #pragma unroll
         for (uint32_t gp = 0; gp < NGRP; gp++) {
             const uint2 e = bl8(c.v, lo + gp * r);
@@ -645,7 +663,7 @@ __device__ __forceinline__ void homsum_tile_lane(const Geom& g, const Tiling& tl
     }
     __builtin_amdgcn_sched_barrier(0);
     const uint32_t w = has ? dwidth(mx) : 0u;
-    const uint32_t bs = has ? (nj == L ? full_block_bytes(w) : block_bytes(w, nj)) : 0u;
+    const uint32_t bs = has ? (nj == L ? full_block_bytes(w) : lane_block_bytes(w, nj)) : 0u;
     const uint32_t incl = wave_scan_dpp(bs);
     const uint32_t excl = incl - bs;
     const uint32_t total = __builtin_amdgcn_readlane(incl, 63);
@@ -879,10 +897,18 @@ __global__ void __launch_bounds__(64 * kLaneWaves, 5) k_homsum_lane_batch_staged
         (uint64_t) jidx * b.tl.ntiles + t);
 }
 
+// Byte j (0..15) of a 16-byte chunk held in registers.
+__device__ __forceinline__ uint32_t chunk_byte(const uint4& x, uint32_t j)
+{
+    const uint32_t w = j < 8 ? (j < 4 ? x.x : x.y) : (j < 12 ? x.z : x.w);
+    return (w >> (8 * (j & 3))) & 0xFFu;
+}
+
 // Copy n bytes from src (16-byte aligned, a stage slot) to dst, one wave:
 // 16-byte stores on dst's 16-byte grid, the partial first and last chunk
-// byte by byte, so no byte outside [dst, dst + n) is written.  The source is
-// read through a range-checked view, four chunks per lane in flight.
+// byte by byte from registers, so no byte outside [dst, dst + n) is
+// written.  The source is read through a range-checked view, four chunks
+// per lane in flight; a partial first chunk reads the slot's first 16 bytes.
 __device__ __forceinline__ void copy_wave(uint8_t* dst, const uint8_t* src, uint32_t n)
 {
     const uint32_t lane = threadIdx.x & 63;
@@ -894,10 +920,9 @@ __device__ __forceinline__ void copy_wave(uint8_t* dst, const uint8_t* src, uint
         uint4 x[4];
 #pragma unroll
         for (uint32_t u = 0; u < 4; u++) {
-            const uintptr_t ca = c0 + 16ull * (base + 64 * u + lane);
-            x[u] = make_uint4(0, 0, 0, 0);
-            if (ca >= d0 && ca + 16 <= d1)
-                x[u] = bl16(v, (uint32_t) (ca - d0));
+            const uint32_t c = base + 64 * u + lane;
+            const uintptr_t ca = c0 + 16ull * c;
+            x[u] = c < nch ? bl16(v, ca >= d0 ? (uint32_t) (ca - d0) : 0u) : make_uint4(0, 0, 0, 0);
         }
 #pragma unroll
         for (uint32_t u = 0; u < 4; u++) {
@@ -908,9 +933,10 @@ __device__ __forceinline__ void copy_wave(uint8_t* dst, const uint8_t* src, uint
             if (ca >= d0 && ca + 16 <= d1) {
                 *(uint4*) ca = x[u];
             } else {
-                const uintptr_t s = ca > d0 ? ca : d0, e = ca + 16 < d1 ? ca + 16 : d1;
-                for (uintptr_t p = s; p < e; p++)
-                    *(uint8_t*) p = src[p - d0];
+                // x[u] holds source bytes from destination byte sb on
+                const uintptr_t sb = ca > d0 ? ca : d0, e = ca + 16 < d1 ? ca + 16 : d1;
+                for (uintptr_t p = sb; p < e; p++)
+                    *(uint8_t*) p = (uint8_t) chunk_byte(x[u], (uint32_t) (p - sb));
             }
         }
     }
