@@ -237,12 +237,25 @@ def main():
     from zccl_amd._lib import lib as _zl
     _zl().zccl_tune_layout(1, 0 if args.layout == "lane" else 1)
     if args.copy_calibration:
-        # elementwise kernel reading 256 MiB and writing 256 MiB per launch
+        # copies reading 256 MiB and writing 256 MiB per launch: torch's
+        # elementwise kernel, and the library's calibration copies with 16-byte
+        # global loads (mode 0) and 16-byte buffer loads into LDS (mode 1, the
+        # sum kernels' staging path)
+        import ctypes
+
         a = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
         b = torch.empty_like(a)
         for _ in range(3):
             torch.mul(a, 2.0, out=b)
+        s = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+        for mode in (0, 1):
+            for _ in range(3):
+                rc = _zl().zccl_calib_copy(ctypes.c_void_p(a.data_ptr()), ctypes.c_void_p(b.data_ptr()),
+                                           a.numel() * 4, mode, s)
+                if rc:
+                    raise RuntimeError(f"calibration copy failed ({rc})")
         torch.cuda.synchronize()
+        assert torch.equal(a, b)
         print(json.dumps({"copy_read_bytes_per_launch": a.numel() * 4,
                           "copy_write_bytes_per_launch": b.numel() * 4}))
         return
@@ -358,10 +371,11 @@ def main():
     t_doc = timed(lambda: ar.reduce_scatter_doc(xs), 2)
     t_homo_rs = t_comp + ms_per_step / 1e3
 
-    # the uniform batch of 36-value blocks runs the lane-layout sum (or, with
-    # --layout tiles, the slot-layout sum at 256 threads)
+    # the uniform batch of 36-value blocks runs the two-pass lane-layout sum
+    # (three kernels: sum into the stage area, scan of tile sizes, placement)
+    # or, with --layout tiles, the slot-layout sum at 256 threads
     if L == 36 and args.layout == "lane":
-        homsum_kernel = "zk::k_homsum_lane_batch"
+        homsum_kernel = "zk::k_homsum_lane_batch_staged + zk::k_scan_lane_batch + zk::k_place_lane_batch"
     elif L == 36 and _zl().zccl_tune_get_threads(1) == 256:
         homsum_kernel = "zk::k_homsum_batch_slots"
     else:

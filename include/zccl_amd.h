@@ -155,13 +155,20 @@ int zccl_homsum_dev(const unsigned char *const *ins, const void *const *indices,
  * zccl_sum_stage_bytes(zccl_num_tiles(...)) bytes (16-byte aligned) without
  * waiting on other tiles; pass two scans the tile sizes and moves the bytes
  * to their stream offsets.  Same output as zccl_homsum_dev.  Used when
- * blockSize is 36 and the stage area is large enough; otherwise this is
- * zccl_homsum_dev. */
+ * blockSize is 36, the sum has at least 1024 tiles (zccl_num_tiles) and the
+ * stage area is large enough; otherwise this is zccl_homsum_dev. */
 size_t zccl_sum_stage_bytes(size_t max_tiles);
 int zccl_homsum_dev_staged(const unsigned char *const *ins, const void *const *indices, int k,
     size_t nbEle, int blockSize, int nchunks, unsigned char *out, size_t out_cap, void *index,
     unsigned long long *d_size, void *workspace, size_t workspace_bytes, void *stage, size_t stage_bytes,
     void *stream);
+
+/* No counterpart in the reference: a measurement aid.  Copies `bytes` (a
+ * multiple of 16, 16-byte aligned buffers) from src to dst with 16-byte
+ * global loads (mode 0) or 16-byte buffer loads into LDS (mode 1, the load
+ * path of the sum kernels), so rocprofv3's byte counters can be scaled
+ * against a launch of known traffic.  0 on success. */
+int zccl_calib_copy(const void *src, void *dst, size_t bytes, int mode, void *stream);
 
 /* No counterpart in the reference: zccl_homsum_dev for callers that cannot
  * allocate (code under HIP graph capture, tight serving loops).  It takes no
@@ -207,9 +214,9 @@ zccl_batch *zccl_batch_decompress_create(int njobs, const size_t *nbEle, const f
 size_t zccl_batch_tiles(const zccl_batch *batch);
 int zccl_batch_launch(const zccl_batch *batch, void *workspace, size_t workspace_bytes, void *stream);
 /* The two-pass form (see zccl_homsum_dev_staged) of a homomorphic-sum batch
- * whose jobs share one geometry with blockSize 36, given a stage area of
- * zccl_sum_stage_bytes(zccl_batch_tiles()) bytes; any other batch launches as
- * zccl_batch_launch. */
+ * whose jobs share one geometry with blockSize 36 and that has at least 1024
+ * tiles, given a stage area of zccl_sum_stage_bytes(zccl_batch_tiles())
+ * bytes; any other batch launches as zccl_batch_launch. */
 int zccl_batch_launch_staged(const zccl_batch *batch, void *workspace, size_t workspace_bytes, void *stage,
     size_t stage_bytes, void *stream);
 void zccl_batch_destroy(zccl_batch *batch);

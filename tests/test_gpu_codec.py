@@ -120,21 +120,47 @@ def test_random_against_oracle(codec, kind, n, eb, L, T):
     assert np.array_equal(bits(dh.cpu().numpy()), bits(O.decompress(ch, n, eb, L, T)))
 
 
-# the two forms of the blockSize-36 sum: two-pass (stage area) and
-# single-pass (decoupled lookback)
+# the two forms of the blockSize-36 sum: two-pass (stage area; sums of at
+# least 1024 tiles) and single-pass (decoupled lookback)
 FORMS = [None, False]
 FORM_IDS = ["two_pass", "lookback"]
+N_TWO_PASS = 2_400_017  # 1042 tiles at L = 36, T = 16
+
+
+def form_n(codec, stage, small, L=36, T=16):
+    """Element count for a test of `stage`'s form: large enough for the
+    two-pass form to run (asserted), `small` for the single-pass one."""
+    if stage is False:
+        return small
+    assert codec.num_tiles(N_TWO_PASS, L, T) >= codec.STAGED_MIN_TILES
+    return N_TWO_PASS
+
+
+def sum_form(codec, ss, stage, out=None):
+    """codec.homsum in `stage`'s form.  The two-pass form gets a stage area
+    filled with 0xFF beforehand, and the sum must have written the first
+    tile's stream offset into it (so the two-pass kernels ran)."""
+    n, L, T = ss[0].n, ss[0].L, ss[0].T
+    if stage is False or L != codec.STAGED_L:
+        return codec.homsum(ss, out=out, stage=stage)
+    st = codec.StageArea(codec.num_tiles(n, L, T))
+    st.buf.fill_(0xFF)
+    h = codec.homsum(ss, out=out, stage=st)
+    torch.cuda.synchronize()
+    assert int(st.buf[:8].view(torch.int64).item()) == 8 * T + 4, "two-pass form did not run"
+    return h
 
 
 @pytest.mark.parametrize("stage", FORMS, ids=FORM_IDS)
 @pytest.mark.parametrize("k", [1, 3, 8, 17, 32, 33, 40])
 def test_fused_k_operand_sum(codec, k, stage):
-    n, eb, L, T = 200_001, 1e-3, 36, 16
-    say("homsum k", k, FORM_IDS[FORMS.index(stage)])
+    eb, L, T = 1e-3, 36, 16
+    n = form_n(codec, stage, 200_001)
+    say("homsum k", k, FORM_IDS[FORMS.index(stage)], n)
     base = gen.make("grf", n, 3)
     xs = [gen.perturb(base, 100 + r) for r in range(k)]
     ss = [codec.compress(dev(x), eb, L, T) for x in xs]
-    h = codec.homsum(ss, stage=stage)
+    h = sum_form(codec, ss, stage)
     torch.cuda.synchronize()
     codec.check_errors()
     ref = O.homsum([O.compress(x, eb, L, T) for x in xs], n, L, T)
@@ -146,14 +172,13 @@ def test_fused_k_operand_sum(codec, k, stage):
 def test_fused_sum_every_width(codec, kind, L, stage):
     """Eight operands whose blocks take every width (0..24 for the ladder,
     21..32 for the wide kind), so the decoder meets each split of a width
-    into high bytes and low bits, r = 0 included; bytes equal the oracle's.
-    The wide sums have tiles too large for a wave's LDS buffer, which are
-    encoded half a wave at a time."""
-    n, eb, T, k = 40_000, 1.0, 4, 8
-    say("homsum widths", kind, L, FORM_IDS[FORMS.index(stage)])
+    into high bytes and low bits, r = 0 included; bytes equal the oracle's."""
+    eb, T, k = 1.0, 4, 8
+    n = form_n(codec, stage, 40_000, L, T) if L == 36 else 40_000
+    say("homsum widths", kind, L, FORM_IDS[FORMS.index(stage)], n)
     xs = [gen.make(kind, n, 500 + r) for r in range(k)]
     ss = [codec.compress(dev(x), eb, L, T) for x in xs]
-    h = codec.homsum(ss, stage=stage)
+    h = sum_form(codec, ss, stage)
     torch.cuda.synchronize()
     codec.check_errors()
     cs = [O.compress(x, eb, L, T) for x in xs]
@@ -167,12 +192,13 @@ def test_fused_sum_tiles_wider_than_lds(codec, stage):
     """Integer data of magnitude up to 2^26 at eb = 1: every block is 28 to
     31 bits wide, so a 64-block tile of the sum holds more than a wave's
     8 KB LDS buffer and is encoded half a wave at a time."""
-    n, eb, L, T, k = 50_000, 1.0, 36, 4, 8
-    say("homsum tiles wider than LDS", FORM_IDS[FORMS.index(stage)])
+    eb, L, T, k = 1.0, 36, 4, 8
+    n = form_n(codec, stage, 50_000, L, T)
+    say("homsum tiles wider than LDS", FORM_IDS[FORMS.index(stage)], n)
     rng = np.random.default_rng(77)
     xs = [rng.integers(-2 ** 26, 2 ** 26, n).astype(np.float32) for _ in range(k)]
     ss = [codec.compress(dev(x), eb, L, T) for x in xs]
-    h = codec.homsum(ss, stage=stage)
+    h = sum_form(codec, ss, stage)
     torch.cuda.synchronize()
     codec.check_errors()
     cs = [O.compress(x, eb, L, T) for x in xs]
@@ -187,14 +213,15 @@ def test_fused_sum_tiles_wider_than_lds(codec, stage):
 def test_sum_output_too_small_is_reported(codec, stage):
     """A sum whose output does not fit out_cap raises the capacity error and
     leaves size 0, in both forms."""
-    n, eb, L, T = 120_000, 1e-3, 36, 8
-    say("homsum nospace", FORM_IDS[FORMS.index(stage)])
+    eb, L, T = 1e-3, 36, 8
+    n = form_n(codec, stage, 120_000, L, T)
+    say("homsum nospace", FORM_IDS[FORMS.index(stage)], n)
     x = gen.make("grf", n, 41)
     sx = codec.compress(dev(x), eb, L, T)
     torch.cuda.synchronize()
     codec.check_errors()
     small = codec.empty_stream(n, L, T, capacity=sx.size // 2)
-    codec.homsum([sx, sx], out=small, stage=stage)
+    sum_form(codec, [sx, sx], stage, out=small)
     torch.cuda.synchronize()
     with pytest.raises(Exception):
         codec.check_errors()
@@ -407,7 +434,8 @@ def test_batch_plans_uniform_and_mixed(codec):
     from zccl_amd import batch
 
     L, T, eb = 36, 8, 1e-3
-    for sizes in ([100_000] * 5, [7, 40, 1000, 123_457, 9, 60_001]):
+    # uniform batches below and above the two-pass threshold (1024 tiles), and a mixed one
+    for sizes in ([100_000] * 5, [650_000] * 4, [7, 40, 1000, 123_457, 9, 60_001]):
         say("batch sizes", sizes)
         xs = [gen.make("grf", n, 11 + i) for i, n in enumerate(sizes) if n >= T]
         ns = [x.size for x in xs]

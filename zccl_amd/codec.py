@@ -120,8 +120,9 @@ def default_stage(tiles: int) -> StageArea:
     return st
 
 
-# blockSize of the two-pass sum
+# the two-pass sum: blockSize 36, sums of at least this many tiles
 STAGED_L = 36
+STAGED_MIN_TILES = 1024
 
 
 @dataclass
@@ -223,8 +224,8 @@ def homsum(streams: Sequence[DeviceStream], out: Optional[DeviceStream] = None,
             raise ValueError("operands differ in (n, L, T)")
     w = _ws_for(n, L, T, ws)
     st = None
-    if L == STAGED_L and stage is not False:
-        tiles = num_tiles(n, L, T)
+    tiles = num_tiles(n, L, T)
+    if L == STAGED_L and stage is not False and tiles >= STAGED_MIN_TILES:
         st = stage if stage is not None else default_stage(tiles)
         if st.max_tiles < tiles:
             raise _lib.ZcclError(-6, f"stage area holds {st.max_tiles} tiles, sum needs {tiles}")

@@ -332,11 +332,16 @@ extern "C" int zccl_homsum_dev(const unsigned char* const* ins, const void* cons
 
 extern "C" size_t zccl_sum_stage_bytes(size_t max_tiles) { return (size_t) stage_area_bytes(max_tiles); }
 
-// A stage area serves the two-pass form when it holds every tile of the sum
-// and is 16-byte aligned.
+// The two-pass form runs for sums of at least this many tiles; smaller ones
+// take the single-pass form (one launch instead of three).
+constexpr uint64_t kStagedMinTiles = 1024;
+
+// A stage area serves the two-pass form when the sum is large enough and the
+// area holds every tile of it and is 16-byte aligned.
 static bool stage_ok(const void* stage, size_t stage_bytes, uint64_t tiles)
 {
-    return stage && ((uintptr_t) stage & 15) == 0 && stage_bytes >= stage_area_bytes(tiles);
+    return tiles >= kStagedMinTiles && stage && ((uintptr_t) stage & 15) == 0 &&
+           stage_bytes >= stage_area_bytes(tiles);
 }
 
 extern "C" int zccl_homsum_dev_staged(const unsigned char* const* ins, const void* const* indices, int k,
