@@ -868,11 +868,22 @@ __global__ void __launch_bounds__(64 * kLaneWaves, 4) k_homsum_lane_mixed(BatchA
 
 // ---------------------------------------------------------------- two-pass sum
 
+// The hardware hands workgroup b of a launch to XCD b % 8.  The index this
+// returns gives each XCD a contiguous run of workgroups (XCD x: runs of
+// about nwg / 8 starting at x * nwg / 8), so tiles that share cache lines at
+// their edges, and their index entries, meet in one L2.  A bijection on
+// [0, nwg).
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nwg)
+{
+    const uint32_t q = nwg / 8, rem = nwg % 8, x = b % 8, s = b / 8;
+    return x * q + min(x, rem) + s;
+}
+
 // Pass one of a single sum: tile t of workgroup-wave t / kLaneWaves, slot t.
 __global__ void __launch_bounds__(64 * kLaneWaves, 4) k_homsum_lane_staged(HomsumArgs a, uint8_t* stage)
 {
     const uint32_t wv = threadIdx.x >> 6;
-    const uint64_t t = (uint64_t) blockIdx.x * kLaneWaves + wv;
+    const uint64_t t = (uint64_t) xcd_block(blockIdx.x, gridDim.x) * kLaneWaves + wv;
     if (t >= a.job.tl.ntiles)
         return;
     const ArgIO src{a.job.io};
@@ -880,16 +891,18 @@ __global__ void __launch_bounds__(64 * kLaneWaves, 4) k_homsum_lane_staged(Homsu
         a.job.io.size_out, t, nullptr, a.ws, 1, zk_lds + wv * (kOutBuf / 4), stage, a.job.tl.ntiles, t);
 }
 
-// Pass one of a uniform batch: launch tile b is tile b / njobs of job
-// b % njobs (jobs side by side), slot job * ntiles + tile.
+// Pass one of a uniform batch: a workgroup takes kLaneWaves consecutive
+// tiles of one job (ceil(ntiles / kLaneWaves) workgroups per job, jobs in
+// order of the XCD-contiguous index), slot job * ntiles + tile.
 __global__ void __launch_bounds__(64 * kLaneWaves, 5) k_homsum_lane_batch_staged(BatchArgs b, uint8_t* stage)
 {
     const uint32_t wv = threadIdx.x >> 6;
-    const uint64_t tg = (uint64_t) blockIdx.x * kLaneWaves + wv;
-    if (tg >= b.total_tiles)
+    const uint32_t wpj = (uint32_t) ceil_div(b.tl.ntiles, kLaneWaves);
+    const uint32_t g = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t jidx = __builtin_amdgcn_readfirstlane(g / wpj);
+    const uint32_t t = __builtin_amdgcn_readfirstlane((g % wpj) * kLaneWaves + wv);
+    if (jidx >= b.njobs || t >= b.tl.ntiles)
         return;
-    const uint32_t jidx = __builtin_amdgcn_readfirstlane((uint32_t) (tg % b.njobs));
-    const uint32_t t = __builtin_amdgcn_readfirstlane((uint32_t) (tg / b.njobs));
     const HomsumIO* io = (const HomsumIO*) b.jobs + jidx;
     const SumIO src{io};
     homsum_tile_lane<true>(b.g, b.tl, b.p.k, src, ldc(&io->out), ldc(&io->out_cap), ldc(&io->index),
@@ -1052,21 +1065,22 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_lane_batch(BatchArgs b, c
 
 __global__ void __launch_bounds__(64 * kLaneWaves) k_place_lane(HomsumArgs a, const uint8_t* stage)
 {
-    const uint64_t t = (uint64_t) blockIdx.x * kLaneWaves + (threadIdx.x >> 6);
+    const uint64_t t = (uint64_t) xcd_block(blockIdx.x, gridDim.x) * kLaneWaves + (threadIdx.x >> 6);
     if (t >= a.job.tl.ntiles)
         return;
     place_tile(a.job.g, a.job.tl, (uint32_t) t, a.job.io.out, a.job.io.out_cap, a.job.io.index, a.job.io.size_out,
         stage, a.job.tl.ntiles, t, a.ws);
 }
 
-// Uniform batch: launch tile b is tile b / njobs of job b % njobs.
+// Uniform batch: workgroups as in pass one.
 __global__ void __launch_bounds__(64 * kLaneWaves) k_place_lane_batch(BatchArgs b, const uint8_t* stage)
 {
-    const uint64_t tg = (uint64_t) blockIdx.x * kLaneWaves + (threadIdx.x >> 6);
-    if (tg >= b.total_tiles)
+    const uint32_t wpj = (uint32_t) ceil_div(b.tl.ntiles, kLaneWaves);
+    const uint32_t g = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t jidx = __builtin_amdgcn_readfirstlane(g / wpj);
+    const uint32_t t = __builtin_amdgcn_readfirstlane((g % wpj) * kLaneWaves + (threadIdx.x >> 6));
+    if (jidx >= b.njobs || t >= b.tl.ntiles)
         return;
-    const uint32_t jidx = __builtin_amdgcn_readfirstlane((uint32_t) (tg % b.njobs));
-    const uint32_t t = __builtin_amdgcn_readfirstlane((uint32_t) (tg / b.njobs));
     const HomsumIO* io = (const HomsumIO*) b.jobs + jidx;
     place_tile(b.g, b.tl, t, ldc(&io->out), ldc(&io->out_cap), ldc(&io->index), ldc(&io->size_out), stage,
         b.total_tiles, (uint64_t) jidx * b.tl.ntiles + t, b.ws);
@@ -1109,14 +1123,13 @@ hipError_t launch_homsum_lane_batch_staged(const BatchArgs& b, uint8_t* stage, h
 {
     if (!b.uniform)
         return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_homsum_lane_batch_staged, dim3((uint32_t) ceil_div(b.total_tiles, kLaneWaves)),
-        dim3(64 * kLaneWaves), lane_lds_bytes(), s, b, stage);
+    const uint32_t groups = (uint32_t) (ceil_div(b.tl.ntiles, kLaneWaves) * b.njobs);
+    hipLaunchKernelGGL(k_homsum_lane_batch_staged, dim3(groups), dim3(64 * kLaneWaves), lane_lds_bytes(), s, b, stage);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess)
         return e;
     hipLaunchKernelGGL(k_scan_lane_batch, dim3(b.njobs), dim3(kScanThreads), 0, s, b, (const uint8_t*) stage);
-    hipLaunchKernelGGL(k_place_lane_batch, dim3((uint32_t) ceil_div(b.total_tiles, kLaneWaves)),
-        dim3(64 * kLaneWaves), 0, s, b, (const uint8_t*) stage);
+    hipLaunchKernelGGL(k_place_lane_batch, dim3(groups), dim3(64 * kLaneWaves), 0, s, b, (const uint8_t*) stage);
     return hipGetLastError();
 }
 
