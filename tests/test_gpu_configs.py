@@ -124,3 +124,131 @@ def test_mixed_fields_512cube_properties(codec, eb):
         codec.check_errors()
         del xs, ar, fused, ring, y
         torch.cuda.empty_cache()
+
+
+def test_config2_full_size_bit_exact(codec):
+    """The headline workload at full size: 8 ranks x 512^3 FP32 GRF (k^-3,
+    shared seed) + N(0, 0.05 std) per rank, relative eb 1e-3, L = 36, T = 16.
+    All 64 segment streams (batched compress), the 8 fused sums of the
+    bench's one-launch batch (two-pass form) and the decompressed field
+    equal the oracle's bytes and bits; the per-call fused sums and the
+    decompressed field's error bound agree."""
+    from zccl_amd import batch, synth
+    from zccl_amd.allreduce import StagedAllreduce
+
+    P, side, L, T = 8, 512, 36, 16
+    count = side ** 3
+    say("config 2 fields")
+    base = synth.grf3d(side, seed=20261202)
+    xs = [synth.perturb(base, 0.05, seed=1000 + r) for r in range(P)]
+    del base
+    eb = synth.rel_to_abs_eb(xs, 1e-3)
+    ar = StagedAllreduce(P=P, count=count, eb=eb, L=L, T=T)
+    ar.compress_all(xs)
+    outs = [codec.empty_stream(ar.cnts[j], L, T) for j in range(P)]
+    plan = batch.homsum_plan([[ar.streams[r][j] for r in range(P)] for j in range(P)], outs)
+    assert plan.staged and plan.tiles >= codec.STAGED_MIN_TILES
+    ws = codec.Workspace(plan.tiles)
+    plan.launch(ws)
+    torch.cuda.synchronize()
+    codec.check_errors(ws)
+    per_call = ar.reduce_scatter_fused()
+    y = ar.decompress_all(outs)
+    torch.cuda.synchronize()
+    codec.check_errors()
+    for j in range(P):
+        assert torch.equal(per_call[j].data[: per_call[j].size], outs[j].data[: outs[j].size]), j
+    say("config 2 oracle", eb)
+    yh = y.cpu().numpy()
+    for j in range(P):
+        a, n = ar.displs[j], ar.cnts[j]
+        want = []
+        for r in range(P):
+            c = O.compress(xs[r][a: a + n].cpu().numpy(), eb, L, T)
+            assert np.array_equal(ar.streams[r][j].to_host(), c), (r, j)
+            want.append(c)
+        hw = O.homsum(want, n, L, T)
+        assert np.array_equal(outs[j].to_host(), hw), j
+        assert np.array_equal(bits(yh[a: a + n]), bits(O.decompress(hw, n, eb, L, T))), j
+        say("segment", j, "ok", hw.size)
+    truth = torch.zeros(count, dtype=torch.float64, device="cuda")
+    absum = torch.zeros_like(truth)
+    for x in xs:
+        truth += x.double()
+        absum += x.double().abs()
+    assert bool(((y.double() - truth).abs() <= P * eb + 2.0 ** -21 * absum).all())
+
+
+def test_config4_uneven_batch_graph_vs_oracle(codec):
+    """The batched form: 16 simulated ranks x 32 buffers of log-uniform sizes
+    (4 KB..8 MB of gradient-like values: N(0, 1e-2), 1 % outliers at 100x,
+    distinct per rank), relative eb 1e-3 per buffer.  One compress launch of
+    all 512 buffers and one fused-sum launch of the 32 16-operand sums (a
+    mixed batch: single-pass form), captured in a HIP graph and replayed;
+    then a uniform batch of 2 x 16 operands of 2 400 017 values (two-pass
+    form) in a graph.  Every replay equals the eager result and the oracle."""
+    from zccl_amd import batch, workloads
+
+    L, T, ranks, nslot = 36, 16, 16, 32
+    sizes = [max(n, T) for n in workloads.log_uniform_sizes(nslot, 4096, 8 << 20, 77)]
+    say("config 4 sizes", min(sizes), max(sizes), sum(sizes) * ranks * 4 / 1e9, "GB")
+    flat, views = workloads.gradient_buffers([n for _ in range(ranks) for n in sizes], 4097)
+    xs = [[views[r * nslot + i] for i in range(nslot)] for r in range(ranks)]
+    ebs = workloads.per_buffer_eb([torch.cat([xs[r][i] for r in range(ranks)]) for i in range(nslot)], 1e-3)
+
+    def run_graph(cp, hp, ws, outs, sums):
+        cp.launch(ws)
+        hp.launch(ws)
+        torch.cuda.synchronize()
+        codec.check_errors(ws)
+        eager = [s.data[: s.size].clone() for s in sums]
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            cp.launch(ws)
+            hp.launch(ws)
+        for rep in range(2):
+            for o in outs + sums:
+                o.data.zero_()
+            g.replay()
+            torch.cuda.synchronize()
+            for o in outs + sums:
+                o.invalidate()
+            codec.check_errors(ws)
+            for e, s in zip(eager, sums):
+                assert torch.equal(s.data[: s.size], e), rep
+            _ = flat[:1024].sum().item()  # eager work between replays
+
+    outs = [[codec.empty_stream(n, L, T) for n in sizes] for _ in range(ranks)]
+    sums = [codec.empty_stream(n, L, T) for n in sizes]
+    cp = batch.compress_plan([x for r in range(ranks) for x in xs[r]], ebs * ranks,
+                             [o for r in range(ranks) for o in outs[r]])
+    hp = batch.homsum_plan([[outs[r][i] for r in range(ranks)] for i in range(nslot)], sums)
+    assert not hp.staged
+    ws = codec.Workspace(max(cp.tiles, hp.tiles))
+    run_graph(cp, hp, ws, [o for r in range(ranks) for o in outs[r]], sums)
+    say("config 4 oracle (mixed)")
+    for i, n in enumerate(sizes):
+        want = []
+        for r in range(ranks):
+            c = O.compress(xs[r][i].cpu().numpy(), ebs[i], L, T)
+            assert np.array_equal(outs[r][i].to_host(), c), (r, i)
+            want.append(c)
+        assert np.array_equal(sums[i].to_host(), O.homsum(want, n, L, T)), i
+
+    # uniform jobs above the two-pass threshold
+    n2, njob = 2_400_017, 2
+    say("config 4 uniform two-pass batch")
+    flat2, v2 = workloads.gradient_buffers([n2] * (ranks * njob), 4098)
+    eb2 = workloads.per_buffer_eb([flat2], 1e-3)[0]
+    outs2 = [codec.empty_stream(n2, L, T) for _ in range(ranks * njob)]
+    sums2 = [codec.empty_stream(n2, L, T) for _ in range(njob)]
+    cp2 = batch.compress_plan(v2, [eb2] * len(v2), outs2)
+    hp2 = batch.homsum_plan([[outs2[r * njob + j] for r in range(ranks)] for j in range(njob)], sums2)
+    assert hp2.staged and hp2.tiles >= codec.STAGED_MIN_TILES
+    ws2 = codec.Workspace(max(cp2.tiles, hp2.tiles))
+    run_graph(cp2, hp2, ws2, outs2, sums2)
+    for j in range(njob):
+        want = [O.compress(v2[r * njob + j].cpu().numpy(), eb2, L, T) for r in range(ranks)]
+        for r in range(ranks):
+            assert np.array_equal(outs2[r * njob + j].to_host(), want[r]), (r, j)
+        assert np.array_equal(sums2[j].to_host(), O.homsum(want, n2, L, T)), j
