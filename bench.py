@@ -219,6 +219,54 @@ def bench_sz3d(side, Bs, T, rel_eb, steps, warmup, cpu_seconds, with_cpu):
     return res
 
 
+def testfloat_leg(a, b, rel_eb, L, T, timed):
+    """The reference driver's two-operand comparison
+    (examples/testfloat_homocompress_fastmode1_arg.c:181-282) on two FP32
+    fields of one size, on the GPU: eb = larger value range x rel (FP32);
+    DOC = decompress both + FP32 add + compress the sum; homomorphic = one add
+    of the two streams.  Throughputs in the driver's convention (4 N bytes /
+    time), and the quality of both results against the FP32 sum."""
+    import torch
+
+    from zccl_amd import codec, metrics, synth
+
+    n = a.numel()
+    eb = synth.rel_to_abs_eb([a, b], rel_eb)
+    ca = codec.compress(a, eb, L, T)
+    cb = codec.compress(b, eb, L, T)
+    da = torch.empty(n, dtype=torch.float32, device="cuda")
+    db = torch.empty_like(da)
+    dsum = torch.empty_like(da)
+    cdoc = codec.empty_stream(n, L, T)
+    chomo = codec.empty_stream(n, L, T)
+
+    def doc():
+        codec.decompress(ca, eb, out=da)
+        codec.decompress(cb, eb, out=db)
+        torch.add(da, db, out=dsum)
+        codec.compress(dsum, eb, L, T, out=cdoc)
+
+    def homo():
+        codec.homsum([ca, cb], out=chomo)
+
+    doc()
+    homo()
+    torch.cuda.synchronize()
+    codec.check_errors()
+    t_doc = timed(doc, 10)
+    t_homo = timed(homo, 10)
+    ori = a + b
+    q_doc = metrics.error_evaluation(ori, codec.decompress(cdoc, eb), cdoc.size, eb)
+    q_homo = metrics.error_evaluation(ori, codec.decompress(chomo, eb), chomo.size, eb)
+    return {
+        "elements": n, "abs_eb": eb, "rel_eb": rel_eb,
+        "doc_ms": round(t_doc * 1e3, 4), "homomorphic_ms": round(t_homo * 1e3, 4),
+        "doc_GBps": round(n * 4 / t_doc / 1e9, 1), "homomorphic_GBps": round(n * 4 / t_homo / 1e9, 1),
+        "speedup": round(t_doc / t_homo, 2),
+        "doc_quality": q_doc, "homomorphic_quality": q_homo,
+    }
+
+
 def main():
     args = parse_args()
     if args.gpus != 1:
@@ -364,6 +412,16 @@ def main():
     max_err = float(err.max().item())
     n_over = int((err > P * eb).sum().item())
     del truth, err
+    # the reference driver's quality metrics of the reduced field against
+    # the FP32 sum of the ranks' fields in rank order (add_vectors)
+    from zccl_amd import metrics
+
+    ori32 = xs[0].clone()
+    for x in xs[1:]:
+        ori32 += x
+    quality = metrics.error_evaluation(ori32, y, sum(o.size for o in outs), P * eb)
+    del ori32
+    testfloat = testfloat_leg(xs[0][: seg_n], xs[1][: seg_n], args.rel_eb, L, T, timed)
 
     # full staged allreduce (compress all + fused reduce-scatter + decompress all)
     t_all = timed(lambda: (ar.compress_all(xs), plan.launch(ws), ar.decompress_all(outs, out=y)), 3)
@@ -456,6 +514,8 @@ def main():
         "homomorphic_reduce_scatter_ms": round(t_homo_rs * 1e3, 3),
         "doc_reduce_scatter_ms": round(t_doc * 1e3, 3),
         "homomorphic_vs_doc": round(t_doc / t_homo_rs, 2),
+        "quality": quality,
+        "testfloat_two_operand": testfloat,
         "max_abs_err": max_err,
         "max_err_over_P_eb": round(max_err / (P * eb), 4),
         "n_err_over_P_eb": n_over,

@@ -71,3 +71,32 @@ def test_3d_invalid_geometry_rejected():
     assert lib.zccl_sz3d_sizes(2, 2, 2, 4, 16, ctypes.byref(worst), None) != 0   # empty sub-volume
     assert lib.zccl_sz3d_sizes(8, 8, 8, 0, 1, ctypes.byref(worst), None) != 0    # block side < 1
     assert lib.zccl_sz3d_sizes(8, 8, 8, 65, 1, ctypes.byref(worst), None) != 0   # block side > 64
+
+
+def test_reference_headers_agree_with_ours(tmp_path):
+    """One C translation unit includes the reference's own prototypes
+    (ZCCL/include/ZCCL_float.h, ZCCLd_float.h) and include/zccl_amd.h: any
+    drop-in declaration whose types differ from the reference's fails to
+    compile as conflicting types.  Skipped where the reference tree is
+    absent."""
+    import subprocess
+
+    ref_inc = "/root/reference/ZCCL/include"
+    if not os.path.isdir(ref_inc):
+        pytest.skip("reference headers not present")
+    src = tmp_path / "both.c"
+    src.write_text('#include "ZCCL_float.h"\n#include "ZCCLd_float.h"\n#include "zccl_amd.h"\n'
+                   "void *keep[] = {\n"
+                   "  (void *) ZCCL_float_openmp_threadblock_arg, (void *) ZCCL_float_single_thread_arg,\n"
+                   "  (void *) ZCCL_float_openmp_threadblock,\n"
+                   "  (void *) ZCCL_float_decompress_openmp_threadblock_arg,\n"
+                   "  (void *) ZCCL_float_decompress_single_thread_arg,\n"
+                   "  (void *) ZCCL_float_decompress_openmp_threadblock,\n"
+                   "  (void *) ZCCL_float_homomophic_add_openmp_threadblock,\n"
+                   "  (void *) ZCCL_float_homomophic_add_single_thread,\n"
+                   "  (void *) ZCCL_float_homomophic_add_openmp_threadblock_record,\n"
+                   "};\n")
+    out = subprocess.run(["gcc", "-std=gnu99", "-fsyntax-only", "-I", ref_inc, "-I",
+                          os.path.join(ROOT, "include"), "-I", "/opt/conda/include", str(src)],
+                         capture_output=True, text=True)
+    assert out.returncode == 0, out.stderr

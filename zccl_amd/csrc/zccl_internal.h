@@ -27,6 +27,7 @@
 
 #include <stddef.h>
 #include <stdint.h>
+#include <string.h>
 
 #if defined(__HIPCC__)
 #define ZK_HD __host__ __device__ inline
@@ -418,5 +419,38 @@ enum : unsigned long long {
     ZK_ERR_OVERRUN = 1ull << 4,   // stream walk left its stated size
     ZK_ERR_CAPACITY = 1ull << 5,  // workspace too small for the launch
 };
+
+// Exact size of a host stream: its last header entry locates chunk T-1,
+// whose blocks (and the remainder segment's) are walked by their width bytes.
+// Nothing past the bytes the walk proves to be in the stream is read, and the
+// walk never leaves the geometry's worst-case size; 0 when the header or a
+// width byte is malformed.  Host code (the drop-in layer sizes host copies
+// with it).  ZCCL/src/ZCCLd_float.c:597-745 (the same walk).
+inline uint64_t host_stream_bytes(const unsigned char* in, const Geom& g)
+{
+    const uint64_t worst = worst_case_bytes(g);
+    uint64_t h = 0;
+    memcpy(&h, in + 8ull * (g.T - 1), 8);
+    if (h > worst)
+        return 0;
+    uint64_t pos = 8ull * g.T + h;
+    const uint32_t nseg = g.R ? 2u : 1u;
+    for (uint32_t sidx = 0; sidx < nseg; sidx++) {
+        const uint32_t s = g.T - 1 + sidx;
+        const uint64_t nb = s < g.T ? g.nbT : g.nbR;
+        const uint64_t len = s < g.T ? g.tb : g.R;
+        pos += 4;  // seed
+        for (uint64_t jb = 0; jb < nb; jb++) {
+            if (pos >= worst)
+                return 0;
+            const uint32_t w = in[pos];
+            if (w > 32)
+                return 0;
+            const uint64_t left = len - 1 - jb * g.L;
+            pos += block_bytes(w, (uint32_t) (left < g.L ? left : g.L));
+        }
+    }
+    return pos <= worst ? pos : 0;
+}
 
 }  // namespace zk
