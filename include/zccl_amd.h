@@ -163,6 +163,36 @@ int zccl_homsum_dev_staged(const unsigned char *const *ins, const void *const *i
     unsigned long long *d_size, void *workspace, size_t workspace_bytes, void *stage, size_t stage_bytes,
     void *stream);
 
+/* ------------------------------------------------------------ staged allreduce
+ *
+ * The hZCCL allreduce data flow (MPI_Allreduce_ZCCL_RI2_mt_oa_ho_record,
+ * ZCCL/src/ZCCL_ring_ho.c:177-317) for nranks simulated ranks on one device:
+ * segments of ceil(count/nranks) values (the last short), every rank's
+ * segments compressed in one launch, each segment's nranks streams summed
+ * homomorphically in one launch (the ring's pairwise chain gives the same
+ * bytes), the reduced segments decompressed into recvbuf in one launch.  On
+ * one device the compressed allgather moves nothing: recvbuf is what every
+ * rank holds after it.  absErrBound is the reference's absolute bound (its
+ * `compressionRatio` argument, :195); blockSize and nchunks as in the codec.
+ * All device memory is one caller workspace of
+ * zccl_allreduce_workspace_bytes() bytes (256-byte aligned); run() launches
+ * on `stream`, allocates nothing and returns before the work completes
+ * (kernel-side errors: zccl_allreduce_errors).  Every segment must hold at
+ * least nchunks values; at most 32 ranks. */
+typedef struct zccl_allreduce zccl_allreduce;
+size_t zccl_allreduce_workspace_bytes(int nranks, size_t count, int blockSize, int nchunks);
+zccl_allreduce *zccl_allreduce_create(int nranks, size_t count, float absErrBound, int blockSize, int nchunks,
+    void *workspace, size_t workspace_bytes, int *status);
+/* sendbufs: host array of nranks device pointers to count floats each;
+ * recvbuf: device, count floats. */
+int zccl_allreduce_run(zccl_allreduce *allreduce, const float *const *sendbufs, float *recvbuf, void *stream);
+/* The reduced stream of segment j after a run (device bytes and index), its
+ * element count and its offset in recvbuf. */
+int zccl_allreduce_segment(const zccl_allreduce *allreduce, int j, const unsigned char **bytes,
+    const void **index, size_t *nbEle, size_t *offset);
+int zccl_allreduce_errors(const zccl_allreduce *allreduce, unsigned long long *errors, int clear, void *stream);
+void zccl_allreduce_destroy(zccl_allreduce *allreduce);
+
 /* No counterpart in the reference: a measurement aid.  Copies `bytes` (a
  * multiple of 16, 16-byte aligned buffers) from src to dst with 16-byte
  * global loads (mode 0) or 16-byte buffer loads into LDS (mode 1, the load

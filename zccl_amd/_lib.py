@@ -36,6 +36,8 @@ EXPORTS = (
     "zccl_workspace_init", "zccl_workspace_errors",
     "zccl_compress_dev", "zccl_decompress_dev", "zccl_homsum_dev", "zccl_homsum_dev_noalloc", "zccl_parse_dev",
     "zccl_sum_stage_bytes", "zccl_homsum_dev_staged", "zccl_batch_launch_staged", "zccl_calib_copy",
+    "zccl_allreduce_workspace_bytes", "zccl_allreduce_create", "zccl_allreduce_run", "zccl_allreduce_segment",
+    "zccl_allreduce_errors", "zccl_allreduce_destroy",
     "zccl_case_count_dev",
     "zccl_batch_homsum_create", "zccl_batch_compress_create", "zccl_batch_decompress_create",
     "zccl_batch_tiles", "zccl_batch_launch", "zccl_batch_destroy",
@@ -103,6 +105,12 @@ def lib() -> ctypes.CDLL:
         "zccl_homsum_dev_noalloc": ([vp, vp, i32, sz, i32, i32, vp, sz, vp, vp, vp], i32),
         "zccl_sum_stage_bytes": ([sz], sz),
         "zccl_calib_copy": ([vp, vp, sz, i32, vp], i32),
+        "zccl_allreduce_workspace_bytes": ([i32, sz, i32, i32], sz),
+        "zccl_allreduce_create": ([i32, sz, ctypes.c_float, i32, i32, vp, sz, vp], vp),
+        "zccl_allreduce_run": ([vp, vp, vp, vp], i32),
+        "zccl_allreduce_segment": ([vp, i32, vp, vp, vp, vp], i32),
+        "zccl_allreduce_errors": ([vp, vp, i32, vp], i32),
+        "zccl_allreduce_destroy": ([vp], None),
         "zccl_homsum_dev_staged": ([vp, vp, i32, sz, i32, i32, vp, sz, vp, vp, vp, sz, vp, sz, vp], i32),
         "zccl_batch_launch_staged": ([vp, vp, sz, vp, sz, vp], i32),
         "zccl_parse_dev": ([vp, sz, sz, i32, i32, vp, vp, vp, sz, vp], i32),

@@ -164,3 +164,82 @@ class StagedAllreduce:
                 y = codec.decompress(tmp[send], self.eb)
                 part[dst][send] += y
         return [part[(j) % P][j] for j in range(P)]
+
+
+class DeviceAllreduce:
+    """The same data flow as StagedAllreduce, run by the library's C host
+    code (zccl_allreduce_*: one compress launch, one fused-sum launch and one
+    decompress launch over a caller workspace)."""
+
+    def __init__(self, P: int, count: int, eb: float, L: int = 36, T: int = 16):
+        import ctypes
+
+        from ._lib import check, lib
+
+        self.P, self.count, self.eb, self.L, self.T = P, count, eb, L, T
+        nbytes = int(lib().zccl_allreduce_workspace_bytes(P, count, L, T))
+        if not nbytes:
+            raise ValueError("invalid allreduce geometry (every segment needs at least T values)")
+        self.ws = torch.empty(nbytes + 256, dtype=torch.uint8, device="cuda")
+        base = self.ws.data_ptr()
+        self._ws_ptr = (base + 255) // 256 * 256
+        st = ctypes.c_int(0)
+        h = lib().zccl_allreduce_create(P, count, ctypes.c_float(eb), L, T, ctypes.c_void_p(self._ws_ptr), nbytes,
+                                        ctypes.byref(st))
+        check(st.value, "allreduce create")
+        self.handle = ctypes.c_void_p(h)
+        self.cnts, self.displs = segments(count, P)
+
+    def run(self, xs: Sequence[torch.Tensor], out: Optional[torch.Tensor] = None) -> torch.Tensor:
+        import ctypes
+
+        from ._lib import check, lib
+
+        if len(xs) != self.P or any(x.numel() != self.count or x.dtype != torch.float32 or not x.is_contiguous()
+                                    for x in xs):
+            raise ValueError("one contiguous float32 buffer of count values per rank")
+        y = out if out is not None else torch.empty(self.count, dtype=torch.float32, device="cuda")
+        send = (ctypes.c_void_p * self.P)(*[x.data_ptr() for x in xs])
+        check(lib().zccl_allreduce_run(self.handle, send, ctypes.c_void_p(y.data_ptr()),
+                                       ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)), "allreduce run")
+        return y
+
+    def segment_stream(self, j: int) -> codec.DeviceStream:
+        """The reduced stream of segment j after a run: views of its bytes
+        and index inside the workspace (valid until the next run)."""
+        import ctypes
+
+        from ._lib import check, lib
+
+        b, ix, n, off = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_size_t(), ctypes.c_size_t()
+        check(lib().zccl_allreduce_segment(self.handle, j, ctypes.byref(b), ctypes.byref(ix), ctypes.byref(n),
+                                           ctypes.byref(off)), "allreduce segment")
+        base = self.ws.data_ptr()
+        cap = codec.worst_case_bytes(n.value, self.L, self.T)
+        ixb = codec.index_bytes(n.value, self.L, self.T)
+        data = self.ws[b.value - base: b.value - base + cap]
+        index = self.ws[ix.value - base: ix.value - base + ixb]
+        return codec.DeviceStream(data, index, n.value, self.L, self.T)
+
+    def errors(self) -> int:
+        import ctypes
+
+        from ._lib import check, lib
+
+        e = ctypes.c_ulonglong(0)
+        check(lib().zccl_allreduce_errors(self.handle, ctypes.byref(e), 1,
+                                          ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)), "allreduce errors")
+        return int(e.value)
+
+    def close(self) -> None:
+        from ._lib import lib
+
+        if getattr(self, "handle", None) is not None:
+            lib().zccl_allreduce_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

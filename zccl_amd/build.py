@@ -18,7 +18,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libzccl_amd.so")
 
-SOURCES = ["zccl_kernels.hip", "zccl_lane.hip", "zccl_sz3d.hip", "zccl_capi.hip", "zccl_calib.hip"]
+SOURCES = ["zccl_kernels.hip", "zccl_lane.hip", "zccl_sz3d.hip", "zccl_capi.hip", "zccl_calib.hip", "zccl_allreduce.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
