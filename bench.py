@@ -267,6 +267,60 @@ def testfloat_leg(a, b, rel_eb, L, T, timed):
     }
 
 
+def dropin_leg(a, b, eb, L, T, timed):
+    """The reference-ABI calls with device pointers on two FP32 fields of one
+    size (the drop-in decompress, ZCCL/include/ZCCLd_float.h:21, and
+    homomorphic add, :29), timed per call, beside the index rebuild (parse)
+    every such call runs on a stream that carries no index, timed alone."""
+    import ctypes
+
+    import torch
+
+    from zccl_amd import codec
+    from zccl_amd._lib import lib
+
+    n = a.numel()
+    lib().zccl_set_num_chunks(T)
+    try:
+        ca = codec.compress(a, eb, L, T)
+        cb = codec.compress(b, eb, L, T)
+        torch.cuda.synchronize()
+        y = torch.empty(n, dtype=torch.float32, device="cuda")
+        out = torch.zeros(codec.worst_case_bytes(n, L, T), dtype=torch.uint8, device="cuda")
+        size = ctypes.c_size_t(0)
+        f_eb = ctypes.c_float(eb)
+
+        def dec():
+            lib().ZCCL_float_decompress_openmp_threadblock_arg(ctypes.c_void_p(y.data_ptr()), n, f_eb, L,
+                                                               ctypes.c_void_p(ca.data.data_ptr()))
+
+        def add():
+            size.value = 0
+            lib().ZCCL_float_homomophic_add_openmp_threadblock(ctypes.c_void_p(out.data_ptr()), ctypes.byref(size), n,
+                                                               f_eb, L, ctypes.c_void_p(ca.data.data_ptr()),
+                                                               ctypes.c_void_p(cb.data.data_ptr()))
+
+        dec()
+        add()
+        assert lib().zccl_last_error() == 0
+        ok = bool(torch.equal(y, codec.decompress(ca, eb))) and \
+            bool(torch.equal(out[: size.value], codec.homsum([ca, cb]).data[: size.value]))
+        t_dec = timed(dec, 10)
+        t_add = timed(add, 10)
+        ws = codec.default_workspace(codec.num_tiles(n, L, T))
+        t_parse = timed(lambda: codec.from_bytes(ca.data, n, L, T, ws=ws), 10)
+        t_kdec = timed(lambda: codec.decompress(ca, eb, out=y), 10)
+    finally:
+        lib().zccl_set_num_chunks(0)
+    return {
+        "elements": n, "T": T,
+        "decompress_ms": round(t_dec * 1e3, 4), "homomorphic_add_ms": round(t_add * 1e3, 4),
+        "parse_ms": round(t_parse * 1e3, 4), "device_api_decompress_ms": round(t_kdec * 1e3, 4),
+        "parse_share_of_decompress": round(t_parse / t_dec, 3),
+        "equal_device_api": ok,
+    }
+
+
 def main():
     args = parse_args()
     if args.gpus != 1:
@@ -422,9 +476,21 @@ def main():
     quality = metrics.error_evaluation(ori32, y, sum(o.size for o in outs), P * eb)
     del ori32
     testfloat = testfloat_leg(xs[0][: seg_n], xs[1][: seg_n], args.rel_eb, L, T, timed)
+    dropin = dropin_leg(xs[0][: seg_n], xs[1][: seg_n], eb, L, T, timed)
+    log(f"drop-in (device pointers, 2^24): {dropin}")
 
     # full staged allreduce (compress all + fused reduce-scatter + decompress all)
     t_all = timed(lambda: (ar.compress_all(xs), plan.launch(ws), ar.decompress_all(outs, out=y)), 3)
+    # the same data flow run by the library's C host code (three launches)
+    from zccl_amd.allreduce import DeviceAllreduce
+
+    car = DeviceAllreduce(P, count, eb, L, T)
+    yc = car.run(xs)
+    torch.cuda.synchronize()
+    c_equal = bool(torch.equal(yc, y)) and car.errors() == 0
+    t_all_c = timed(lambda: car.run(xs, out=yc), 5)
+    car.close()
+    del car, yc
     # the reference's decompress-operate-compress ring on the same data
     t_doc = timed(lambda: ar.reduce_scatter_doc(xs), 2)
     t_homo_rs = t_comp + ms_per_step / 1e3
@@ -510,12 +576,15 @@ def main():
         "compress_GBps_input": round(P * count * 4 / 1e9 / t_comp, 1),
         "decompress_GBps_output": round(count * 4 / 1e9 / t_dec, 1),
         "allreduce_ms": round(t_all * 1e3, 3),
+        "allreduce_c_ms": round(t_all_c * 1e3, 3),
+        "allreduce_c_equal": c_equal,
         "allreduce_autotune": [tune_c, tune_d],
         "homomorphic_reduce_scatter_ms": round(t_homo_rs * 1e3, 3),
         "doc_reduce_scatter_ms": round(t_doc * 1e3, 3),
         "homomorphic_vs_doc": round(t_doc / t_homo_rs, 2),
         "quality": quality,
         "testfloat_two_operand": testfloat,
+        "dropin_device_pointers": dropin,
         "max_abs_err": max_err,
         "max_err_over_P_eb": round(max_err / (P * eb), 4),
         "n_err_over_P_eb": n_over,
