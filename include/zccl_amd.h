@@ -213,10 +213,15 @@ int zccl_homsum_dev_noalloc(const unsigned char *const *ins, const void *const *
     unsigned long long *d_size, void *stream);
 
 /* Rebuild the index of a device stream that has none (e.g. produced by the
- * CPU library); reads at most in_cap bytes of `in`. */
+ * CPU library); reads at most in_cap bytes of `in`.  The reference walks
+ * each chunk sequentially (ZCCL/src/ZCCLd_float.c:597-745); with a workspace
+ * of zccl_parse_workspace_bytes() bytes and chunks of at least 256 blocks
+ * (blockSize <= 64) the walk runs in parallel over 4 KB tiles of every
+ * chunk, otherwise one wave walks each chunk. */
 int zccl_parse_dev(const unsigned char *in, size_t in_cap, size_t nbEle, int blockSize,
     int nchunks, void *index, unsigned long long *d_size, void *workspace, size_t workspace_bytes,
     void *stream);
+size_t zccl_parse_workspace_bytes(size_t nbEle, int blockSize, int nchunks);
 
 /* ------------------------------------------------------------ batches
  *

@@ -230,18 +230,93 @@ def test_sum_output_too_small_is_reported(codec, stage):
 
 # ------------------------------------------------------------ foreign streams
 
-@pytest.mark.parametrize("kind,n,eb,L,T", [("walk", 100_003, 1e-2, 36, 16), ("sines", 5000, 1e-4, 7, 3),
-                                           ("walk", 17, 1e-2, 36, 16), ("zeros", 999, 1e-2, 36, 4)])
+def expected_index(cx, n, L, T):
+    """The index a parse must rebuild, from the oracle's walk: widths, the
+    offset of every group's first block (16 blocks per group at L <= 64,
+    fewer for longer blocks; an empty segment's group points past its seed),
+    every segment's seed offset and the stream size, at the layout of
+    zccl_internal.h make_geom."""
+    w, bo, so, tot = O.parse(cx, n, L, T)
+    tb, R = n // T, n % T
+    nbT = -(-(tb - 1) // L)
+    nbR = -(-(R - 1) // L) if R else 0
+    IG = max(1, min(16, 576 // ((L + 3) // 4)))
+    gT = -(-nbT // IG) if nbT else 1
+    gR = (-(-nbR // IG) if nbR else 1) if R else 0
+    groups = []
+    for s in range(T + (1 if R else 0)):
+        nb, g = (nbT, gT) if s < T else (nbR, gR)
+        b0 = s * nbT
+        groups += [int(bo[b0 + q * IG]) for q in range(g)] if nb else [int(so[s]) + 4]
+    return w, np.array(groups, dtype=np.uint64), np.asarray(so, dtype=np.uint64), int(tot)
+
+
+def check_full_index(s, cx):
+    w, groups, seeds, tot = expected_index(cx, s.n, s.L, s.T)
+    ix = s.index.cpu().numpy()
+    nblocks = w.size
+    g0 = -(-nblocks // 16) * 16
+    assert np.array_equal(ix[:nblocks], w)
+    assert np.array_equal(ix[g0: g0 + 8 * groups.size].view(np.uint64), groups)
+    s0 = g0 + 8 * groups.size
+    assert np.array_equal(ix[s0: s0 + 8 * seeds.size].view(np.uint64), seeds)
+    assert s.size == tot == cx.size
+
+
+PARSE_CASES = [("walk", 100_003, 1e-2, 36, 16), ("sines", 5000, 1e-4, 7, 3), ("walk", 17, 1e-2, 36, 16),
+               ("zeros", 999, 1e-2, 36, 4), ("cfg1", 1 << 24, 1e-4, 36, 16), ("grf", 3_000_017, 1e-3, 36, 7),
+               ("slabs", 2_000_000, 1e-2, 36, 8), ("wide", 600_000, 1.0, 36, 2), ("walk", 700_001, 1e-3, 64, 5),
+               ("sines", 400_000, 1e-4, 8, 3), ("grf", 1_000_000, 1e-3, 36, 1)]
+
+
+@pytest.mark.parametrize("kind,n,eb,L,T", PARSE_CASES)
 def test_parse_cpu_stream(codec, kind, n, eb, L, T):
+    """A stream written by the oracle (standing in for the CPU library),
+    uploaded without its index: the rebuilt index equals the oracle's walk
+    in every entry, through the parallel parse (the default workspace has
+    room for its tables; it serves chunks of 256 blocks and more at
+    L <= 64) and through the serial walk (a workspace without that room),
+    and the stream decompresses to the oracle's values."""
     say("parse", kind, n, L, T)
     x = gen.make(kind, n, 5)
     cx = O.compress(x, eb, L, T)
-    s = codec.upload(cx, n, L, T)
+    s = codec.upload(cx, n, L, T)  # default workspace: parallel where it applies
     y = codec.decompress(s, eb)
     torch.cuda.synchronize()
     codec.check_errors()
-    check_index(codec, s, cx)
+    check_full_index(s, cx)
     assert np.array_equal(bits(y.cpu().numpy()), bits(O.decompress(cx, n, eb, L, T)))
+    # the serial walk, forced by a workspace too small for the tables
+    small = codec.Workspace(codec.num_tiles(n, L, T))
+    t = torch.from_numpy(np.ascontiguousarray(cx)).cuda()
+    s2 = codec.from_bytes(t, n, L, T, ws=small)
+    torch.cuda.synchronize()
+    codec.check_errors(small)
+    check_full_index(s2, cx)
+
+
+def test_parallel_parse_rejects_corrupt_streams(codec):
+    """Corrupt foreign streams through the parallel parse: a width byte above
+    32 inside a chunk, and a chunk offset in the header that does not match
+    where the previous chunk's blocks end, are reported; no stream size is
+    left in the index."""
+    n, eb, L, T = 2_000_003, 1e-3, 36, 4
+    x = gen.make("grf", n, 9)
+    cx = O.compress(x, eb, L, T)
+    w, bo, so, tot = O.parse(cx, n, L, T)
+    bad1 = cx.copy()
+    bad1[int(bo[w.size // 3])] = 99
+    bad2 = cx.copy()
+    h = bad2[8:16].view(np.uint64)
+    h[0] += 1  # chunk 1 starts one byte later than chunk 0's blocks end
+    for bad in (bad1, bad2):
+        t = torch.from_numpy(np.ascontiguousarray(bad)).cuda()
+        s = codec.from_bytes(t, n, L, T)
+        torch.cuda.synchronize()
+        with pytest.raises(Exception):
+            codec.check_errors()
+        s.invalidate()
+        assert s.size == 0
 
 
 # ------------------------------------------------------------ drop-in ABI

@@ -37,7 +37,7 @@ EXPORTS = (
     "zccl_compress_dev", "zccl_decompress_dev", "zccl_homsum_dev", "zccl_homsum_dev_noalloc", "zccl_parse_dev",
     "zccl_sum_stage_bytes", "zccl_homsum_dev_staged", "zccl_batch_launch_staged", "zccl_calib_copy",
     "zccl_allreduce_workspace_bytes", "zccl_allreduce_create", "zccl_allreduce_run", "zccl_allreduce_segment",
-    "zccl_allreduce_errors", "zccl_allreduce_destroy",
+    "zccl_allreduce_errors", "zccl_allreduce_destroy", "zccl_parse_workspace_bytes",
     "zccl_case_count_dev",
     "zccl_batch_homsum_create", "zccl_batch_compress_create", "zccl_batch_decompress_create",
     "zccl_batch_tiles", "zccl_batch_launch", "zccl_batch_destroy",
@@ -111,6 +111,7 @@ def lib() -> ctypes.CDLL:
         "zccl_allreduce_segment": ([vp, i32, vp, vp, vp, vp], i32),
         "zccl_allreduce_errors": ([vp, vp, i32, vp], i32),
         "zccl_allreduce_destroy": ([vp], None),
+        "zccl_parse_workspace_bytes": ([sz, i32, i32], sz),
         "zccl_homsum_dev_staged": ([vp, vp, i32, sz, i32, i32, vp, sz, vp, vp, vp, sz, vp, sz, vp], i32),
         "zccl_batch_launch_staged": ([vp, vp, sz, vp, sz, vp], i32),
         "zccl_parse_dev": ([vp, sz, sz, i32, i32, vp, vp, vp, sz, vp], i32),

@@ -260,7 +260,12 @@ def from_bytes(data: torch.Tensor, n: int, L: int, T: int, ws: Optional[Workspac
     if data.dtype != torch.uint8 or not data.is_cuda:
         raise ValueError("data must be a uint8 CUDA tensor")
     index = torch.empty(index_bytes(n, L, T), dtype=torch.uint8, device=data.device)
-    w = _ws_for(n, L, T, ws)
+    if ws is None:
+        # room for the parallel parse's tables
+        need = int(lib().zccl_parse_workspace_bytes(n, L, T))
+        w = default_workspace(max(num_tiles(n, L, T), (need - 256) // 16 + 1))
+    else:
+        w = _ws_for(n, L, T, ws)
     check(lib().zccl_parse_dev(_ptr(data), data.numel(), n, L, T, _ptr(index), None, w.ptr, w.nbytes,
                                _stream_handle()), "parse")
     return DeviceStream(data, index, n, L, T)

@@ -721,9 +721,19 @@ extern "C" int zccl_parse_dev(const unsigned char* in, size_t in_cap, size_t n, 
     a.in_cap = in_cap;
     a.index = (uint8_t*) index;
     a.ws = (WsHeader*) ws;
+    a.ws_bytes = ws_bytes;
     a.size_out = d_size;
     HIP_TRY(launch_parse(a, (hipStream_t) stream));
     return ZCCL_OK;
+}
+
+extern "C" size_t zccl_parse_workspace_bytes(size_t n, int L, int T)
+{
+    Geom g;
+    if (!geom_of(n, L, T, &g))
+        return 0;
+    const size_t base = (size_t) workspace_bytes(1024);
+    return parse_parallel_ok(g) ? std::max<size_t>(base, (size_t) parse_layout(g).bytes) : base;
 }
 
 extern "C" int zccl_case_count_dev(const void* index1, const void* index2, size_t n, int L, int T,
@@ -799,8 +809,13 @@ Ctx* ctx_current(int* rc)
     return ctxs[dev];
 }
 
-int ensure_ws(Ctx& c, uint64_t tiles)
+// A workspace of at least `tiles` tiles that also holds the parallel parse's
+// tables for the geometry (n, L, T) when that parse applies.
+int ensure_ws(Ctx& c, uint64_t tiles, size_t n = 0, int L = 0, int T = 0)
 {
+    Geom g;
+    if (n && geom_of(n, L, T, &g) && parse_parallel_ok(g))
+        tiles = std::max<uint64_t>(tiles, workspace_capacity(parse_layout(g).bytes) + 1);
     if (c.ws.p && c.ws_tiles >= tiles)
         return ZCCL_OK;
     uint64_t want = std::max<uint64_t>(tiles, 1024);
@@ -886,6 +901,7 @@ int stage_stream(Ctx& c, const Geom& g, const unsigned char* bytes, size_t cap, 
     a.in_cap = cap;
     a.index = (uint8_t*) c.ix[slot].p;
     a.ws = (WsHeader*) c.ws.p;
+    a.ws_bytes = c.ws.cap;
     a.size_out = nullptr;
     HIP_TRY(launch_parse(a, c.stream));
     *dptr = d;
@@ -906,7 +922,7 @@ int dropin_compress(unsigned char* outputBytes, const float* x, size_t* outSize,
     if (!c)
         return rc;
     std::lock_guard<std::mutex> lk(c->mu);
-    if ((rc = ensure_ws(*c, zccl_num_tiles(n, L, T))) || (rc = ensure(c->small, 64)) || (rc = ensure(c->ix[0], g.ix_bytes)))
+    if ((rc = ensure_ws(*c, zccl_num_tiles(n, L, T), n, L, T)) || (rc = ensure(c->small, 64)) || (rc = ensure(c->ix[0], g.ix_bytes)))
         return rc;
     size_t avail_x = 0, avail_o = 0;
     bool xdev = device_ptr(x, &avail_x), odev = device_ptr(outputBytes, &avail_o);
@@ -954,7 +970,7 @@ int dropin_decompress(float* newData, size_t n, float eb, int L, int T, const un
     if (!c)
         return rc;
     std::lock_guard<std::mutex> lk(c->mu);
-    if ((rc = ensure_ws(*c, zccl_num_tiles(n, L, T))))
+    if ((rc = ensure_ws(*c, zccl_num_tiles(n, L, T), n, L, T)))
         return rc;
     const unsigned char* din = nullptr;
     if ((rc = stage_stream(*c, g, cmp, cmpSize, sized, 0, &din)))
@@ -993,7 +1009,7 @@ int dropin_add(unsigned char* out, size_t out_cap, bool out_cap_known, size_t* o
     if (!c)
         return rc;
     std::lock_guard<std::mutex> lk(c->mu);
-    if ((rc = ensure_ws(*c, zccl_num_tiles(n, L, T))) || (rc = ensure(c->small, 64)) || (rc = ensure(c->ix[2], g.ix_bytes)))
+    if ((rc = ensure_ws(*c, zccl_num_tiles(n, L, T), n, L, T)) || (rc = ensure(c->small, 64)) || (rc = ensure(c->ix[2], g.ix_bytes)))
         return rc;
     const unsigned char* da = nullptr;
     const unsigned char* db = nullptr;

@@ -125,8 +125,51 @@ struct ParseArgs {
     uint64_t in_cap;               // bytes of `bytes` that may be read
     uint8_t* index;
     WsHeader* ws;
+    uint64_t ws_bytes;             // the workspace's size (the parallel parse keeps its tables there)
     unsigned long long* size_out;
 };
+
+// Parallel parse (zccl_kernels.hip): a chunk's block region (the bytes after
+// its seed) in tiles of kParseTile bytes; per tile a table of candidate first
+// block offsets (0 .. block_bytes(32, L) - 1), then per tile its convergence
+// word, entry and block count / base.  The tables live in the workspace
+// after its header and are zeroed again when the parse ends.
+constexpr uint32_t kParseTile = 4096;
+constexpr uint32_t kParseMaxL = 64;         // at most 265 candidates per tile
+constexpr uint32_t kParseMinBlocks = 256;   // chunks of fewer blocks take the serial walk
+constexpr uint32_t kParseDead = 0xFFFFFFFFu;   // no entry / no common exit
+constexpr uint32_t kParseNoExit = 0xFFFFu;     // table: the walk met a width byte above 32
+
+struct ParseLayout {
+    uint32_t maxb;     // candidates per tile: the largest block's bytes
+    uint32_t tpc;      // tiles per chunk: its worst-case block region / kParseTile
+    uint64_t ntiles;   // T * tpc
+    uint64_t flag_off;  // a word set when any pass finds the stream corrupt
+    uint64_t table_off, conv_off, entry_off, base_off, bytes;  // byte offsets from the workspace start
+};
+
+ZK_HD ParseLayout parse_layout(const Geom& g)
+{
+    ParseLayout p;
+    p.maxb = block_bytes(32, g.L);
+    p.tpc = (uint32_t) ceil_div((uint64_t) g.nbT * p.maxb, kParseTile);
+    p.ntiles = (uint64_t) g.T * p.tpc;
+    uint64_t at = sizeof(WsHeader);
+    p.flag_off = at;
+    at += 16;
+    p.table_off = at;
+    at += p.ntiles * p.maxb * 4;
+    p.conv_off = at;
+    at += p.ntiles * 4;
+    p.entry_off = at;
+    at += p.ntiles * 4;
+    p.base_off = at;
+    at += p.ntiles * 4;
+    p.bytes = align_up(at, 16);
+    return p;
+}
+
+ZK_HD bool parse_parallel_ok(const Geom& g) { return g.L <= kParseMaxL && g.nbT >= kParseMinBlocks; }
 
 // One call of the 3-D codec (zccl_sz3d.hip).  The workspace holds the status
 // words (status[0] stream size, status[1] error bits), the per-block size

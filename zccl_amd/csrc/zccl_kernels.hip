@@ -1526,6 +1526,309 @@ __global__ void __launch_bounds__(64) k_parse(ParseArgs a)
     }
 }
 
+// ------------------------------------------------------------ parallel parse
+//
+// The serial walk above runs one wave per chunk: a 2^24-value stream keeps
+// 16 waves busy for milliseconds.  Here a chunk's block region (the bytes
+// after its seed) is cut into tiles of kParseTile bytes.  The first block
+// starting inside a tile lies within its first maxb = block_bytes(32, L)
+// bytes, so pass one walks full-size blocks from every such candidate offset
+// to the tile's end (exit offset past the tile's end, blocks walked; dead
+// when a width byte exceeds 32).  Walks from different offsets merge within a
+// few blocks, so a tile's surviving candidates usually share one exit and
+// the next tile's entry is known without this tile's.  Pass two (a
+// workgroup per chunk) chains entries from the nearest such tile and scans
+// the block counts; pass three walks each tile from its entry and writes the
+// widths, group offsets, seeds, the chunk's last (possibly short) block, the
+// remainder segment and the stream size.  Same walk as the serial parse
+// (ZCCL/src/ZCCLd_float.c:597-745), same index.
+
+// Block region of chunk c: [cs, ce), cs just past the chunk's seed; ce the
+// next chunk's seed (from the header) or, for the last chunk, its worst-case
+// end.  Both clamped to the readable bytes.
+__device__ __forceinline__ bool parse_region(const ParseArgs& a, const ParseLayout& pl, uint32_t c, uint64_t& cs,
+    uint64_t& ce)
+{
+    const Geom& g = a.g;
+    if (8ull * g.T > a.in_cap)
+        return false;
+    cs = 8ull * g.T + load_u64_bytes(a.bytes + 8ull * c) + 4;
+    ce = c + 1 < g.T ? 8ull * g.T + load_u64_bytes(a.bytes + 8ull * (c + 1)) : cs + (uint64_t) g.nbT * pl.maxb;
+    if (ce > a.in_cap)
+        ce = a.in_cap;
+    return cs <= ce && ce - cs <= (uint64_t) pl.tpc * kParseTile;
+}
+
+// Bytes [ts, te) of the stream into LDS (one wave), dword loads where the
+// dword is readable, bytes at the very end.
+__device__ __forceinline__ void parse_stage(const ParseArgs& a, uint64_t ts, uint64_t te, uint32_t* buf)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t base = ts & ~3ull;
+    const uint32_t ndw = (uint32_t) ((te - base + 3) >> 2);
+    const bool aligned = ((uintptr_t) a.bytes & 3) == 0;
+    for (uint32_t i = lane; i < ndw; i += 64) {
+        const uint64_t q = base + 4ull * i;
+        uint32_t v = 0;
+        if (q + 4 <= a.in_cap) {
+            v = aligned ? *(const uint32_t*) (a.bytes + q) : load_u32_bytes(a.bytes + q);
+        } else {
+            for (uint32_t k = 0; k < 4 && q + k < a.in_cap; k++)
+                v |= (uint32_t) a.bytes[q + k] << (8 * k);
+        }
+        buf[i] = v;
+    }
+}
+
+// A corrupt stream: the sticky workspace error and the parse's own flag
+// (which makes k_parse_finish leave no stream size).
+__device__ __forceinline__ void parse_fail(const ParseArgs& a, const ParseLayout& pl)
+{
+    raise_error(a.ws, ZK_ERR_OVERRUN);
+    __hip_atomic_store((unsigned int*) ((char*) a.ws + pl.flag_off), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ uint32_t parse_block_bytes(uint32_t w, uint32_t L)
+{
+    return w ? 1u + ((L + 7u) >> 3) + (w >> 3) * L + (((w & 7u) * L + 7u) >> 3) : 1u;
+}
+
+// Pass one: one wave per tile (workgroups of four).
+__global__ void __launch_bounds__(256) k_parse_spec(ParseArgs a, ParseLayout pl)
+{
+    __shared__ uint32_t buf[4][kParseTile / 4 + 2];
+    const uint32_t wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint64_t gt = (uint64_t) blockIdx.x * 4 + wv;
+    if (gt >= pl.ntiles)
+        return;
+    const uint32_t c = (uint32_t) (gt / pl.tpc), t = (uint32_t) (gt % pl.tpc);
+    uint64_t cs, ce;
+    if (!parse_region(a, pl, c, cs, ce))
+        return;  // pass two reports the chunk
+    const uint64_t ts = cs + (uint64_t) t * kParseTile;
+    if (ts >= ce)
+        return;  // past the chunk's region
+    const uint64_t te = ts + kParseTile < ce ? ts + kParseTile : ce;
+    parse_stage(a, ts, te, buf[wv]);
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    const uint8_t* tb = (const uint8_t*) buf[wv] + (ts & 3);
+    const uint32_t len = (uint32_t) (te - ts);
+    uint32_t* table = (uint32_t*) ((char*) a.ws + pl.table_off) + gt * pl.maxb;
+    uint32_t lo = 0xFFFFFFFFu, hi = 0;
+    for (uint32_t e0 = 0; e0 < pl.maxb; e0 += 64) {
+        const uint32_t e = e0 + lane;
+        uint32_t pos = e, cnt = 0;
+        bool live = e < pl.maxb, dead = false;
+        // walk while some lane's candidate is inside the tile
+        while (__any(live && pos < len)) {
+            if (live && pos < len) {
+                const uint32_t w = tb[pos];
+                if (w > 32) {
+                    dead = true;
+                    live = false;
+                } else {
+                    pos += parse_block_bytes(w, a.g.L);
+                    cnt++;
+                }
+            }
+        }
+        if (e < pl.maxb) {
+            // blocks walked, and the exit past the tile's end (kParseNoExit
+            // when the walk met a width byte above 32: past the last chunk's
+            // blocks that is the next segment's seed, not an error)
+            const uint32_t ex = pos - len;
+            table[e] = (cnt << 16) | (dead ? kParseNoExit : ex);
+            if (!dead) {
+                lo = min(lo, ex);
+                hi = max(hi, ex);
+            }
+        }
+    }
+    // one exit for every surviving candidate: the next tile's entry
+    for (int o = 32; o >= 1; o >>= 1) {
+        lo = min(lo, (uint32_t) __shfl_xor((int) lo, o));
+        hi = max(hi, (uint32_t) __shfl_xor((int) hi, o));
+    }
+    if (lane == 0)
+        ((uint32_t*) ((char*) a.ws + pl.conv_off))[gt] = lo == hi ? lo : kParseDead;
+}
+
+// Pass two: one workgroup per chunk.  Thread i takes a run of consecutive
+// tiles, finds the entry of the first (from the nearest earlier tile with one
+// exit, or the chunk's start), follows the tables through its run, then the
+// runs' block counts are scanned into each tile's first block index.
+constexpr uint32_t kParseResolveThreads = 1024;
+
+__global__ void __launch_bounds__(kParseResolveThreads) k_parse_resolve(ParseArgs a, ParseLayout pl)
+{
+    __shared__ uint32_t s_w[kParseResolveThreads / 64];
+    const uint32_t c = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    uint64_t cs, ce;
+    const bool ok = parse_region(a, pl, c, cs, ce);
+    if (!ok && tid == 0)
+        parse_fail(a, pl);
+    const uint32_t ntc = ok ? (uint32_t) ceil_div(ce - cs, kParseTile) : 0u;
+    if (ok && ntc == 0 && tid == 0)
+        parse_fail(a, pl);  // no bytes for the chunk's blocks
+    const uint32_t per = (uint32_t) ceil_div(ntc, kParseResolveThreads);
+    const uint32_t t0 = min(tid * per, ntc), t1 = min(t0 + per, ntc);
+    const uint64_t g0 = (uint64_t) c * pl.tpc;
+    const uint32_t* table = (const uint32_t*) ((const char*) a.ws + pl.table_off);
+    const uint32_t* conv = (const uint32_t*) ((const char*) a.ws + pl.conv_off);
+    uint32_t* entry = (uint32_t*) ((char*) a.ws + pl.entry_off);
+    uint32_t* base = (uint32_t*) ((char*) a.ws + pl.base_off);
+    uint32_t sum = 0;
+    if (t0 < t1) {
+        // entry of tile t0
+        int u = (int) t0 - 1;
+        while (u >= 0 && conv[g0 + u] == kParseDead)
+            u--;
+        uint32_t e = u < 0 ? 0u : conv[g0 + u];
+        bool dead = false;
+        for (uint32_t v = (uint32_t) (u + 1); v < t0 && !dead; v++) {
+            e = table[(g0 + v) * pl.maxb + e] & 0xFFFFu;
+            dead = e >= pl.maxb;
+        }
+        // a tile after one whose walk found no exit has no entry: past the
+        // last chunk's blocks, or (if the chunk's blocks need it) corrupt
+        for (uint32_t t = t0; t < t1; t++) {
+            uint32_t cnt = 0;
+            if (!dead) {
+                const uint32_t x = table[(g0 + t) * pl.maxb + e];
+                cnt = x >> 16;
+                entry[g0 + t] = e;
+                e = x & 0xFFFFu;
+                dead = e >= pl.maxb;
+            } else {
+                entry[g0 + t] = kParseDead;
+            }
+            base[g0 + t] = cnt;
+            sum += cnt;
+        }
+    }
+    // exclusive scan of the runs' counts (a chunk holds fewer than 2^32 blocks)
+    const uint32_t incl = wave_scan_dpp(sum);
+    if (lane == 63)
+        s_w[wv] = incl;
+    __syncthreads();
+    uint32_t before = incl - sum;
+    for (uint32_t i = 0; i < wv; i++)
+        before += s_w[i];
+    for (uint32_t t = t0; t < t1; t++) {
+        const uint32_t cnt = base[g0 + t];
+        base[g0 + t] = before;
+        before += cnt;
+    }
+}
+
+// Pass three: one wave per tile; lane 0 walks the tile from its entry.
+__global__ void __launch_bounds__(256) k_parse_emit(ParseArgs a, ParseLayout pl)
+{
+    __shared__ uint32_t buf[4][kParseTile / 4 + 2];
+    const uint32_t wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint64_t gt = (uint64_t) blockIdx.x * 4 + wv;
+    if (gt >= pl.ntiles)
+        return;
+    const Geom& g = a.g;
+    const uint32_t c = (uint32_t) (gt / pl.tpc), t = (uint32_t) (gt % pl.tpc);
+    uint64_t cs, ce;
+    if (!parse_region(a, pl, c, cs, ce))
+        return;
+    const uint64_t ts = cs + (uint64_t) t * kParseTile;
+    if (ts >= ce)
+        return;
+    const uint64_t te = ts + kParseTile < ce ? ts + kParseTile : ce;
+    const uint32_t b = ((const uint32_t*) ((const char*) a.ws + pl.base_off))[gt];
+    const uint32_t nb = g.nbT;
+    if (b >= nb)
+        return;  // past the chunk's blocks
+    IndexView ix = index_view(g, a.index);
+    const uint32_t e = ((const uint32_t*) ((const char*) a.ws + pl.entry_off))[gt];
+    if (e == kParseDead) {
+        if (lane == 0)
+            parse_fail(a, pl);
+        return;
+    }
+    parse_stage(a, ts, te, buf[wv]);
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    if (lane != 0)
+        return;
+    const uint8_t* tb = (const uint8_t*) buf[wv] + (ts & 3);
+    const uint64_t b0 = (uint64_t) c * g.nbT, grp0 = (uint64_t) c * g.gT;
+    if (t == 0)
+        ix.seed_off[c] = cs - 4;
+    const uint32_t len = (uint32_t) (te - ts);
+    uint32_t pos = e, j = b;
+    while (pos < len && j < nb) {
+        const uint32_t w = tb[pos];
+        if (w > 32) {
+            parse_fail(a, pl);
+            return;
+        }
+        ix.widths[b0 + j] = (uint8_t) w;
+        if (j % g.IG == 0)
+            ix.group_off[grp0 + j / g.IG] = ts + pos;
+        const uint32_t left = g.tb - 1 - j * g.L;
+        pos += block_bytes(w, left < g.L ? left : g.L);
+        j++;
+    }
+    if (j < nb) {
+        if (te == ce)
+            parse_fail(a, pl);  // the region ended before the chunk's blocks did
+        return;                                 // the chunk goes on in the next tile
+    }
+    // the chunk's last block ends here
+    uint64_t end = ts + pos;
+    if (c + 1 < g.T) {
+        if (end != ce)
+            parse_fail(a, pl);
+        return;
+    }
+    if (g.R) {
+        // the remainder segment (fewer than T values): seed, then its blocks
+        ix.seed_off[g.T] = end;
+        end += 4;
+        const uint64_t rb0 = (uint64_t) g.T * g.nbT, rgrp0 = (uint64_t) g.T * g.gT;
+        if (g.nbR == 0)
+            ix.group_off[rgrp0] = end;
+        for (uint32_t jb = 0; jb < g.nbR; jb++) {
+            if (end >= a.in_cap) {
+                parse_fail(a, pl);
+                return;
+            }
+            const uint32_t w = a.bytes[end];
+            if (w > 32) {
+                parse_fail(a, pl);
+                return;
+            }
+            ix.widths[rb0 + jb] = (uint8_t) w;
+            if (jb % g.IG == 0)
+                ix.group_off[rgrp0 + jb / g.IG] = end;
+            const uint32_t left = g.R - 1 - jb * g.L;
+            end += block_bytes(w, left < g.L ? left : g.L);
+        }
+    }
+    if (end > a.in_cap) {
+        parse_fail(a, pl);
+        return;
+    }
+    ix.meta[0] = end;
+    if (a.size_out)
+        *a.size_out = end;
+}
+
+// After pass three: a stream found corrupt anywhere keeps no size.
+__global__ void k_parse_finish(ParseArgs a, ParseLayout pl)
+{
+    if (threadIdx.x == 0 && *(const unsigned int*) ((const char*) a.ws + pl.flag_off)) {
+        *(unsigned long long*) (a.index + a.g.ix_meta) = 0;
+        if (a.size_out)
+            *a.size_out = 0;
+    }
+}
+
 // ------------------------------------------------------------ case counts
 
 // Block-case histogram of a pairwise add, the statistics the reference's
@@ -1671,7 +1974,24 @@ hipError_t launch_parse(const ParseArgs& a, hipStream_t s)
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, (unsigned long long*) (a.index + a.g.ix_meta), 1ull, 0ull);
     if (a.size_out)
         hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, a.size_out, 1ull, 0ull);
-    hipLaunchKernelGGL(k_parse, dim3(a.g.T), dim3(64), 0, s, a);
+    const ParseLayout pl = parse_layout(a.g);
+    if (parse_parallel_ok(a.g) && a.ws_bytes >= pl.bytes) {
+        const uint32_t groups = (uint32_t) ceil_div(pl.ntiles, 4);
+        // the flag word shares the workspace with lookback status words that
+        // earlier launches leave behind: clear it first
+        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, (unsigned long long*) ((char*) a.ws + pl.flag_off), 2ull,
+            0ull);
+        hipLaunchKernelGGL(k_parse_spec, dim3(groups), dim3(256), 0, s, a, pl);
+        hipLaunchKernelGGL(k_parse_resolve, dim3(a.g.T), dim3(kParseResolveThreads), 0, s, a, pl);
+        hipLaunchKernelGGL(k_parse_emit, dim3(groups), dim3(256), 0, s, a, pl);
+        hipLaunchKernelGGL(k_parse_finish, dim3(1), dim3(64), 0, s, a, pl);
+        // the flag and tables sat in the workspace's lookback area: zero it again
+        const uint64_t words = (pl.bytes - pl.flag_off) / 8;
+        hipLaunchKernelGGL(k_fill_u64, dim3((uint32_t) std::min<uint64_t>(ceil_div(words, 256), 4096)), dim3(256), 0, s,
+            (unsigned long long*) ((char*) a.ws + pl.flag_off), words, 0ull);
+    } else {
+        hipLaunchKernelGGL(k_parse, dim3(a.g.T), dim3(64), 0, s, a);
+    }
     return hipGetLastError();
 }
 
