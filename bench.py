@@ -307,8 +307,7 @@ def dropin_leg(a, b, eb, L, T, timed):
             bool(torch.equal(out[: size.value], codec.homsum([ca, cb]).data[: size.value]))
         t_dec = timed(dec, 10)
         t_add = timed(add, 10)
-        ws = codec.default_workspace(codec.num_tiles(n, L, T))
-        t_parse = timed(lambda: codec.from_bytes(ca.data, n, L, T, ws=ws), 10)
+        t_parse = timed(lambda: codec.from_bytes(ca.data, n, L, T), 10)
         t_kdec = timed(lambda: codec.decompress(ca, eb, out=y), 10)
     finally:
         lib().zccl_set_num_chunks(0)
