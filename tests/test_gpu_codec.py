@@ -266,7 +266,7 @@ def check_full_index(s, cx):
 PARSE_CASES = [("walk", 100_003, 1e-2, 36, 16), ("sines", 5000, 1e-4, 7, 3), ("walk", 17, 1e-2, 36, 16),
                ("zeros", 999, 1e-2, 36, 4), ("cfg1", 1 << 24, 1e-4, 36, 16), ("grf", 3_000_017, 1e-3, 36, 7),
                ("slabs", 2_000_000, 1e-2, 36, 8), ("wide", 600_000, 1.0, 36, 2), ("walk", 700_001, 1e-3, 64, 5),
-               ("sines", 400_000, 1e-4, 8, 3), ("grf", 1_000_000, 1e-3, 36, 1)]
+               ("sines", 400_000, 1e-4, 8, 3), ("grf", 1_000_000, 1e-3, 36, 1), ("zeros", 2_000_000, 1e-2, 36, 4)]
 
 
 @pytest.mark.parametrize("kind,n,eb,L,T", PARSE_CASES)
@@ -293,6 +293,27 @@ def test_parse_cpu_stream(codec, kind, n, eb, L, T):
     torch.cuda.synchronize()
     codec.check_errors(small)
     check_full_index(s2, cx)
+
+
+@pytest.mark.parametrize("fill", ["zeros", "width1", "random"])
+def test_parse_stream_in_larger_buffer(codec, fill):
+    """A foreign stream at the start of a worst-case-sized buffer (the
+    reference ABI passes no stream size): whatever follows the stream (zero
+    bytes, which read as one-byte blocks; bytes of 1, which read as 11-byte
+    blocks; random bytes) leaves the rebuilt index and the size exactly the
+    oracle's."""
+    n, eb, L, T = 1_500_007, 1e-3, 36, 6
+    say("parse in buffer", fill, n, L, T)
+    x = gen.make("grf", n, 11)
+    cx = O.compress(x, eb, L, T)
+    extra = codec.worst_case_bytes(n, L, T) - cx.size
+    rng = np.random.default_rng(3)
+    tail = {"zeros": np.zeros(extra, np.uint8), "width1": np.ones(extra, np.uint8),
+            "random": rng.integers(0, 256, extra, dtype=np.uint8)}[fill]
+    s = codec.upload(np.concatenate([cx, tail]), n, L, T)
+    torch.cuda.synchronize()
+    codec.check_errors()
+    check_full_index(s, cx)
 
 
 def test_parallel_parse_rejects_corrupt_streams(codec):

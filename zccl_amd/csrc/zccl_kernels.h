@@ -134,7 +134,7 @@ struct ParseArgs {
 // block offsets (0 .. block_bytes(32, L) - 1), then per tile its convergence
 // word, entry and block count / base.  The tables live in the workspace
 // after its header and are zeroed again when the parse ends.
-constexpr uint32_t kParseTile = 4096;
+constexpr uint32_t kParseTile = 2048;
 constexpr uint32_t kParseMaxL = 64;         // at most 265 candidates per tile
 constexpr uint32_t kParseMinBlocks = 256;   // chunks of fewer blocks take the serial walk
 constexpr uint32_t kParseDead = 0xFFFFFFFFu;   // no entry / no common exit

@@ -1537,8 +1537,8 @@ __global__ void __launch_bounds__(64) k_parse(ParseArgs a)
 // when a width byte exceeds 32).  Walks from different offsets merge within a
 // few blocks, so a tile's surviving candidates usually share one exit and
 // the next tile's entry is known without this tile's.  Pass two (a
-// workgroup per chunk) chains entries from the nearest such tile and scans
-// the block counts; pass three walks each tile from its entry and writes the
+// workgroup per chunk) chains entries from the previous tile's common exit
+// and scans the block counts; pass three walks each tile from its entry and writes the
 // widths, group offsets, seeds, the chunk's last (possibly short) block, the
 // remainder segment and the stream size.  Same walk as the serial parse
 // (ZCCL/src/ZCCLd_float.c:597-745), same index.
@@ -1559,25 +1559,43 @@ __device__ __forceinline__ bool parse_region(const ParseArgs& a, const ParseLayo
     return cs <= ce && ce - cs <= (uint64_t) pl.tpc * kParseTile;
 }
 
-// Bytes [ts, te) of the stream into LDS (one wave), dword loads where the
-// dword is readable, bytes at the very end.
-__device__ __forceinline__ void parse_stage(const ParseArgs& a, uint64_t ts, uint64_t te, uint32_t* buf)
+// Bytes [ts, te) of the stream into LDS by NT threads: buf[0] holds the
+// 16-byte aligned bytes from ts & ~15 (16-byte loads, all issued before the
+// first LDS store, where the stream pointer is 16-byte aligned; dword loads
+// otherwise), bytes past the readable end read as zero.  Returns the staged
+// dwords.  Every thread of the group calls this.
+template <uint32_t NT>
+__device__ __forceinline__ uint32_t parse_stage(const ParseArgs& a, uint64_t ts, uint64_t te, uint32_t* buf,
+    uint32_t tid)
 {
-    const uint32_t lane = threadIdx.x & 63;
-    const uint64_t base = ts & ~3ull;
-    const uint32_t ndw = (uint32_t) ((te - base + 3) >> 2);
-    const bool aligned = ((uintptr_t) a.bytes & 3) == 0;
-    for (uint32_t i = lane; i < ndw; i += 64) {
-        const uint64_t q = base + 4ull * i;
-        uint32_t v = 0;
-        if (q + 4 <= a.in_cap) {
-            v = aligned ? *(const uint32_t*) (a.bytes + q) : load_u32_bytes(a.bytes + q);
-        } else {
-            for (uint32_t k = 0; k < 4 && q + k < a.in_cap; k++)
-                v |= (uint32_t) a.bytes[q + k] << (8 * k);
+    constexpr uint32_t K = (kParseTile / 16 + 1 + NT - 1) / NT;
+    const uint64_t base = ts & ~15ull;
+    const uint32_t n16 = (uint32_t) ((te - base + 15) >> 4);
+    const bool aligned = ((uintptr_t) a.bytes & 15) == 0;
+    uint4 v[K];
+#pragma unroll
+    for (uint32_t k = 0; k < K; k++) {
+        const uint32_t i = tid + k * NT;
+        const uint64_t q = base + 16ull * i;
+        v[k] = make_uint4(0, 0, 0, 0);
+        if (i < n16) {
+            if (aligned && q + 16 <= a.in_cap) {
+                v[k] = *(const uint4*) (a.bytes + q);
+            } else {
+                uint32_t d[4] = {0, 0, 0, 0};
+                for (uint32_t b = 0; b < 16 && q + b < a.in_cap; b++)
+                    d[b >> 2] |= (uint32_t) a.bytes[q + b] << (8 * (b & 3));
+                v[k] = make_uint4(d[0], d[1], d[2], d[3]);
+            }
         }
-        buf[i] = v;
     }
+#pragma unroll
+    for (uint32_t k = 0; k < K; k++) {
+        const uint32_t i = tid + k * NT;
+        if (i < n16)
+            reinterpret_cast<uint4*>(buf)[i] = v[k];
+    }
+    return 4 * n16;
 }
 
 // A corrupt stream: the sticky workspace error and the parse's own flag
@@ -1593,14 +1611,57 @@ __device__ __forceinline__ uint32_t parse_block_bytes(uint32_t w, uint32_t L)
     return w ? 1u + ((L + 7u) >> 3) + (w >> 3) * L + (((w & 7u) * L + 7u) >> 3) : 1u;
 }
 
-// Pass one: one wave per tile (workgroups of four).
+// One bit per nonzero dword of a wave's staged bytes (ndw dwords), built with
+// ballots: the walks below cross runs of zero bytes (width-0 blocks of one
+// byte each) a bitmap word at a time.
+constexpr uint32_t kParseNzWords = 2 * ((kParseTile / 4 + 4 + 63) / 64);
+
+__device__ __forceinline__ void parse_nonzero_map(const uint32_t* buf, uint32_t ndw, uint32_t* nzb)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    for (uint32_t i = 0; i < kParseNzWords / 2; i++) {
+        const uint32_t d = 64 * i + lane;
+        const unsigned long long m = __ballot(d < ndw && buf[d] != 0);
+        if (lane == 0) {
+            nzb[2 * i] = (uint32_t) m;
+            nzb[2 * i + 1] = (uint32_t) (m >> 32);
+        }
+    }
+}
+
+// Index of the first nonzero staged byte at or after byte q, or lim when every
+// byte in [q, lim) is zero (byte indices count from the staged buffer's start).
+__device__ __forceinline__ uint32_t parse_next_nonzero(const uint32_t* buf, const uint32_t* nzb, uint32_t q, uint32_t lim)
+{
+    if (q >= lim)
+        return lim;
+    const uint32_t d = q >> 2;
+    const uint32_t v = buf[d] >> (8 * (q & 3));
+    if (v)
+        return min(q + ((uint32_t) __builtin_ctz(v) >> 3), lim);
+    const uint32_t kend = (lim + 127) >> 7;  // bitmap words of the dwords below ceil(lim / 4)
+    uint32_t k = (d + 1) >> 5;
+    if (k >= kend)
+        return lim;
+    uint32_t m = nzb[k] & (~0u << ((d + 1) & 31));
+    while (!m) {
+        if (++k >= kend)
+            return lim;
+        m = nzb[k];
+    }
+    const uint32_t d2 = 32 * k + (uint32_t) __builtin_ctz(m);
+    return min(4 * d2 + ((uint32_t) __builtin_ctz(buf[d2]) >> 3), lim);
+}
+
+// Pass one: one workgroup per tile; wave w walks candidates 64w .. 64w + 63
+// (and 256 further on for the widest blocks).
 __global__ void __launch_bounds__(256) k_parse_spec(ParseArgs a, ParseLayout pl)
 {
-    __shared__ uint32_t buf[4][kParseTile / 4 + 2];
-    const uint32_t wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const uint64_t gt = (uint64_t) blockIdx.x * 4 + wv;
-    if (gt >= pl.ntiles)
-        return;
+    __shared__ uint32_t buf[kParseTile / 4 + 5];  // one dword past the staged bytes for the 8-byte window
+    __shared__ uint32_t nzb[kParseNzWords];
+    __shared__ uint32_t s_lo[4], s_hi[4];
+    const uint32_t tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+    const uint64_t gt = blockIdx.x;
     const uint32_t c = (uint32_t) (gt / pl.tpc), t = (uint32_t) (gt % pl.tpc);
     uint64_t cs, ce;
     if (!parse_region(a, pl, c, cs, ce))
@@ -1609,27 +1670,48 @@ __global__ void __launch_bounds__(256) k_parse_spec(ParseArgs a, ParseLayout pl)
     if (ts >= ce)
         return;  // past the chunk's region
     const uint64_t te = ts + kParseTile < ce ? ts + kParseTile : ce;
-    parse_stage(a, ts, te, buf[wv]);
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-    const uint8_t* tb = (const uint8_t*) buf[wv] + (ts & 3);
+    const uint32_t ndw = parse_stage<256>(a, ts, te, buf, tid);
+    __syncthreads();
+    if (wv == 0)
+        parse_nonzero_map(buf, ndw, nzb);
+    __syncthreads();
+    const uint32_t off = (uint32_t) (ts & 15);
+    const uint8_t* tb = (const uint8_t*) buf + off;
     const uint32_t len = (uint32_t) (te - ts);
+    const uint32_t hb = 1u + ((a.g.L + 7u) >> 3);
     uint32_t* table = (uint32_t*) ((char*) a.ws + pl.table_off) + gt * pl.maxb;
     uint32_t lo = 0xFFFFFFFFu, hi = 0;
-    for (uint32_t e0 = 0; e0 < pl.maxb; e0 += 64) {
+    for (uint32_t e0 = 64 * wv; e0 < pl.maxb; e0 += 256) {
         const uint32_t e = e0 + lane;
-        uint32_t pos = e, cnt = 0;
-        bool live = e < pl.maxb, dead = false;
-        // walk while some lane's candidate is inside the tile
-        while (__any(live && pos < len)) {
-            if (live && pos < len) {
-                const uint32_t w = tb[pos];
-                if (w > 32) {
-                    dead = true;
-                    live = false;
-                } else {
-                    pos += parse_block_bytes(w, a.g.L);
-                    cnt++;
+        // one step per loop trip for every lane still inside the tile: the
+        // byte at pos and the 8-byte window from it; a width byte above 32
+        // ends the walk (dead), a zero byte is a one-byte block and the
+        // window's run of zero bytes is taken at once; a run that fills the
+        // window goes on through the nonzero-dword bitmap
+        uint32_t pos = e < pl.maxb ? e : len, cnt = 0;
+        bool dead = false;
+        while (true) {
+            const bool act = pos < len;
+            if (!__any(act))
+                break;
+            const uint32_t q = off + (act ? pos : 0u);
+            const uint64_t x = ((uint64_t) buf[(q >> 2) + 1] << 32 | buf[q >> 2]) >> (8 * (q & 3));
+            const uint32_t w = (uint32_t) x & 0xFFu;
+            const uint32_t span = 8u - (q & 3);
+            const uint32_t zr = x ? (uint32_t) __builtin_ctzll(x) >> 3 : span;
+            const uint32_t wc = min(w, 32u);
+            const uint32_t bsz = hb + __umul24(wc >> 3, a.g.L) + ((__umul24(wc & 7u, a.g.L) + 7u) >> 3);
+            const uint32_t z = min(zr, len - min(pos, len));
+            const bool bad = act && w > 32;
+            dead |= bad;
+            cnt += act && !bad ? (w ? 1u : z) : 0u;
+            pos = act ? (bad ? len : pos + (w ? bsz : z)) : pos;
+            const bool longz = act && w == 0 && zr == span && pos < len;
+            if (__any(longz)) {
+                if (longz) {
+                    const uint32_t nz = parse_next_nonzero(buf, nzb, off + pos, off + len) - off;
+                    cnt += nz - pos;
+                    pos = nz;
                 }
             }
         }
@@ -1650,20 +1732,50 @@ __global__ void __launch_bounds__(256) k_parse_spec(ParseArgs a, ParseLayout pl)
         lo = min(lo, (uint32_t) __shfl_xor((int) lo, o));
         hi = max(hi, (uint32_t) __shfl_xor((int) hi, o));
     }
-    if (lane == 0)
+    if (lane == 0) {
+        s_lo[wv] = lo;
+        s_hi[wv] = hi;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        lo = min(min(s_lo[0], s_lo[1]), min(s_lo[2], s_lo[3]));
+        hi = max(max(s_hi[0], s_hi[1]), max(s_hi[2], s_hi[3]));
         ((uint32_t*) ((char*) a.ws + pl.conv_off))[gt] = lo == hi ? lo : kParseDead;
+    }
+}
+
+// Exclusive scan over a workgroup of kParseResolveThreads threads.
+constexpr uint32_t kParseResolveThreads = 1024;
+
+__device__ __forceinline__ uint32_t parse_block_excl(uint32_t v, uint32_t* s_w)
+{
+    const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const uint32_t incl = wave_scan_dpp(v);
+    __syncthreads();  // s_w free
+    if (lane == 63)
+        s_w[wv] = incl;
+    __syncthreads();
+    uint32_t before = incl - v;
+    for (uint32_t i = 0; i < wv; i++)
+        before += s_w[i];
+    return before;
 }
 
 // Pass two: one workgroup per chunk.  Thread i takes a run of consecutive
-// tiles, finds the entry of the first (from the nearest earlier tile with one
-// exit, or the chunk's start), follows the tables through its run, then the
-// runs' block counts are scanned into each tile's first block index.
-constexpr uint32_t kParseResolveThreads = 1024;
-
+// tiles; its first tile's entry is the previous tile's common exit (tile 0:
+// the region's start), and the tables carry it through the run.  A run whose
+// previous tile has no common exit waits: when the resolved runs before it
+// hold fewer blocks than the chunk, one thread follows the tables from the
+// last known exit through the runs in order until the chunk's blocks are
+// covered.  The runs' block counts are then scanned into each tile's first
+// block index.
 __global__ void __launch_bounds__(kParseResolveThreads) k_parse_resolve(ParseArgs a, ParseLayout pl)
 {
     __shared__ uint32_t s_w[kParseResolveThreads / 64];
-    const uint32_t c = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    __shared__ uint32_t s_exit[kParseResolveThreads];
+    __shared__ uint32_t s_sum[kParseResolveThreads];
+    __shared__ uint32_t s_first, s_fix;
+    const uint32_t c = blockIdx.x, tid = threadIdx.x;
     uint64_t cs, ce;
     const bool ok = parse_region(a, pl, c, cs, ce);
     if (!ok && tid == 0)
@@ -1672,49 +1784,76 @@ __global__ void __launch_bounds__(kParseResolveThreads) k_parse_resolve(ParseArg
     if (ok && ntc == 0 && tid == 0)
         parse_fail(a, pl);  // no bytes for the chunk's blocks
     const uint32_t per = (uint32_t) ceil_div(ntc, kParseResolveThreads);
-    const uint32_t t0 = min(tid * per, ntc), t1 = min(t0 + per, ntc);
+    const uint32_t nb = a.g.nbT;
     const uint64_t g0 = (uint64_t) c * pl.tpc;
     const uint32_t* table = (const uint32_t*) ((const char*) a.ws + pl.table_off);
     const uint32_t* conv = (const uint32_t*) ((const char*) a.ws + pl.conv_off);
     uint32_t* entry = (uint32_t*) ((char*) a.ws + pl.entry_off);
     uint32_t* base = (uint32_t*) ((char*) a.ws + pl.base_off);
-    uint32_t sum = 0;
-    if (t0 < t1) {
-        // entry of tile t0
-        int u = (int) t0 - 1;
-        while (u >= 0 && conv[g0 + u] == kParseDead)
-            u--;
-        uint32_t e = u < 0 ? 0u : conv[g0 + u];
-        bool dead = false;
-        for (uint32_t v = (uint32_t) (u + 1); v < t0 && !dead; v++) {
-            e = table[(g0 + v) * pl.maxb + e] & 0xFFFFu;
-            dead = e >= pl.maxb;
-        }
-        // a tile after one whose walk found no exit has no entry: past the
-        // last chunk's blocks, or (if the chunk's blocks need it) corrupt
-        for (uint32_t t = t0; t < t1; t++) {
+    // follow the tables from entry e through tiles [r0, r1): entries and
+    // counts written, the block total returned, e left at the exit
+    // (kParseNoExit once a tile had no exit: the later tiles get no entry)
+    auto chain = [&](uint32_t r0, uint32_t r1, uint32_t& e) {
+        uint32_t s = 0;
+        for (uint32_t t = r0; t < r1; t++) {
             uint32_t cnt = 0;
-            if (!dead) {
+            if (e < pl.maxb) {
                 const uint32_t x = table[(g0 + t) * pl.maxb + e];
                 cnt = x >> 16;
                 entry[g0 + t] = e;
                 e = x & 0xFFFFu;
-                dead = e >= pl.maxb;
             } else {
                 entry[g0 + t] = kParseDead;
             }
             base[g0 + t] = cnt;
-            sum += cnt;
+            s += cnt;
+        }
+        return s;
+    };
+    if (tid == 0) {
+        s_first = 0xFFFFFFFFu;
+        s_fix = 0;
+    }
+    __syncthreads();
+    const uint32_t t0 = min(tid * per, ntc), t1 = min(t0 + per, ntc);
+    uint32_t sum = 0, e = 0;
+    if (t0 < t1) {
+        if (t0 > 0)
+            e = conv[g0 + t0 - 1];
+        if (e == kParseDead) {
+            for (uint32_t t = t0; t < t1; t++) {
+                entry[g0 + t] = kParseDead;
+                base[g0 + t] = 0;
+            }
+            atomicMin(&s_first, tid);
+            e = kParseNoExit;
+        } else {
+            sum = chain(t0, t1, e);
         }
     }
-    // exclusive scan of the runs' counts (a chunk holds fewer than 2^32 blocks)
-    const uint32_t incl = wave_scan_dpp(sum);
-    if (lane == 63)
-        s_w[wv] = incl;
+    s_exit[tid] = e;
+    s_sum[tid] = sum;
+    uint32_t before = parse_block_excl(sum, s_w);
+    const uint32_t first = s_first;
+    if (tid == first && before < nb)
+        s_fix = 1;
     __syncthreads();
-    uint32_t before = incl - sum;
-    for (uint32_t i = 0; i < wv; i++)
-        before += s_w[i];
+    if (s_fix) {
+        // rare in valid streams (a tile whose candidate walks kept more than
+        // one exit): serial, up to the end of the chunk's blocks
+        if (tid == 0) {
+            uint32_t x = s_exit[first - 1], run = 0;
+            for (uint32_t i = 0; i < first; i++)
+                run += s_sum[i];
+            for (uint32_t r = first; r < kParseResolveThreads && run < nb && x < pl.maxb; r++) {
+                const uint32_t r0 = min(r * per, ntc), r1 = min(r0 + per, ntc);
+                s_sum[r] = chain(r0, r1, x);
+                run += s_sum[r];
+            }
+        }
+        __syncthreads();
+        before = parse_block_excl(s_sum[tid], s_w);
+    }
     for (uint32_t t = t0; t < t1; t++) {
         const uint32_t cnt = base[g0 + t];
         base[g0 + t] = before;
@@ -1722,11 +1861,17 @@ __global__ void __launch_bounds__(kParseResolveThreads) k_parse_resolve(ParseArg
     }
 }
 
-// Pass three: one wave per tile; lane 0 walks the tile from its entry.
+// Pass three: one wave per tile, walking the tile from its entry.  The walk is
+// the same in every lane; each block's position and width go to the lane of
+// its place in a batch of 64, and a full batch (or the end) writes its widths
+// and group offsets at once.  A run of zero bytes (one-byte width-0 blocks) is
+// written by the whole wave.
 __global__ void __launch_bounds__(256) k_parse_emit(ParseArgs a, ParseLayout pl)
 {
-    __shared__ uint32_t buf[4][kParseTile / 4 + 2];
-    const uint32_t wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    __shared__ uint32_t buf[4][kParseTile / 4 + 4];
+    __shared__ uint32_t nzb[4][kParseNzWords];
+    // (the wave index read as uniform: the walk below then runs on scalar registers)
+    const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const uint64_t gt = (uint64_t) blockIdx.x * 4 + wv;
     if (gt >= pl.ntiles)
         return;
@@ -1750,34 +1895,75 @@ __global__ void __launch_bounds__(256) k_parse_emit(ParseArgs a, ParseLayout pl)
             parse_fail(a, pl);
         return;
     }
-    parse_stage(a, ts, te, buf[wv]);
+    const uint32_t ndw = parse_stage<64>(a, ts, te, buf[wv], lane);
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-    if (lane != 0)
-        return;
-    const uint8_t* tb = (const uint8_t*) buf[wv] + (ts & 3);
+    parse_nonzero_map(buf[wv], ndw, nzb[wv]);
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    const uint32_t off = (uint32_t) (ts & 15);
+    const uint8_t* tb = (const uint8_t*) buf[wv] + off;
     const uint64_t b0 = (uint64_t) c * g.nbT, grp0 = (uint64_t) c * g.gT;
-    if (t == 0)
+    if (t == 0 && lane == 0)
         ix.seed_off[c] = cs - 4;
     const uint32_t len = (uint32_t) (te - ts);
-    uint32_t pos = e, j = b;
-    while (pos < len && j < nb) {
-        const uint32_t w = tb[pos];
-        if (w > 32) {
-            parse_fail(a, pl);
-            return;
+    const uint32_t hb = 1u + ((g.L + 7u) >> 3);
+    const uint32_t last_len = g.tb - 1 - (nb - 1) * g.L;  // values in the chunk's last block
+    uint32_t pos = e, j = b, k = 0;
+    uint32_t my_pos = 0, my_j = 0, my_w = 0;
+    bool bad = false;
+    auto flush = [&]() {
+        if (lane < k) {
+            ix.widths[b0 + my_j] = (uint8_t) my_w;
+            if (my_j % g.IG == 0)
+                ix.group_off[grp0 + my_j / g.IG] = ts + my_pos;
         }
-        ix.widths[b0 + j] = (uint8_t) w;
-        if (j % g.IG == 0)
-            ix.group_off[grp0 + j / g.IG] = ts + pos;
-        const uint32_t left = g.tb - 1 - j * g.L;
-        pos += block_bytes(w, left < g.L ? left : g.L);
+        k = 0;
+    };
+    while (pos < len && j < nb) {
+        const uint32_t w = __builtin_amdgcn_readfirstlane(tb[pos]);
+        if (w > 32) {
+            bad = true;
+            break;
+        }
+        if (w == 0) {
+            const uint32_t nz = parse_next_nonzero(buf[wv], nzb[wv], off + pos + 1, off + len) - off;
+            const uint32_t r = min(nz - pos, nb - j);
+            flush();
+            for (uint32_t i = lane; i < r; i += 64) {
+                const uint32_t jj = j + i;
+                ix.widths[b0 + jj] = 0;
+                if (jj % g.IG == 0)
+                    ix.group_off[grp0 + jj / g.IG] = ts + pos + i;
+            }
+            pos += r;
+            j += r;
+            continue;
+        }
+        if (lane == k) {
+            my_pos = pos;
+            my_j = j;
+            my_w = w;
+        }
+        if (j == nb - 1)
+            pos += block_bytes(w, last_len);
+        else
+            pos += hb + __umul24(w >> 3, g.L) + ((__umul24(w & 7u, g.L) + 7u) >> 3);
         j++;
+        if (++k == 64)
+            flush();
+    }
+    flush();
+    if (lane != 0)
+        return;
+    if (bad) {
+        parse_fail(a, pl);
+        return;
     }
     if (j < nb) {
         if (te == ce)
             parse_fail(a, pl);  // the region ended before the chunk's blocks did
-        return;                                 // the chunk goes on in the next tile
+        return;                 // the chunk goes on in the next tile
     }
     // the chunk's last block ends here
     uint64_t end = ts + pos;
@@ -1819,14 +2005,40 @@ __global__ void __launch_bounds__(256) k_parse_emit(ParseArgs a, ParseLayout pl)
         *a.size_out = end;
 }
 
-// After pass three: a stream found corrupt anywhere keeps no size.
-__global__ void k_parse_finish(ParseArgs a, ParseLayout pl)
+// Before pass one: no stream size yet, and the flag clear (the flag and the
+// tables share the workspace with lookback status words of earlier launches).
+__global__ void k_parse_begin(ParseArgs a, ParseLayout pl)
 {
-    if (threadIdx.x == 0 && *(const unsigned int*) ((const char*) a.ws + pl.flag_off)) {
+    if (threadIdx.x == 0) {
         *(unsigned long long*) (a.index + a.g.ix_meta) = 0;
         if (a.size_out)
             *a.size_out = 0;
+        *(unsigned long long*) ((char*) a.ws + pl.flag_off) = 0;
     }
+}
+
+// After pass three: a stream found corrupt anywhere keeps no size; then the
+// flag and tables are zeroed (the lookback area is zero between launches).
+// Workgroup 0 reads the flag before clearing it; the others clear the rest.
+__global__ void k_parse_finish(ParseArgs a, ParseLayout pl)
+{
+    unsigned long long* w = (unsigned long long*) ((char*) a.ws + pl.flag_off);
+    const uint64_t words = (pl.bytes - pl.flag_off) / 8;
+    if (blockIdx.x == 0) {
+        if (threadIdx.x == 0) {
+            if (*(const unsigned int*) w) {
+                *(unsigned long long*) (a.index + a.g.ix_meta) = 0;
+                if (a.size_out)
+                    *a.size_out = 0;
+            }
+            w[0] = 0;
+            w[1] = 0;
+        }
+        return;
+    }
+    for (uint64_t i = 2 + (uint64_t) (blockIdx.x - 1) * blockDim.x + threadIdx.x; i < words;
+         i += (uint64_t) (gridDim.x - 1) * blockDim.x)
+        w[i] = 0;
 }
 
 // ------------------------------------------------------------ case counts
@@ -1968,30 +2180,25 @@ hipError_t launch_batch(BatchKind kind, const BatchArgs& b, uint32_t threads, si
 
 hipError_t launch_parse(const ParseArgs& a, hipStream_t s)
 {
+    const ParseLayout pl = parse_layout(a.g);
+    if (parse_parallel_ok(a.g) && a.ws_bytes >= pl.bytes) {
+        const uint32_t groups = (uint32_t) ceil_div(pl.ntiles, 4);
+        const uint64_t words = (pl.bytes - pl.flag_off) / 8;
+        hipLaunchKernelGGL(k_parse_begin, dim3(1), dim3(64), 0, s, a, pl);
+        hipLaunchKernelGGL(k_parse_spec, dim3((uint32_t) pl.ntiles), dim3(256), 0, s, a, pl);
+        hipLaunchKernelGGL(k_parse_resolve, dim3(a.g.T), dim3(kParseResolveThreads), 0, s, a, pl);
+        hipLaunchKernelGGL(k_parse_emit, dim3(groups), dim3(256), 0, s, a, pl);
+        hipLaunchKernelGGL(k_parse_finish, dim3(1 + (uint32_t) std::min<uint64_t>(ceil_div(words, 256), 4096)), dim3(256),
+            0, s, a, pl);
+        return hipGetLastError();
+    }
     // the stream size starts at 0 (a fill kernel, safe under graph capture):
     // only a parse that reaches the stream's end writes it, so a failed parse
     // leaves no stale size for later kernels to read up to
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, (unsigned long long*) (a.index + a.g.ix_meta), 1ull, 0ull);
     if (a.size_out)
         hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, a.size_out, 1ull, 0ull);
-    const ParseLayout pl = parse_layout(a.g);
-    if (parse_parallel_ok(a.g) && a.ws_bytes >= pl.bytes) {
-        const uint32_t groups = (uint32_t) ceil_div(pl.ntiles, 4);
-        // the flag word shares the workspace with lookback status words that
-        // earlier launches leave behind: clear it first
-        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, (unsigned long long*) ((char*) a.ws + pl.flag_off), 2ull,
-            0ull);
-        hipLaunchKernelGGL(k_parse_spec, dim3(groups), dim3(256), 0, s, a, pl);
-        hipLaunchKernelGGL(k_parse_resolve, dim3(a.g.T), dim3(kParseResolveThreads), 0, s, a, pl);
-        hipLaunchKernelGGL(k_parse_emit, dim3(groups), dim3(256), 0, s, a, pl);
-        hipLaunchKernelGGL(k_parse_finish, dim3(1), dim3(64), 0, s, a, pl);
-        // the flag and tables sat in the workspace's lookback area: zero it again
-        const uint64_t words = (pl.bytes - pl.flag_off) / 8;
-        hipLaunchKernelGGL(k_fill_u64, dim3((uint32_t) std::min<uint64_t>(ceil_div(words, 256), 4096)), dim3(256), 0, s,
-            (unsigned long long*) ((char*) a.ws + pl.flag_off), words, 0ull);
-    } else {
-        hipLaunchKernelGGL(k_parse, dim3(a.g.T), dim3(64), 0, s, a);
-    }
+    hipLaunchKernelGGL(k_parse, dim3(a.g.T), dim3(64), 0, s, a);
     return hipGetLastError();
 }
 
