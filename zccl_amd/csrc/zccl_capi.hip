@@ -124,13 +124,14 @@ extern "C" int zccl_tune_threads(int op, int threads)
 
 extern "C" int zccl_tune_get_threads(int op) { return op < 0 || op > 2 ? 0 : g_threads[op].load(); }
 
-// Layout of the homomorphic sum for 36-value blocks: 0 one lane per block
-// (zccl_lane.hip, the default), 1 workgroup tiles (slot layout).
-static std::atomic<int> g_layout[3] = {{1}, {0}, {1}};
+// Layout of the homomorphic sum (op 1) and of decompress (op 2) for 36-value
+// blocks: 0 one lane per block (zccl_lane.hip, the default), 1 workgroup
+// tiles (slot layout).
+static std::atomic<int> g_layout[3] = {{1}, {0}, {0}};
 
 extern "C" int zccl_tune_layout(int op, int layout)
 {
-    if (op != 1 || layout < 0 || layout > 1)
+    if ((op != 1 && op != 2) || layout < 0 || layout > 1)
         return ZCCL_EINVAL;
     g_layout[op].store(layout);
     return ZCCL_OK;
@@ -141,7 +142,7 @@ extern "C" int zccl_tune_get_layout(int op) { return op < 0 || op > 2 ? -1 : g_l
 // Tiling of operation op for geometry g.
 static Tiling op_tiling(int op, const Geom& g)
 {
-    if (op == 1 && g_layout[1].load() == 0 && lane_layout_ok(g))
+    if ((op == 1 || op == 2) && g_layout[op].load() == 0 && lane_layout_ok(g))
         return make_lane_tiling(g);
     return make_tiling(g, op_threads(op));
 }
@@ -274,7 +275,7 @@ extern "C" int zccl_decompress_dev(const unsigned char* in, const void* index, s
         return ZCCL_EINVAL;
     DecompressArgs a;
     a.job.g = g;
-    a.job.tl = make_tiling(g, op_threads(2));
+    a.job.tl = op_tiling(2, g);
     int rc = check_ws(ws, ws_bytes, a.job.tl.ntiles);
     if (rc)
         return rc;
@@ -283,7 +284,10 @@ extern "C" int zccl_decompress_dev(const unsigned char* in, const void* index, s
     a.job.io.in[0].index = (const uint8_t*) index;
     a.job.io.out = out;
     a.ws = (WsHeader*) ws;
-    HIP_TRY(launch_decompress(a, (hipStream_t) stream));
+    if (is_lane_tiling(g, a.job.tl))
+        HIP_TRY(launch_decompress_lane(a, (hipStream_t) stream));
+    else
+        HIP_TRY(launch_decompress(a, (hipStream_t) stream));
     return ZCCL_OK;
 }
 
@@ -435,7 +439,7 @@ struct zccl_batch {
     size_t tile0_off;                 // byte offset of the tile starts in `host`
     void* owned;                      // device copy owned by the plan (create()), else null
     bool bound;
-    bool lane;                        // homomorphic sum in the lane layout
+    bool lane;                        // homomorphic sum or decompress in the lane layout
 };
 
 namespace {
@@ -456,7 +460,8 @@ zccl_batch* batch_plan(BatchKind kind, const std::vector<Job>& jobs, uint32_t ch
     b->kind = kind;
     b->owned = nullptr;
     b->bound = false;
-    b->lane = kind == BatchKind::Homsum && nj > 0 && is_lane_tiling(jobs[0].g, jobs[0].tl);
+    b->lane = (kind == BatchKind::Homsum || kind == BatchKind::Decompress) && nj > 0 &&
+              is_lane_tiling(jobs[0].g, jobs[0].tl);
     uint32_t threads = 64;
     size_t lds = 0;
     std::vector<unsigned long long> tile0(nj + 1, 0);
@@ -487,6 +492,8 @@ zccl_batch* batch_plan(BatchKind kind, const std::vector<Job>& jobs, uint32_t ch
     a.interleave = uniform && nj > 1;
     a.total_chains = chains;
     a.total_tiles = tile0[nj];
+    for (size_t j = 0; j < nj; j++)
+        a.max_segments = std::max(a.max_segments, jobs[j].g.S);
     if (uniform) {
         a.g = jobs[0].g;
         a.tl = jobs[0].tl;
@@ -631,7 +638,7 @@ extern "C" zccl_batch* zccl_batch_decompress_plan(int njobs, const size_t* nbEle
             *st = ZCCL_EINVAL;
             return nullptr;
         }
-        d.tl = make_tiling(d.g, op_threads(2));
+        d.tl = op_tiling(2, d.g);
         d.p = op_params(d.tl, 0.0, ebs[j], 1);
         d.io.in[0].bytes = ins[j];
         d.io.in[0].index = (const uint8_t*) indices[j];
@@ -675,8 +682,10 @@ extern "C" int zccl_batch_launch(const zccl_batch* b, void* ws, size_t ws_bytes,
         return rc;
     BatchArgs a = b->args;
     a.ws = (WsHeader*) ws;
-    if (b->lane)
+    if (b->lane && b->kind == BatchKind::Homsum)
         HIP_TRY(launch_homsum_lane_batch(a, (hipStream_t) stream));
+    else if (b->lane)
+        HIP_TRY(launch_decompress_lane_batch(a, (hipStream_t) stream));
     else
         HIP_TRY(launch_batch(b->kind, a, b->threads, b->lds, (hipStream_t) stream));
     return ZCCL_OK;
@@ -687,7 +696,8 @@ extern "C" int zccl_batch_launch_staged(const zccl_batch* b, void* ws, size_t ws
 {
     if (!b || !b->bound)
         return ZCCL_EINVAL;
-    if (!(b->lane && b->args.uniform && stage_ok(stage, stage_bytes, b->args.total_tiles)))
+    if (!(b->lane && b->kind == BatchKind::Homsum && b->args.uniform &&
+          stage_ok(stage, stage_bytes, b->args.total_tiles)))
         return zccl_batch_launch(b, ws, ws_bytes, stream);
     int rc = check_ws(ws, ws_bytes, b->args.total_tiles);
     if (rc)

@@ -112,6 +112,8 @@ struct BatchArgs {
     uint32_t uniform;
     uint32_t interleave;
     uint32_t total_chains;
+    uint32_t max_segments;  // the most segments of any job
+    uint32_t pad0;
     uint64_t total_tiles;
     Geom g;
     Tiling tl;
@@ -205,6 +207,8 @@ hipError_t launch_parse(const ParseArgs& a, hipStream_t s);
 size_t lane_lds_bytes();
 hipError_t launch_homsum_lane(const HomsumArgs& a, hipStream_t s);
 hipError_t launch_homsum_lane_batch(const BatchArgs& b, hipStream_t s);
+hipError_t launch_decompress_lane(const DecompressArgs& a, hipStream_t s);
+hipError_t launch_decompress_lane_batch(const BatchArgs& b, hipStream_t s);
 
 // Two-pass lane sum.  Pass one sums and encodes every tile into a slot of a
 // stage area with no wait on other tiles; pass two scans the tile sizes (one

@@ -1676,7 +1676,6 @@ __global__ void __launch_bounds__(256) k_parse_spec(ParseArgs a, ParseLayout pl)
         parse_nonzero_map(buf, ndw, nzb);
     __syncthreads();
     const uint32_t off = (uint32_t) (ts & 15);
-    const uint8_t* tb = (const uint8_t*) buf + off;
     const uint32_t len = (uint32_t) (te - ts);
     const uint32_t hb = 1u + ((a.g.L + 7u) >> 3);
     uint32_t* table = (uint32_t*) ((char*) a.ws + pl.table_off) + gt * pl.maxb;

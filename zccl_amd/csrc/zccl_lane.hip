@@ -80,6 +80,9 @@ __device__ __forceinline__ uint32_t lane_block_bytes(uint32_t w, uint32_t n)
 __device__ __forceinline__ uint32_t high_bytes(uint32_t w) { return mad_u24(w >> 3, L, 0u); }
 
 using View = __amdgpu_buffer_rsrc_t;
+// Four floats at 4-byte alignment (one 16-byte store where the address is
+// only dword aligned).
+typedef float f32x4u __attribute__((ext_vector_type(4), aligned(4)));
 
 // Range-checked view of [base, base + bytes) (base 4-byte aligned, bytes a
 // multiple of 4).  A buffer load returns zeros for every dword that does not
@@ -404,6 +407,47 @@ constexpr uint32_t kStageHalf = kOutBuf / 2;
 constexpr uint32_t kStageCap = kStageHalf - 64;
 static_assert(kStageHalf % 16 == 0, "stage halves stay 16-byte aligned");
 
+// A lane's block from tile bytes staged in LDS at st (offset o, index width
+// w): aligned dword reads (a misaligned LDS read is legal but slow).
+__device__ __forceinline__ void read_block_staged(const uint32_t* st, uint32_t o, uint32_t w, Block& q)
+{
+    const uint32_t* p = st + (o >> 2);
+    uint32_t D[HEADW + 1];
+#pragma unroll
+    for (uint32_t i = 0; i < HEADW + 1; i++)
+        D[i] = p[i];
+#pragma unroll
+    for (uint32_t i = 0; i < HEADW; i++)
+        q.h[i] = __builtin_amdgcn_alignbyte(D[i + 1], D[i], o);  // uses o & 3
+    const uint32_t r = w & 7, lo = o + 1 + SB + high_bytes(w);
+    uint32_t a = lo;
+#pragma unroll
+    for (uint32_t gp = 0; gp < NGRP; gp++, a += r) {
+        const uint32_t* pg = st + (a >> 2);
+        const uint32_t E0 = pg[0], E1 = pg[1], E2 = pg[2];
+        // the big-endian dwords at byte a & 3 (alignbyte uses a & 3)
+        q.ge[2 * gp] = __builtin_bswap32(__builtin_amdgcn_alignbyte(E1, E0, a));
+        q.ge[2 * gp + 1] = __builtin_bswap32(__builtin_amdgcn_alignbyte(E2, E1, a));
+    }
+}
+
+// The same straight from a view, per lane (tiles too large to stage).
+__device__ __forceinline__ void read_block_direct(View v, uint32_t o, uint32_t w, Block& q)
+{
+    const uint4 h0 = bl16(v, o), h1 = bl16(v, o + 16);
+    const uint3 h2 = bl12(v, o + 32);
+    q.h[0] = h0.x; q.h[1] = h0.y; q.h[2] = h0.z; q.h[3] = h0.w;
+    q.h[4] = h1.x; q.h[5] = h1.y; q.h[6] = h1.z; q.h[7] = h1.w;
+    q.h[8] = h2.x; q.h[9] = h2.y; q.h[10] = h2.z;
+    const uint32_t r = w & 7, lo = o + 1 + SB + high_bytes(w);
+#pragma unroll
+    for (uint32_t gp = 0; gp < NGRP; gp++) {
+        const uint2 e = bl8(v, lo + gp * r);
+        q.ge[2 * gp] = __builtin_bswap32(e.x);
+        q.ge[2 * gp + 1] = __builtin_bswap32(e.y);
+    }
+}
+
 // One tile of a fused k-operand sum, executed by one wave.  obuf: the wave's
 // LDS buffer (kOutBuf bytes, 16-byte aligned).
 // Staged (pass one of the two-pass sum): no lookback; the tile's bytes go to
@@ -502,44 +546,10 @@ __device__ __forceinline__ void homsum_tile_lane(const Geom& g, const Tiling& tl
         }
         return true;
     };
-    // this lane's block from the staged bytes (aligned dword reads: a
-    // misaligned LDS read is legal but slow)
     auto read_staged = [&](const Loc& c, uint32_t half, Block& q) {
-        const uint32_t* st = obuf + half * (kStageHalf / 4);
-        const uint32_t* p = st + (c.o >> 2);
-        uint32_t D[HEADW + 1];
-#pragma unroll
-        for (uint32_t i = 0; i < HEADW + 1; i++)
-            D[i] = p[i];
-#pragma unroll
-        for (uint32_t i = 0; i < HEADW; i++)
-            q.h[i] = __builtin_amdgcn_alignbyte(D[i + 1], D[i], c.o);  // uses o & 3
-        const uint32_t r = c.w & 7, lo = c.o + 1 + SB + high_bytes(c.w);
-        uint32_t a = lo;
-#pragma unroll
-        for (uint32_t gp = 0; gp < NGRP; gp++, a += r) {
-            const uint32_t* pg = st + (a >> 2);
-            const uint32_t E0 = pg[0], E1 = pg[1], E2 = pg[2];
-            // the big-endian dwords at byte a & 3 (alignbyte uses a & 3)
-            q.ge[2 * gp] = __builtin_bswap32(__builtin_amdgcn_alignbyte(E1, E0, a));
-            q.ge[2 * gp + 1] = __builtin_bswap32(__builtin_amdgcn_alignbyte(E2, E1, a));
-        }
+        read_block_staged(obuf + half * (kStageHalf / 4), c.o, c.w, q);
     };
-    // the same from the view, per lane (tiles too large to stage)
-    auto read_direct = [&](const Loc& c, Block& q) {
-        const uint4 h0 = bl16(c.v, c.o), h1 = bl16(c.v, c.o + 16);
-        const uint3 h2 = bl12(c.v, c.o + 32);
-        q.h[0] = h0.x; q.h[1] = h0.y; q.h[2] = h0.z; q.h[3] = h0.w;
-        q.h[4] = h1.x; q.h[5] = h1.y; q.h[6] = h1.z; q.h[7] = h1.w;
-        q.h[8] = h2.x; q.h[9] = h2.y; q.h[10] = h2.z;
-        const uint32_t r = c.w & 7, lo = c.o + 1 + SB + high_bytes(c.w);
-#pragma unroll
-        for (uint32_t gp = 0; gp < NGRP; gp++) {
-            const uint2 e = bl8(c.v, lo + gp * r);
-            q.ge[2 * gp] = __builtin_bswap32(e.x);
-            q.ge[2 * gp + 1] = __builtin_bswap32(e.y);
-        }
-    };
+    auto read_direct = [&](const Loc& c, Block& q) { read_block_direct(c.v, c.o, c.w, q); };
 
     uint32_t acc[L];
 #pragma unroll
@@ -788,6 +798,405 @@ __device__ __forceinline__ void homsum_tile_lane(const Geom& g, const Tiling& tl
     }
 }
 
+// ---------------------------------------------------------------- decompress
+//
+// Two passes, no wait between tiles.  Pass one: lane j decodes block j of the
+// wave's tile (the sum's loaders and decoders, one operand) and the wave
+// writes the tile's sum of deltas.  A scan per segment turns the sums into
+// each tile's carry, starting from the segment's seed.  Pass two decodes the
+// tile again, takes the running sums of the deltas in the lane, places the
+// blocks with one scan of the block sums across the wave, dequantizes, and
+// stores the values through the wave's LDS buffer, half a tile at a time,
+// with 16-byte stores.  Arithmetic: prefix sums of the deltas in int32 from
+// the segment's seed, x' = (float) q * eb (ZCCL/src/ZCCLd_float.c:588-628).
+//
+// The tile sums live in the workspace's lookback status array of the parity
+// that the next lookback launch clears (the one other than its own), so the
+// two passes leave no state that a later launch must see cleared.
+__device__ __forceinline__ uint32_t* dec_sums(WsHeader* ws)
+{
+    const unsigned long long par = (__hip_atomic_load(&ws->seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >> 32) & 1ull;
+    return (uint32_t*) ((char*) ws + sizeof(WsHeader) + (par ^ 1ull) * ldc(&ws->capacity) * kStatusStride * 8);
+}
+
+// The segment's seed (0 with an error when its offset is past the stream).
+__device__ __forceinline__ uint32_t segment_seed(const Geom& g, const uint8_t* bytes, const uint8_t* index,
+    uint32_t s, WsHeader* ws)
+{
+    const IndexView iv = index_view(g, index);
+    const uint64_t so = iv.seed_off[s], size = iv.meta[0];
+    if (so + 4 <= size)
+        return load_u32_bytes(bytes + so);
+    raise_error(ws, ZK_ERR_OVERRUN);
+    return 0;
+}
+
+// Launch tile tg of a decompress in the lane layout: one stream
+// (DecompressArgs), or a batch (uniform: shared geometry and bound, per-job
+// pointers; mixed: per-job descriptors).
+struct DecJob {
+    Geom g;
+    Tiling tl;
+    float eb;
+    const uint8_t* bytes;
+    const uint8_t* index;
+    float* out;
+    uint32_t t;         // tile inside the job
+    uint64_t tile0;     // the job's first launch tile
+};
+
+__device__ __forceinline__ bool dec_job(const DecompressArgs& a, uint64_t tg, DecJob& d)
+{
+    if (tg >= a.job.tl.ntiles)
+        return false;
+    d.g = a.job.g;
+    d.tl = a.job.tl;
+    d.eb = a.job.p.eb;
+    d.bytes = a.job.io.in[0].bytes;
+    d.index = a.job.io.in[0].index;
+    d.out = a.job.io.out;
+    d.t = (uint32_t) tg;
+    d.tile0 = 0;
+    return true;
+}
+
+__device__ __forceinline__ bool dec_job(const BatchArgs& b, uint64_t tg, DecJob& d)
+{
+    if (tg >= b.total_tiles)
+        return false;
+    if (b.uniform) {
+        const uint32_t j = __builtin_amdgcn_readfirstlane((uint32_t) (tg / b.tl.ntiles));
+        const DecompressIO* io = (const DecompressIO*) b.jobs + j;
+        d.g = b.g;
+        d.tl = b.tl;
+        d.eb = b.p.eb;
+        d.bytes = ldc(&io->in[0].bytes);
+        d.index = ldc(&io->in[0].index);
+        d.out = ldc(&io->out);
+        d.tile0 = (uint64_t) j * b.tl.ntiles;
+    } else {
+        uint32_t lo = 0, hi = b.njobs;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) / 2;
+            if (ldc(&b.job_tile0[mid]) <= tg)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        const uint32_t j = __builtin_amdgcn_readfirstlane(lo);
+        const DecompressJob* job = (const DecompressJob*) b.jobs + j;
+        d.g = job->g;
+        d.tl = job->tl;
+        d.eb = ldc(&job->p.eb);
+        d.bytes = ldc(&job->io.in[0].bytes);
+        d.index = ldc(&job->io.in[0].index);
+        d.out = ldc(&job->io.out);
+        d.tile0 = ldc(&b.job_tile0[j]);
+    }
+    d.t = __builtin_amdgcn_readfirstlane((uint32_t) (tg - d.tile0));
+    return true;
+}
+
+// One tile of a decompress pass as it moves through the pipeline: its launch
+// index, the lane's width byte and the tile's offset and stream size from the
+// index (job, geometry and tile are derived again where needed, with scalar
+// work, so that three tiles in flight hold few registers); then the lane's
+// block located in the tile bytes, and the bytes staged.
+struct DecPre {
+    bool ok;
+    uint64_t idx;
+    uint64_t goff, size;
+    uint32_t wq;
+    uint32_t carry;    // pass two: the tile's carry
+};
+
+struct DecLoc {
+    View v;
+    uint32_t w, bs, o, lim, nj;
+    bool has, skip, staged;
+};
+
+// Issue the index reads of launch tile tg (and its carry in pass two).
+template <int Pass, typename Args>
+__device__ __forceinline__ DecPre dec_prefetch(const Args& a, uint64_t tg, const uint32_t* sums)
+{
+    DecPre p;
+    DecJob d;
+    p.ok = dec_job(a, tg, d);
+    if (!p.ok)
+        return p;
+    const Geom& g = d.g;
+    const Tile x = tile_of(g, d.tl, d.t);
+    p.idx = tg;
+    const IndexView iv = index_view(g, d.index);
+    p.goff = iv.group_off[x.grp0];
+    p.size = iv.meta[0];
+    const uint32_t lane = threadIdx.x & 63;
+    p.wq = lane < x.nblk ? (uint32_t) iv.widths[x.g0 + lane] : 0u;
+    p.carry = Pass == 2 ? sums[tg] : 0u;
+    return p;
+}
+
+// Locate the lane's block of tile p (one scan of the block sizes) and stage
+// the tile's bytes into LDS half `half` of the wave buffer (16-byte
+// direct-to-LDS loads, zeros past the stream); tiles too large to stage are
+// read per lane.
+template <typename Args>
+__device__ __forceinline__ DecLoc dec_locate(const Args& a, const DecPre& p, uint32_t* obuf, uint32_t half,
+    bool& bad_width)
+{
+    DecJob d;
+    dec_job(a, p.idx, d);
+    const Geom& g = d.g;
+    const Tile x = tile_of(g, d.tl, d.t);
+    DecLoc c;
+    const uint32_t lane = threadIdx.x & 63;
+    c.has = lane < x.nblk;
+    c.nj = c.has ? min(tile_block_len(g, x, lane), L) : 0u;
+    c.w = p.wq;
+    if (c.w > 32) {
+        bad_width = true;
+        c.w = 0;
+    }
+    c.bs = c.has ? (c.nj == L ? full_block_bytes(c.w) : lane_block_bytes(c.w, c.nj)) : 0u;
+    const uint32_t binc = wave_scan_dpp(c.bs);
+    const uint32_t tot0 = __builtin_amdgcn_readlane(binc, 63);
+    c.skip = p.goff > p.size || tot0 > p.size - p.goff;
+    const uint8_t* base = d.bytes + p.goff;
+    const uint32_t delta = (uint32_t) ((uintptr_t) base & 3u);
+    const uint64_t vlen = c.skip ? 0ull : align_up(delta + (p.size - p.goff), 4);
+    c.lim = vlen > 0xFFFFFFFCull ? 0xFFFFFFFCu : (uint32_t) vlen;
+    c.v = make_view(base - delta, vlen);
+    c.o = delta + binc - c.bs;
+    const uint32_t nch = (tot0 + delta + 15) >> 4;
+    c.staged = !c.skip && 16 * nch <= kStageCap;
+    if (c.staged) {
+        uint8_t* st = (uint8_t*) obuf + half * kStageHalf;
+        for (uint32_t run = 0; run * 64 < nch; run++) {
+            const uint32_t ch = run * 64 + lane;
+            if (ch < nch)
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(c.v, (__attribute__((address_space(3))) void*) (st + run * 1024),
+                    16, 16 * ch, 0, 0, 0);
+        }
+    }
+    return c;
+}
+
+// Store the wave's blocks of values (lane l: v[0..nj), at base + 36 l; the
+// lanes below nblk hold one, all whole but perhaps the last) so that every
+// store instruction writes contiguous, 16-byte aligned chunks.  The blocks
+// start D dwords past the 16-byte grid; chunk c of the wave covers dwords
+// [4c - D, 4c - D + 4) from base.  A quarter of the wave (16 blocks, 144
+// chunks) at a time: its lanes write their values to LDS (ob) at their
+// offsets in the quarter's chunks, the lane before the quarter its last D
+// values, and the wave stores the chunks with streaming (non-temporal)
+// stores.  Lane 0's first chunk is partly the previous tile's and the last
+// whole block's last D values start the next tile's (both by dword stores),
+// and a segment's short last block goes value by value.  (Unaligned HBM
+// writes, and 16-byte stores strided 144 bytes across the lanes, both ran at
+// under half the rate of aligned contiguous ones.)
+__device__ __forceinline__ void store_blocks(float* base, const float (&v)[L], uint32_t nj, uint32_t nblk,
+    uint32_t* ob)
+{
+    typedef unsigned int u32x4v __attribute__((ext_vector_type(4)));
+    const uint32_t lane = threadIdx.x & 63;
+    const bool full = lane < nblk && nj == L;
+    const unsigned long long fm = __ballot(full);
+    const uint32_t nfull = (uint32_t) __popcll(fm);  // whole blocks: a prefix of the lanes
+    const bool next_full = lane < 63 && ((fm >> (lane + 1)) & 1ull);
+    const uint32_t D = __builtin_amdgcn_readfirstlane((uint32_t) (((uintptr_t) base >> 2) & 3u));
+    float* a0 = base - D;  // 16-byte aligned
+    const uint32_t nch = 9 * nfull;
+    float* of = reinterpret_cast<float*>(ob);
+#pragma unroll
+    for (uint32_t q = 0; q < 4; q++) {
+        const uint32_t c0 = 144 * q;
+        if (c0 >= nch)
+            break;
+        const uint32_t c1 = min(c0 + 144, nch);
+        if (full && (lane >> 4) == q) {
+            const uint32_t at = D + L * (lane & 15);
+#pragma unroll
+            for (uint32_t j = 0; j < L; j++)
+                if (j < L - 3 || at + j < 576)
+                    of[at + j] = v[j];
+        }
+        if (full && q > 0 && lane == 16 * q - 1) {
+#pragma unroll
+            for (uint32_t j = L - 3; j < L; j++)
+                if (j >= L - D)
+                    of[j - (L - D)] = v[j];
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        for (uint32_t c = c0 + lane; c < c1; c += 64) {
+            const uint4 x = reinterpret_cast<const uint4*>(ob)[c - c0];
+            float* g = a0 + 4 * c;
+            if (c == 0 && D) {
+                if (D <= 1)
+                    g[1] = __uint_as_float(x.y);
+                if (D <= 2)
+                    g[2] = __uint_as_float(x.z);
+                g[3] = __uint_as_float(x.w);
+            } else {
+                const u32x4v y = {x.x, x.y, x.z, x.w};
+                __builtin_nontemporal_store(y, reinterpret_cast<u32x4v*>(g));
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    }
+    float* p = base + lane * L;
+    if (full && D && !next_full) {
+#pragma unroll
+        for (uint32_t m = L - 3; m < L; m++)
+            if (m >= L - D)
+                p[m] = v[m];
+    }
+    if (lane < nblk && !full) {
+#pragma unroll
+        for (uint32_t j = 0; j < L; j++)
+            if (j < nj)
+                p[j] = v[j];
+    }
+}
+
+// A decompress pass over launch tiles first, first + stride, ... of one
+// wave.  Pass 1 writes each tile's sum of deltas to sums[idx]; pass 2 reads
+// the tile's carry from sums[idx] and writes its values.  Per step, tile i
+// is decoded while tile i+1's bytes move into the other LDS half and tile
+// i+2's index entries are on their way.  The lane decoders are the sum's
+// (whole blocks below width 16 that the reads covered take the fast one, the
+// rest the general one; a lane that does not use the fast decoder runs it at
+// width 0, which adds zeros).
+template <int Pass, typename Args>
+__device__ __forceinline__ void dec_run(const Args& a, WsHeader* ws, uint64_t first, uint64_t stride, uint32_t* obuf)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    uint32_t* sums = dec_sums(ws);
+    bool bad_width = false, bad_byte = false, bad_over = false;
+    DecPre pc = dec_prefetch<Pass>(a, first, sums);
+    if (!pc.ok)
+        return;
+    __builtin_amdgcn_s_waitcnt(0);
+    DecLoc cc = dec_locate(a, pc, obuf, 0, bad_width);
+    DecPre pn = dec_prefetch<Pass>(a, first + stride, sums);
+    for (uint32_t half = 0;; half ^= 1u) {
+        __builtin_amdgcn_s_waitcnt(0);  // tile i's bytes in LDS, tile i+1's index entries in registers
+        Block q;
+        q.w = cc.w;
+        q.o = cc.o;
+        q.lim = cc.lim;
+        if (cc.staged)
+            read_block_staged(obuf + half * (kStageHalf / 4), cc.o, cc.w, q);
+        else
+            read_block_direct(cc.v, cc.o, cc.w, q);
+        __builtin_amdgcn_sched_barrier(0);
+        DecLoc cn;
+        DecPre pnn;
+        pnn.ok = false;
+        if (pn.ok) {
+            cn = dec_locate(a, pn, obuf, half ^ 1u, bad_width);
+            pnn = dec_prefetch<Pass>(a, pn.idx + stride, sums);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        const bool fast = !cc.has || cc.w == 0 || (cc.nj == L && cc.w < 16 && (cc.staged || cc.o + cc.bs + 3 <= cc.lim));
+        const bool use = fast && !cc.skip;
+        bad_byte |= use && cc.has && (cc.w || cc.staged || cc.o + 4 <= cc.lim) && (q.h[0] & 0xFFu) != cc.w;
+        bad_over |= cc.skip;
+        q.w = use ? cc.w : 0u;
+        uint32_t acc[L];
+#pragma unroll
+        for (uint32_t j = 0; j < L; j++)
+            acc[j] = 0;
+        decode_fast<true>(q, acc);
+        const bool gen = !fast && !cc.skip;
+        if (__any(gen)) {
+            if (gen)
+                bad_byte |= decode_general(cc.v, cc.o, cc.w, cc.nj, acc);
+        }
+        if (cc.nj < L) {
+#pragma unroll
+            for (uint32_t j = 0; j < L; j++)
+                if (j >= cc.nj)
+                    acc[j] = 0;
+        }
+        DecJob d;
+        dec_job(a, pc.idx, d);
+        const Geom& g = d.g;
+        const Tile x = tile_of(g, d.tl, d.t);
+        if constexpr (Pass == 1) {
+            uint32_t b = 0;
+#pragma unroll
+            for (uint32_t j = 0; j < L; j++)
+                b += acc[j];
+            const uint32_t inc = wave_scan_dpp(b);
+            if (lane == 63)
+                sums[pc.idx] = inc;
+        } else {
+#pragma unroll
+            for (uint32_t j = 1; j < L; j++)
+                acc[j] += acc[j - 1];
+            const uint32_t bsum = acc[L - 1];
+            const uint32_t carry = pc.carry + wave_scan_dpp(bsum) - bsum;
+            const float eb = d.eb;
+            float vals[L];
+#pragma unroll
+            for (uint32_t jj = 0; jj < L; jj++)
+                vals[jj] = dequant(carry + acc[jj], eb);
+            // (this tile's LDS half is free: its bytes are in registers)
+            store_blocks(d.out + x.e0, vals, cc.nj, x.nblk, obuf + half * (kStageHalf / 4));
+            if (lane == 0 && x.lt == 0)
+                d.out[x.seg_a] = dequant(segment_seed(g, d.bytes, d.index, x.s, ws), eb);
+        }
+        if (!pn.ok)
+            break;
+        pc = pn;
+        cc = cn;
+        pn = pnn;
+    }
+    if (__any(bad_width) && lane == 0)
+        raise_error(ws, ZK_ERR_WIDTH);
+    if (__any(bad_byte) && lane == 0)
+        raise_error(ws, ZK_ERR_MISMATCH);
+    if (__any(bad_over) && lane == 0)
+        raise_error(ws, ZK_ERR_OVERRUN);
+}
+
+// Scan of one segment's tile sums into carries (in place): the seed plus the
+// sums of the segment's earlier tiles, mod 2^32.  One workgroup.
+constexpr uint32_t kDecScanThreads = 1024;
+
+__device__ __forceinline__ void decompress_scan_lane(const Geom& g, const Tiling& tl, const uint8_t* bytes,
+    const uint8_t* index, uint32_t s, uint32_t* sums, uint64_t tile0, WsHeader* ws)
+{
+    __shared__ uint32_t s_w[kDecScanThreads / 64];
+    __shared__ uint32_t s_seed;
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const uint32_t first = s < g.T ? s * tl.tilesT : g.T * tl.tilesT;
+    const uint32_t cnt = s < g.T ? tl.tilesT : tl.tilesR;
+    if (tid == 0)
+        s_seed = segment_seed(g, bytes, index, s, ws);
+    uint32_t* p = sums + tile0 + first;
+    const uint32_t per = (cnt + kDecScanThreads - 1) / kDecScanThreads;
+    const uint32_t a = min(tid * per, cnt), b = min(a + per, cnt);
+    uint32_t run = 0;
+    for (uint32_t i = a; i < b; i++)
+        run += p[i];
+    const uint32_t inc = wave_scan_dpp(run);
+    if (lane == 63)
+        s_w[wv] = inc;
+    __syncthreads();
+    uint32_t before = s_seed + inc - run;
+    for (uint32_t i = 0; i < wv; i++)
+        before += s_w[i];
+    for (uint32_t i = a; i < b; i++) {
+        const uint32_t v = p[i];
+        p[i] = before;
+        before += v;
+    }
+}
+
 }  // namespace
 
 // One fused sum (kernel arguments hold its geometry and pointers).
@@ -864,6 +1273,65 @@ __global__ void __launch_bounds__(64 * kLaneWaves, 4) k_homsum_lane_mixed(BatchA
     const SumIO src{&job->io};
     homsum_tile_lane<false>(g, tl, ldc(&job->p.k), src, ldc(&job->io.out), ldc(&job->io.out_cap), ldc(&job->io.index),
         ldc(&job->io.size_out), t, status + tile0 * kStatusStride, b.ws, b.total_chains, zk_lds + wv * (kOutBuf / 4));
+}
+
+// The two passes: a persistent grid, wave w of workgroup b starting at launch
+// tile 4b + w and striding by the grid's waves.
+template <int Pass, typename Args>
+__device__ __forceinline__ void dec_pass(const Args& a, WsHeader* ws)
+{
+    const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    dec_run<Pass>(a, ws, (uint64_t) blockIdx.x * kLaneWaves + wv, (uint64_t) gridDim.x * kLaneWaves,
+        zk_lds + wv * (kOutBuf / 4));
+}
+
+// five workgroups per CU: the LDS buffers' limit
+constexpr uint32_t kDecGroupsPerCU = 5;
+
+__global__ void __launch_bounds__(64 * kLaneWaves, kDecGroupsPerCU) k_decompress_lane_sums(DecompressArgs a)
+{
+    dec_pass<1>(a, a.ws);
+}
+__global__ void __launch_bounds__(64 * kLaneWaves, kDecGroupsPerCU) k_decompress_lane_out(DecompressArgs a)
+{
+    dec_pass<2>(a, a.ws);
+}
+__global__ void __launch_bounds__(64 * kLaneWaves, kDecGroupsPerCU) k_decompress_lane_batch_sums(BatchArgs b)
+{
+    dec_pass<1>(b, b.ws);
+}
+__global__ void __launch_bounds__(64 * kLaneWaves, kDecGroupsPerCU) k_decompress_lane_batch_out(BatchArgs b)
+{
+    dec_pass<2>(b, b.ws);
+}
+
+// Scans: workgroup (s, j) takes segment s of job j.
+__global__ void __launch_bounds__(kDecScanThreads) k_decompress_lane_scan(DecompressArgs a)
+{
+    if (blockIdx.x >= a.job.g.S)
+        return;
+    decompress_scan_lane(a.job.g, a.job.tl, a.job.io.in[0].bytes, a.job.io.in[0].index, blockIdx.x, dec_sums(a.ws), 0,
+        a.ws);
+}
+
+__global__ void __launch_bounds__(kDecScanThreads) k_decompress_lane_batch_scan(BatchArgs b)
+{
+    const uint32_t j = blockIdx.y;
+    if (b.uniform) {
+        if (blockIdx.x >= b.g.S)
+            return;
+        const DecompressIO* io = (const DecompressIO*) b.jobs + j;
+        decompress_scan_lane(b.g, b.tl, ldc(&io->in[0].bytes), ldc(&io->in[0].index), blockIdx.x, dec_sums(b.ws),
+            (uint64_t) j * b.tl.ntiles, b.ws);
+    } else {
+        const DecompressJob* job = (const DecompressJob*) b.jobs + j;
+        const Geom g = job->g;
+        if (blockIdx.x >= g.S)
+            return;
+        const Tiling tl = job->tl;
+        decompress_scan_lane(g, tl, ldc(&job->io.in[0].bytes), ldc(&job->io.in[0].index), blockIdx.x, dec_sums(b.ws),
+            ldc(&b.job_tile0[j]), b.ws);
+    }
 }
 
 // ---------------------------------------------------------------- two-pass sum
@@ -1102,6 +1570,41 @@ hipError_t launch_homsum_lane_batch(const BatchArgs& b, hipStream_t s)
         hipLaunchKernelGGL(k_homsum_lane_batch, dim3(groups), dim3(64 * kLaneWaves), lane_lds_bytes(), s, b);
     else
         hipLaunchKernelGGL(k_homsum_lane_mixed, dim3(groups), dim3(64 * kLaneWaves), lane_lds_bytes(), s, b);
+    return hipGetLastError();
+}
+
+// Workgroups of a persistent decompress pass: as many as the device holds at
+// once, or one per four tiles when that is fewer.
+static uint32_t dec_groups(uint64_t tiles)
+{
+    static int cus[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64)
+        dev = 0;
+    if (!cus[dev]) {
+        int n = 0;
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0)
+            n = 256;
+        cus[dev] = n;
+    }
+    return (uint32_t) std::min<uint64_t>(ceil_div(tiles, kLaneWaves), (uint64_t) cus[dev] * kDecGroupsPerCU);
+}
+
+hipError_t launch_decompress_lane(const DecompressArgs& a, hipStream_t s)
+{
+    const dim3 grid(dec_groups(a.job.tl.ntiles)), block(64 * kLaneWaves);
+    hipLaunchKernelGGL(k_decompress_lane_sums, grid, block, lane_lds_bytes(), s, a);
+    hipLaunchKernelGGL(k_decompress_lane_scan, dim3(a.job.g.S), dim3(kDecScanThreads), 0, s, a);
+    hipLaunchKernelGGL(k_decompress_lane_out, grid, block, lane_lds_bytes(), s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_decompress_lane_batch(const BatchArgs& b, hipStream_t s)
+{
+    const dim3 grid(dec_groups(b.total_tiles)), block(64 * kLaneWaves);
+    hipLaunchKernelGGL(k_decompress_lane_batch_sums, grid, block, lane_lds_bytes(), s, b);
+    hipLaunchKernelGGL(k_decompress_lane_batch_scan, dim3(b.max_segments, b.njobs), dim3(kDecScanThreads), 0, s, b);
+    hipLaunchKernelGGL(k_decompress_lane_batch_out, grid, block, lane_lds_bytes(), s, b);
     return hipGetLastError();
 }
 
