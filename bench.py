@@ -63,6 +63,11 @@ def parse_args():
     ap.add_argument("--no-3d", action="store_true", help="skip the 3-D codec measurement")
     ap.add_argument("--side3d", type=int, default=512, help="grid side of the 3-D codec's field")
     ap.add_argument("--bs3d", type=int, default=4, help="block side of the 3-D codec")
+    ap.add_argument("--tune-out", default=None,
+                    help="write the autotuned configurations (per op, blockSize, operand range, size class) "
+                         "to this JSON table")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the autotuner sweeps; knobs from the committed table where it has the workload")
     ap.add_argument("--layout", choices=("lane", "tiles"), default="lane",
                     help="sum layout for 36-value blocks: one lane per block, or workgroup tiles")
     return ap.parse_args()
@@ -388,7 +393,15 @@ def main():
         ar._cplan_key = None  # plans capture the tiling: rebuild per candidate
         ar.compress_all(xs)
 
-    tune_c = autotune.tune("compress", autotune.timed(compress_fresh))
+    table = autotune.load()  # the committed table of winners (zccl_amd/tuned.json)
+    table_out = {}
+    device_name = torch.cuda.get_device_name(0)
+    if args.no_tune:
+        tune_c = {"op": "compress", "config": autotune.apply(table, "compress", L, 1, seg_n) or
+                  autotune.get_knobs("compress"), "source": "table"}
+    else:
+        tune_c = autotune.tune("compress", autotune.timed(compress_fresh))
+        autotune.record(table_out, "compress", L, 1, seg_n, tune_c, device_name)
     ar._cplan_key = None
     ar.compress_all(xs)  # builds the batch plan
     t_comp = timed(lambda: ar.compress_all(xs), 3)
@@ -400,8 +413,32 @@ def main():
     log(f"compressed: total {total_c / 1e6:.1f} MB, CR {cr:.2f}, compress {t_comp * 1e3:.2f} ms")
 
     outs = [codec.empty_stream(seg_n, L, T) for _ in range(P)]
+    sum_ops = [[streams[r][j] for r in range(P)] for j in range(P)]
+
+    # the sum's knobs (layout, two-pass threshold, slot workgroup size and
+    # stage) autotuned on this workload: each candidate builds its plan
+    # untimed and times one launch
+    def homsum_run():
+        p = batch.homsum_plan(sum_ops, outs)
+        w = codec.Workspace(p.tiles + 1024)
+        p.launch(w)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        p.launch(w)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    if args.layout == "tiles":
+        tune_h = {"op": "homsum", "config": autotune.get_knobs("homsum"), "source": "--layout tiles"}
+    elif args.no_tune:
+        tune_h = {"op": "homsum", "config": autotune.apply(table, "homsum", L, P, seg_n) or
+                  autotune.get_knobs("homsum"), "source": "table"}
+    else:
+        tune_h = autotune.tune("homsum", homsum_run)
+        autotune.record(table_out, "homsum", L, P, seg_n, tune_h, device_name)
+        codec.check_errors()
     # one launch sums all P segments (P operands each), lookback chains interleaved
-    plan = batch.homsum_plan([[streams[r][j] for r in range(P)] for j in range(P)], outs)
+    plan = batch.homsum_plan(sum_ops, outs)
     ws = codec.Workspace(plan.tiles + 1024)
 
     for _ in range(args.warmup):
@@ -453,7 +490,12 @@ def main():
         ar._dplan_key = None
         ar.decompress_all(outs, out=y)
 
-    tune_d = autotune.tune("decompress", autotune.timed(decompress_fresh))
+    if args.no_tune:
+        tune_d = {"op": "decompress", "config": autotune.apply(table, "decompress", L, 1, seg_n) or
+                  autotune.get_knobs("decompress"), "source": "table"}
+    else:
+        tune_d = autotune.tune("decompress", autotune.timed(decompress_fresh))
+        autotune.record(table_out, "decompress", L, 1, seg_n, tune_d, device_name)
     ar._dplan_key = None
     ar.decompress_all(outs, out=y)  # builds the batch plan
     t_dec = timed(lambda: ar.decompress_all(outs, out=y), 3)
@@ -494,12 +536,16 @@ def main():
     t_doc = timed(lambda: ar.reduce_scatter_doc(xs), 2)
     t_homo_rs = t_comp + ms_per_step / 1e3
 
-    # the uniform batch of 36-value blocks runs the two-pass lane-layout sum
-    # (three kernels: sum into the stage area, scan of tile sizes, placement)
-    # or, with --layout tiles, the slot-layout sum at 256 threads
-    if L == 36 and args.layout == "lane":
-        homsum_kernel = "zk::k_homsum_lane_batch_staged + zk::k_scan_lane_batch + zk::k_place_lane_batch"
-    elif L == 36 and _zl().zccl_tune_get_threads(1) == 256:
+    # the uniform batch of 36-value blocks runs, as the knobs chose, the
+    # two-pass lane-layout sum (three kernels: sum into the stage area, scan
+    # of tile sizes, placement), the single-pass lane sum, or the slot-layout
+    # sum at 256 threads
+    hk = autotune.get_knobs("homsum")
+    two_pass = L == 36 and hk["layout"] == 0 and plan.tiles >= hk["two_pass"]
+    if L == 36 and hk["layout"] == 0:
+        homsum_kernel = ("zk::k_homsum_lane_batch_staged + zk::k_scan_lane_batch + zk::k_place_lane_batch"
+                         if two_pass else "zk::k_homsum_lane_batch")
+    elif L == 36 and hk["threads"] == 256:
         homsum_kernel = "zk::k_homsum_batch_slots"
     else:
         homsum_kernel = "zk::k_homsum_batch"
@@ -514,7 +560,8 @@ def main():
         "alg_bytes_per_launch": int(alg_per_launch),
         "avg_launch_us": round(avg_launch_s * 1e6, 2),
     }
-    if os.path.exists(args.traffic):
+    if os.path.exists(args.traffic) and two_pass:
+        # (the committed counters are those of the two-pass lane kernels)
         with open(args.traffic) as f:
             tr = json.load(f)
         roof["traffic"] = tr.get("hbm_bytes_per_launch")
@@ -577,7 +624,11 @@ def main():
         "allreduce_ms": round(t_all * 1e3, 3),
         "allreduce_c_ms": round(t_all_c * 1e3, 3),
         "allreduce_c_equal": c_equal,
-        "allreduce_autotune": [tune_c, tune_d],
+        "autotune": {"chosen": {t["op"]: t["config"] for t in (tune_c, tune_h, tune_d)},
+                     "keys": {op: autotune.key(op, L, k, seg_n) for op, k in
+                              (("compress", 1), ("homsum", P), ("decompress", 1))},
+                     "sweeps": [t for t in (tune_c, tune_h, tune_d) if "median_s" in t],
+                     "table": os.path.relpath(autotune.DEFAULT_TABLE, ROOT)},
         "homomorphic_reduce_scatter_ms": round(t_homo_rs * 1e3, 3),
         "doc_reduce_scatter_ms": round(t_doc * 1e3, 3),
         "homomorphic_vs_doc": round(t_doc / t_homo_rs, 2),
@@ -594,6 +645,8 @@ def main():
     if served:
         res["serve_p50_ms"] = served["loads"][-1]["p50_ms"]
         res["serve_p99_ms"] = served["loads"][-1]["p99_ms"]
+    if args.tune_out and table_out:
+        res["autotune"]["written"] = autotune.save(table_out, args.tune_out)
     print(json.dumps(res), flush=True)
 
 

@@ -286,11 +286,30 @@ int zccl_tune_get_threads(int op);
 
 /* Layout of the homomorphic sum (op 1) and of decompress (op 2) for 36-value
  * blocks: 0 one lane per block, a wave per 64-block tile (the default); 1
- * workgroup tiles of the zccl_tune_threads size.  Affects launches and plans created afterwards
- * and zccl_num_tiles() never shrinks below either layout's need; never the
- * stream bytes.  No reference counterpart. */
+ * workgroup tiles of the zccl_tune_threads size.  Affects launches and
+ * plans created afterwards, and zccl_num_tiles() never shrinks below either
+ * layout's need; never the stream bytes.  No reference counterpart. */
 int zccl_tune_layout(int op, int layout);
 int zccl_tune_get_layout(int op);
+
+/* Smallest operand stage, in bytes, of the slot-layout homomorphic sum
+ * (layout 1): a multiple of 32 up to 65536, 0 for the default.  Larger
+ * stages take more operands per staging round and fewer workgroups per CU.
+ * Affects launches and plans created afterwards.  No reference counterpart. */
+int zccl_tune_stage_bytes(unsigned int bytes);
+unsigned int zccl_tune_get_stage_bytes(void);
+
+/* Sums of at least this many tiles take the two-pass form when a stage area
+ * is given (zccl_homsum_dev_staged, zccl_batch_launch_staged); smaller ones
+ * the single-pass form.  0 restores the default (1024).  No reference
+ * counterpart. */
+int zccl_tune_two_pass_min_tiles(unsigned long long min_tiles);
+unsigned long long zccl_tune_get_two_pass_min_tiles(void);
+
+/* Workgroups per CU of the persistent lane-layout decompress grid (1..5; 0
+ * restores the default, 4).  No reference counterpart. */
+int zccl_tune_decompress_groups(int groups_per_cu);
+int zccl_tune_get_decompress_groups(void);
 
 /* Block-case histogram of two indexed streams (see the _record add). */
 int zccl_case_count_dev(const void *index1, const void *index2, size_t nbEle, int blockSize,

@@ -44,6 +44,8 @@ EXPORTS = (
     "zccl_batch_homsum_plan", "zccl_batch_compress_plan", "zccl_batch_decompress_plan",
     "zccl_batch_desc_bytes", "zccl_batch_desc_host", "zccl_batch_bind",
     "zccl_tune_threads", "zccl_tune_get_threads", "zccl_tune_layout", "zccl_tune_get_layout",
+    "zccl_tune_stage_bytes", "zccl_tune_get_stage_bytes", "zccl_tune_two_pass_min_tiles",
+    "zccl_tune_get_two_pass_min_tiles", "zccl_tune_decompress_groups", "zccl_tune_get_decompress_groups",
     "SZp_compress3D_fast_openmp", "SZp_decompress3D_fast_openmp",
     "zccl_sz3d_sizes", "zccl_sz3d_compress_dev", "zccl_sz3d_decompress_dev", "zccl_sz3d_status",
 )
@@ -138,6 +140,12 @@ def lib() -> ctypes.CDLL:
         "zccl_tune_get_threads": ([i32], i32),
         "zccl_tune_layout": ([i32, i32], i32),
         "zccl_tune_get_layout": ([i32], i32),
+        "zccl_tune_stage_bytes": ([ctypes.c_uint], i32),
+        "zccl_tune_get_stage_bytes": ([], ctypes.c_uint),
+        "zccl_tune_two_pass_min_tiles": ([ctypes.c_ulonglong], i32),
+        "zccl_tune_get_two_pass_min_tiles": ([], ctypes.c_ulonglong),
+        "zccl_tune_decompress_groups": ([i32], i32),
+        "zccl_tune_get_decompress_groups": ([], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

@@ -18,7 +18,7 @@ import torch
 
 from . import _lib
 from ._lib import check, lib
-from .codec import STAGED_L, STAGED_MIN_TILES, DeviceStream, Workspace, _stream_handle, default_stage, default_workspace
+from .codec import STAGED_L, DeviceStream, Workspace, _stream_handle, default_stage, default_workspace, staged_min_tiles
 
 
 class BatchPlan:
@@ -49,7 +49,7 @@ class BatchPlan:
             raise _lib.ZcclError(-6, f"workspace holds {w.max_tiles} tiles, batch needs {self.tiles}")
         for o in self.outs:
             o.invalidate()
-        if self.staged and stage is not False and self.tiles >= STAGED_MIN_TILES:
+        if self.staged and stage is not False and self.tiles >= staged_min_tiles():
             st = stage if stage is not None else default_stage(self.tiles)
             if st.max_tiles < self.tiles:
                 raise _lib.ZcclError(-6, f"stage area holds {st.max_tiles} tiles, batch needs {self.tiles}")

@@ -120,9 +120,14 @@ def default_stage(tiles: int) -> StageArea:
     return st
 
 
-# the two-pass sum: blockSize 36, sums of at least this many tiles
+# the two-pass sum: blockSize 36, sums of at least staged_min_tiles() tiles
+# (a library tunable, 1024 by default)
 STAGED_L = 36
 STAGED_MIN_TILES = 1024
+
+
+def staged_min_tiles() -> int:
+    return int(lib().zccl_tune_get_two_pass_min_tiles())
 
 
 @dataclass
@@ -225,7 +230,7 @@ def homsum(streams: Sequence[DeviceStream], out: Optional[DeviceStream] = None,
     w = _ws_for(n, L, T, ws)
     st = None
     tiles = num_tiles(n, L, T)
-    if L == STAGED_L and stage is not False and tiles >= STAGED_MIN_TILES:
+    if L == STAGED_L and stage is not False and tiles >= staged_min_tiles():
         st = stage if stage is not None else default_stage(tiles)
         if st.max_tiles < tiles:
             raise _lib.ZcclError(-6, f"stage area holds {st.max_tiles} tiles, sum needs {tiles}")

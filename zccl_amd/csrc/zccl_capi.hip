@@ -139,6 +139,10 @@ extern "C" int zccl_tune_layout(int op, int layout)
 
 extern "C" int zccl_tune_get_layout(int op) { return op < 0 || op > 2 ? -1 : g_layout[op].load(); }
 
+extern "C" int zccl_tune_stage_bytes(unsigned int bytes) { return set_stage_min_bytes(bytes) ? ZCCL_OK : ZCCL_EINVAL; }
+
+extern "C" unsigned int zccl_tune_get_stage_bytes(void) { return get_stage_min_bytes(); }
+
 // Tiling of operation op for geometry g.
 static Tiling op_tiling(int op, const Geom& g)
 {
@@ -337,14 +341,30 @@ extern "C" int zccl_homsum_dev(const unsigned char* const* ins, const void* cons
 extern "C" size_t zccl_sum_stage_bytes(size_t max_tiles) { return (size_t) stage_area_bytes(max_tiles); }
 
 // The two-pass form runs for sums of at least this many tiles; smaller ones
-// take the single-pass form (one launch instead of three).
+// take the single-pass form (one launch instead of three).  Tunable.
 constexpr uint64_t kStagedMinTiles = 1024;
+static std::atomic<unsigned long long> g_staged_min_tiles{kStagedMinTiles};
+
+extern "C" int zccl_tune_two_pass_min_tiles(unsigned long long min_tiles)
+{
+    g_staged_min_tiles.store(min_tiles ? min_tiles : kStagedMinTiles);
+    return ZCCL_OK;
+}
+
+extern "C" unsigned long long zccl_tune_get_two_pass_min_tiles(void) { return g_staged_min_tiles.load(); }
+
+extern "C" int zccl_tune_decompress_groups(int groups_per_cu)
+{
+    return set_decompress_groups_per_cu(groups_per_cu) ? ZCCL_OK : ZCCL_EINVAL;
+}
+
+extern "C" int zccl_tune_get_decompress_groups(void) { return get_decompress_groups_per_cu(); }
 
 // A stage area serves the two-pass form when the sum is large enough and the
 // area holds every tile of it and is 16-byte aligned.
 static bool stage_ok(const void* stage, size_t stage_bytes, uint64_t tiles)
 {
-    return tiles >= kStagedMinTiles && stage && ((uintptr_t) stage & 15) == 0 &&
+    return tiles >= g_staged_min_tiles.load() && stage && ((uintptr_t) stage & 15) == 0 &&
            stage_bytes >= stage_area_bytes(tiles);
 }
 

@@ -196,6 +196,11 @@ struct Sz3Args {
 
 size_t compress_lds_bytes(const Tiling& tl);
 uint32_t stage_bytes_for(const Tiling& tl, int k);
+// Smallest operand stage of the slot-layout sum (bytes, a multiple of 32; 0:
+// the default); larger stages take more operands per round and fewer
+// workgroups per CU.
+bool set_stage_min_bytes(uint32_t bytes);
+uint32_t get_stage_min_bytes();
 size_t operand_lds_bytes(const Tiling& tl, int k, uint32_t stage);
 
 hipError_t launch_compress(const CompressArgs& a, hipStream_t s);
@@ -209,6 +214,11 @@ hipError_t launch_homsum_lane(const HomsumArgs& a, hipStream_t s);
 hipError_t launch_homsum_lane_batch(const BatchArgs& b, hipStream_t s);
 hipError_t launch_decompress_lane(const DecompressArgs& a, hipStream_t s);
 hipError_t launch_decompress_lane_batch(const BatchArgs& b, hipStream_t s);
+// Workgroups per CU of the persistent lane decompress grid (1 ..
+// kDecGroupsMax; 0 restores the default, 4).
+constexpr int kDecGroupsMax = 5;
+bool set_decompress_groups_per_cu(int groups);
+int get_decompress_groups_per_cu();
 
 // Two-pass lane sum.  Pass one sums and encodes every tile into a slot of a
 // stage area with no wait on other tiles; pass two scans the tile sizes (one

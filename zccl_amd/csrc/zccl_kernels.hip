@@ -24,6 +24,8 @@
 //   sum        seeds and deltas add in int32; blocks re-encoded     :1072, 1099-1219
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 #include "zccl_device.h"
 #include "zccl_internal.h"
 #include "zccl_kernels.h"
@@ -2091,12 +2093,27 @@ size_t compress_lds_bytes(const Tiling& tl) { return (size_t) align_up(tl.tile_b
 // Decompress (k = 1): one worst-case tile.  Homomorphic sum: two halves of
 // at least kStageMin / 2 bytes each; the whole stage holds one worst-case
 // operand (aligned chunks plus slack).
+static std::atomic<uint32_t> g_stage_min{kStageMin};
+
+bool set_stage_min_bytes(uint32_t bytes)
+{
+    if (bytes == 0)
+        bytes = kStageMin;
+    if (bytes % 32 || bytes > 65536)
+        return false;
+    g_stage_min.store(bytes);
+    return true;
+}
+
+uint32_t get_stage_min_bytes() { return g_stage_min.load(); }
+
 uint32_t stage_bytes_for(const Tiling& tl, int k)
 {
     if (k <= 1)
         return (uint32_t) align_up(tl.tile_bytes_max + 64, 16);
     uint64_t need = align_up(tl.tile_bytes_max + 48, 32);
-    return (uint32_t) (need < kStageMin ? kStageMin : need);
+    const uint32_t m = g_stage_min.load();
+    return (uint32_t) (need < m ? m : need);
 }
 
 size_t operand_lds_bytes(const Tiling& tl, int k, uint32_t stage)

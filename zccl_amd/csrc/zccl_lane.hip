@@ -29,6 +29,8 @@
 //   width      (int)log2f((float)max)+1             ZCCL/src/ZCCLd_float.c:1205
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 #include "zccl_device.h"
 #include "zccl_internal.h"
 #include "zccl_kernels.h"
@@ -1287,6 +1289,7 @@ __device__ __forceinline__ void dec_pass(const Args& a, WsHeader* ws)
 
 // five workgroups per CU: the LDS buffers' limit
 constexpr uint32_t kDecGroupsPerCU = 5;
+static_assert(kDecGroupsPerCU == (uint32_t) kDecGroupsMax, "the launch bound is the grid's limit");
 
 __global__ void __launch_bounds__(64 * kLaneWaves, kDecGroupsPerCU) k_decompress_lane_sums(DecompressArgs a)
 {
@@ -1573,8 +1576,25 @@ hipError_t launch_homsum_lane_batch(const BatchArgs& b, hipStream_t s)
     return hipGetLastError();
 }
 
-// Workgroups of a persistent decompress pass: as many as the device holds at
-// once, or one per four tiles when that is fewer.
+// (four measured fastest on the 8 x 2^24 batch, the grid's limit being five)
+constexpr int kDecGroupsDefault = 4;
+static std::atomic<int> g_dec_groups{kDecGroupsDefault};
+
+bool set_decompress_groups_per_cu(int groups)
+{
+    if (groups == 0)
+        groups = kDecGroupsDefault;
+    if (groups < 1 || groups > kDecGroupsMax)
+        return false;
+    g_dec_groups.store(groups);
+    return true;
+}
+
+int get_decompress_groups_per_cu() { return g_dec_groups.load(); }
+
+// Workgroups of a persistent decompress pass: the tuned number per CU (at
+// most as many as the device holds at once), or one per four tiles when that
+// is fewer.
 static uint32_t dec_groups(uint64_t tiles)
 {
     static int cus[64] = {0};
@@ -1587,7 +1607,7 @@ static uint32_t dec_groups(uint64_t tiles)
             n = 256;
         cus[dev] = n;
     }
-    return (uint32_t) std::min<uint64_t>(ceil_div(tiles, kLaneWaves), (uint64_t) cus[dev] * kDecGroupsPerCU);
+    return (uint32_t) std::min<uint64_t>(ceil_div(tiles, kLaneWaves), (uint64_t) cus[dev] * g_dec_groups.load());
 }
 
 hipError_t launch_decompress_lane(const DecompressArgs& a, hipStream_t s)
