@@ -63,6 +63,9 @@ def parse_args():
     ap.add_argument("--no-3d", action="store_true", help="skip the 3-D codec measurement")
     ap.add_argument("--side3d", type=int, default=512, help="grid side of the 3-D codec's field")
     ap.add_argument("--bs3d", type=int, default=4, help="block side of the 3-D codec")
+    ap.add_argument("--codec-only", action="store_true",
+                    help="only the batched compress (every rank's P segments) and decompress (rank 0's P "
+                         "segments) of the headline fields, timed with HIP events; one JSON line")
     ap.add_argument("--tune-out", default=None,
                     help="write the autotuned configurations (per op, blockSize, operand range, size class) "
                          "to this JSON table")
@@ -325,6 +328,51 @@ def dropin_leg(a, b, eb, L, T, timed):
     }
 
 
+def codec_leg(ar, xs, steps, warmup):
+    """Batched compress of every rank's P segments (P*P jobs) and batched
+    decompress of rank 0's P segments (P jobs), each one launch, timed with
+    HIP events on the launch stream; fractions of HBM peak on the algorithmic
+    bytes (floats in + stream bytes out, and the reverse)."""
+    import numpy as np
+    import torch
+
+    from zccl_amd import codec
+
+    P = ar.P
+    stream = torch.cuda.current_stream()
+    y = torch.empty(ar.count, dtype=torch.float32, device="cuda")
+
+    def events(fn):
+        for _ in range(warmup):
+            fn()
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+        torch.cuda.synchronize()
+        for a, b in ev:
+            a.record(stream)
+            fn()
+            b.record(stream)
+        torch.cuda.synchronize()
+        return float(np.mean([a.elapsed_time(b) for a, b in ev])) / 1e3
+
+    t_c = events(lambda: ar.compress_all(xs))
+    codec.check_errors()
+    cbytes = sum(int(s.size) for row in ar.streams for s in row)
+    segs = [ar.streams[0][j] for j in range(P)]
+    t_d = events(lambda: ar.decompress_all(segs, out=y))
+    codec.check_errors()
+    alg_c = P * ar.count * 4 + cbytes
+    dbytes = sum(int(s.size) for s in segs)
+    alg_d = ar.count * 4 + dbytes
+    return {"workload": f"compress {P * P} x {ar.cnts[0]} values (one batched launch), decompress {P} x "
+                        f"{ar.cnts[0]} values (one batched launch), L={ar.L}, T={ar.T}",
+            "compress_us": round(t_c * 1e6, 1), "compress_alg_bytes": alg_c,
+            "compress_GBps": round(alg_c / t_c / 1e9, 1), "compress_frac": round(alg_c / t_c / 1e9 / HBM_PEAK_GBPS, 4),
+            "decompress_us": round(t_d * 1e6, 1), "decompress_alg_bytes": alg_d,
+            "decompress_GBps": round(alg_d / t_d / 1e9, 1),
+            "decompress_frac": round(alg_d / t_d / 1e9 / HBM_PEAK_GBPS, 4),
+            "steps": steps, "warmup": warmup}
+
+
 def main():
     args = parse_args()
     if args.gpus != 1:
@@ -376,6 +424,9 @@ def main():
     ar = StagedAllreduce(P=P, count=count, eb=eb, L=L, T=T)
     seg_n = ar.cnts[0]
     log(f"eb={eb:.6g} segments={ar.cnts[0]}x{P}")
+    if args.codec_only:
+        print(json.dumps(codec_leg(ar, xs, args.steps, args.warmup)), flush=True)
+        return
 
     def timed(fn, reps):
         torch.cuda.synchronize()
@@ -500,6 +551,10 @@ def main():
     ar.decompress_all(outs, out=y)  # builds the batch plan
     t_dec = timed(lambda: ar.decompress_all(outs, out=y), 3)
     codec.check_errors()
+    # the batched compress and decompress alone, HIP events, fractions of peak
+    codec_fracs = codec_leg(ar, xs, 5, 2)
+    log(f"codec: compress {codec_fracs['compress_us']:.1f} us ({codec_fracs['compress_frac']:.3f} of peak), "
+        f"decompress {codec_fracs['decompress_us']:.1f} us ({codec_fracs['decompress_frac']:.3f})")
     truth = torch.zeros(count, dtype=torch.float64, device="cuda")
     for x in xs:
         truth += x.double()
@@ -621,6 +676,7 @@ def main():
         "compression_ratio": round(cr, 3),
         "compress_GBps_input": round(P * count * 4 / 1e9 / t_comp, 1),
         "decompress_GBps_output": round(count * 4 / 1e9 / t_dec, 1),
+        "codec": codec_fracs,
         "allreduce_ms": round(t_all * 1e3, 3),
         "allreduce_c_ms": round(t_all_c * 1e3, 3),
         "allreduce_c_equal": c_equal,
