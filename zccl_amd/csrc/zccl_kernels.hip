@@ -1655,12 +1655,20 @@ __device__ __forceinline__ uint32_t parse_next_nonzero(const uint32_t* buf, cons
     return min(4 * d2 + ((uint32_t) __builtin_ctz(buf[d2]) >> 3), lim);
 }
 
-// Pass one: one workgroup per tile; wave w walks candidates 64w .. 64w + 63
-// (and 256 further on for the widest blocks).
+// Pass one: one workgroup per tile.  Wave 0 walks candidates 0..63 and
+// records, for every offset one of its walks steps on, the lane and the
+// blocks it had walked there; then the four waves walk the remaining
+// candidates, each stopping at the first recorded offset: from there its
+// walk is that lane's, so its exit is that lane's and its count its own
+// blocks so far plus what that lane walked from the offset on.  Walks merge
+// within a few blocks, so the later candidates take a few steps instead of
+// the whole tile.
 __global__ void __launch_bounds__(256) k_parse_spec(ParseArgs a, ParseLayout pl)
 {
     __shared__ uint32_t buf[kParseTile / 4 + 5];  // one dword past the staged bytes for the 8-byte window
     __shared__ uint32_t nzb[kParseNzWords];
+    __shared__ uint32_t seen[kParseTile];          // offset -> wave-0 lane << 16 | blocks before it, or ~0
+    __shared__ uint32_t s_cnt[64], s_ex[64];       // wave 0's walks: blocks, exit (kParseNoExit: dead)
     __shared__ uint32_t s_lo[4], s_hi[4];
     const uint32_t tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
     const uint64_t gt = blockIdx.x;
@@ -1673,6 +1681,8 @@ __global__ void __launch_bounds__(256) k_parse_spec(ParseArgs a, ParseLayout pl)
         return;  // past the chunk's region
     const uint64_t te = ts + kParseTile < ce ? ts + kParseTile : ce;
     const uint32_t ndw = parse_stage<256>(a, ts, te, buf, tid);
+    for (uint32_t i = tid; i < kParseTile; i += 256)
+        seen[i] = ~0u;
     __syncthreads();
     if (wv == 0)
         parse_nonzero_map(buf, ndw, nzb);
@@ -1682,19 +1692,35 @@ __global__ void __launch_bounds__(256) k_parse_spec(ParseArgs a, ParseLayout pl)
     const uint32_t hb = 1u + ((a.g.L + 7u) >> 3);
     uint32_t* table = (uint32_t*) ((char*) a.ws + pl.table_off) + gt * pl.maxb;
     uint32_t lo = 0xFFFFFFFFu, hi = 0;
-    for (uint32_t e0 = 64 * wv; e0 < pl.maxb; e0 += 256) {
-        const uint32_t e = e0 + lane;
-        // one step per loop trip for every lane still inside the tile: the
-        // byte at pos and the 8-byte window from it; a width byte above 32
-        // ends the walk (dead), a zero byte is a one-byte block and the
-        // window's run of zero bytes is taken at once; a run that fills the
-        // window goes on through the nonzero-dword bitmap
-        uint32_t pos = e < pl.maxb ? e : len, cnt = 0;
-        bool dead = false;
+    // Walk candidate e to the tile's end, one step per loop trip for every
+    // lane still inside the tile: the byte at pos and the 8-byte window from
+    // it; a width byte above 32 ends the walk (dead), a zero byte is a
+    // one-byte block and the window's run of zero bytes is taken at once; a
+    // run that fills the window goes on through the nonzero-dword bitmap.
+    // record: note every offset stepped on; else stop at a noted offset.
+    // Returns the exit past the tile's end or kParseNoExit, and the blocks.
+    auto walk = [&](uint32_t e, bool record, uint32_t& cnt) -> uint32_t {
+        uint32_t pos = e < pl.maxb ? e : len, ex = kParseNoExit;
+        bool dead = false, merged = false;
+        cnt = 0;
         while (true) {
-            const bool act = pos < len;
+            bool act = pos < len;
             if (!__any(act))
                 break;
+            if (record) {
+                if (act)
+                    seen[pos] = (lane << 16) | cnt;
+            } else if (act) {
+                const uint32_t m = seen[pos];
+                if (m != ~0u) {
+                    const uint32_t l0 = m >> 16;
+                    ex = s_ex[l0];
+                    cnt += s_cnt[l0] - (m & 0xFFFFu);
+                    merged = true;
+                    pos = len;
+                    act = false;
+                }
+            }
             const uint32_t q = off + (act ? pos : 0u);
             const uint64_t x = ((uint64_t) buf[(q >> 2) + 1] << 32 | buf[q >> 2]) >> (8 * (q & 3));
             const uint32_t w = (uint32_t) x & 0xFFu;
@@ -1716,17 +1742,32 @@ __global__ void __launch_bounds__(256) k_parse_spec(ParseArgs a, ParseLayout pl)
                 }
             }
         }
+        return merged ? ex : (dead ? kParseNoExit : pos - len);
+    };
+    // blocks walked and the exit past the tile's end (kParseNoExit when the
+    // walk met a width byte above 32: past the last chunk's blocks that is
+    // the next segment's seed, not an error)
+    auto keep = [&](uint32_t e, uint32_t ex, uint32_t cnt) {
         if (e < pl.maxb) {
-            // blocks walked, and the exit past the tile's end (kParseNoExit
-            // when the walk met a width byte above 32: past the last chunk's
-            // blocks that is the next segment's seed, not an error)
-            const uint32_t ex = pos - len;
-            table[e] = (cnt << 16) | (dead ? kParseNoExit : ex);
-            if (!dead) {
+            table[e] = (cnt << 16) | ex;
+            if (ex != kParseNoExit) {
                 lo = min(lo, ex);
                 hi = max(hi, ex);
             }
         }
+    };
+    if (wv == 0) {
+        uint32_t cnt;
+        const uint32_t ex = walk(lane, true, cnt);
+        s_cnt[lane] = cnt;
+        s_ex[lane] = ex;
+        keep(lane, ex, cnt);
+    }
+    __syncthreads();
+    for (uint32_t e0 = 64 + 64 * wv; e0 < pl.maxb; e0 += 256) {
+        uint32_t cnt;
+        const uint32_t ex = walk(e0 + lane, false, cnt);
+        keep(e0 + lane, ex, cnt);
     }
     // one exit for every surviving candidate: the next tile's entry
     for (int o = 32; o >= 1; o >>= 1) {
