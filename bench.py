@@ -63,6 +63,9 @@ def parse_args():
     ap.add_argument("--no-3d", action="store_true", help="skip the 3-D codec measurement")
     ap.add_argument("--side3d", type=int, default=512, help="grid side of the 3-D codec's field")
     ap.add_argument("--bs3d", type=int, default=4, help="block side of the 3-D codec")
+    ap.add_argument("--dropin-only", action="store_true",
+                    help="only the reference-ABI drop-in calls (device pointers) and the parse they run, on "
+                         "the first segment of the headline fields; one JSON line")
     ap.add_argument("--codec-only", action="store_true",
                     help="only the batched compress (every rank's P segments) and decompress (rank 0's P "
                          "segments) of the headline fields, timed with HIP events; one JSON line")
@@ -426,6 +429,16 @@ def main():
     log(f"eb={eb:.6g} segments={ar.cnts[0]}x{P}")
     if args.codec_only:
         print(json.dumps(codec_leg(ar, xs, args.steps, args.warmup)), flush=True)
+        return
+    if args.dropin_only:
+        def timed_only(fn, reps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                fn()
+            torch.cuda.synchronize()
+            return (time.perf_counter() - t0) / reps
+        print(json.dumps(dropin_leg(xs[0][: seg_n], xs[1][: seg_n], eb, L, T, timed_only)), flush=True)
         return
 
     def timed(fn, reps):
