@@ -284,9 +284,10 @@ int zccl_batch_bind(zccl_batch *batch, void *device_desc);
 int zccl_tune_threads(int op, int threads);
 int zccl_tune_get_threads(int op);
 
-/* Layout of the homomorphic sum (op 1) and of decompress (op 2) for 36-value
- * blocks: 0 one lane per block, a wave per 64-block tile (the default); 1
- * workgroup tiles of the zccl_tune_threads size.  Affects launches and
+/* Layout of compress (op 0), the homomorphic sum (op 1) and decompress (op 2)
+ * for 36-value blocks: 0 one lane per block (the default; a wave per 64-block
+ * tile, compress a workgroup of four waves per 256-block tile); 1 workgroup
+ * tiles of the zccl_tune_threads size.  Affects launches and
  * plans created afterwards, and zccl_num_tiles() never shrinks below either
  * layout's need; never the stream bytes.  No reference counterpart. */
 int zccl_tune_layout(int op, int layout);

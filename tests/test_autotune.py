@@ -34,7 +34,7 @@ def test_knobs_set_read_back_and_reset(defaults):
 
 
 @pytest.mark.parametrize("op,cfg", [("decompress", {"groups": 6}), ("homsum", {"stage": 100}),
-                                    ("compress", {"threads": 96}), ("compress", {"layout": 0})])
+                                    ("compress", {"threads": 96}), ("compress", {"layout": 2})])
 def test_knobs_reject_out_of_range(defaults, op, cfg):
     with pytest.raises(Exception):
         autotune.set_knobs(op, cfg)

@@ -6,8 +6,9 @@ Knobs (C ABI ``zccl_tune_*``; none changes the stream bytes):
 * ``threads``  -- workgroup size of the workgroup-tiled (slot) kernels, and
                   with it the blocks per tile and the length of each
                   lookback chain (ops compress, homsum, decompress);
-* ``layout``   -- 0 one lane per block, a wave per 64-block tile; 1 the
-                  workgroup-tiled kernels (homsum and decompress, blockSize 36);
+* ``layout``   -- 0 one lane per block (a wave per 64-block tile; compress a
+                  workgroup of four waves per 256 blocks); 1 the
+                  workgroup-tiled kernels (every op, blockSize 36);
 * ``stage``    -- smallest operand stage of the slot-layout sum, in bytes;
 * ``two_pass`` -- sums of at least this many tiles take the two-pass form;
 * ``groups``   -- workgroups per CU of the persistent lane decompress.
@@ -36,7 +37,7 @@ CANDIDATES = {"compress": (256, 512), "homsum": (128, 256), "decompress": (256, 
 # configuration spaces: every knob an op reads; the layout-1 knobs only vary
 # where layout 1 runs
 SPACES: Dict[str, List[Dict[str, int]]] = {
-    "compress": [{"threads": t} for t in (256, 512)],
+    "compress": [{"layout": 0}] + [{"layout": 1, "threads": t} for t in (256, 512)],
     "homsum": ([{"layout": 0, "two_pass": tp} for tp in (1024, 1 << 40)] +
                [{"layout": 1, "threads": t, "stage": st} for t in (128, 256) for st in (0, 32768)]),
     "decompress": ([{"layout": 0, "groups": g} for g in (3, 4, 5)] +
@@ -72,9 +73,7 @@ def set_knobs(op: str, cfg: Dict[str, int]) -> None:
 def get_knobs(op: str) -> Dict[str, int]:
     """The library's current values of every knob op reads."""
     L = lib()
-    k = {"threads": get_threads(op)}
-    if op in ("homsum", "decompress"):
-        k["layout"] = int(L.zccl_tune_get_layout(OPS[op]))
+    k = {"threads": get_threads(op), "layout": int(L.zccl_tune_get_layout(OPS[op]))}
     if op == "homsum":
         k["stage"] = int(L.zccl_tune_get_stage_bytes())
         k["two_pass"] = int(L.zccl_tune_get_two_pass_min_tiles())
@@ -85,9 +84,7 @@ def get_knobs(op: str) -> Dict[str, int]:
 
 def reset(op: str) -> None:
     """Library defaults for op's knobs."""
-    cfg = {"threads": 0}
-    if op in ("homsum", "decompress"):
-        cfg["layout"] = 0
+    cfg = {"threads": 0, "layout": 0}
     if op == "homsum":
         cfg.update(stage=0, two_pass=0)
     if op == "decompress":

@@ -124,14 +124,14 @@ extern "C" int zccl_tune_threads(int op, int threads)
 
 extern "C" int zccl_tune_get_threads(int op) { return op < 0 || op > 2 ? 0 : g_threads[op].load(); }
 
-// Layout of the homomorphic sum (op 1) and of decompress (op 2) for 36-value
-// blocks: 0 one lane per block (zccl_lane.hip, the default), 1 workgroup
-// tiles (slot layout).
-static std::atomic<int> g_layout[3] = {{1}, {0}, {0}};
+// Layout of compress (op 0), the homomorphic sum (op 1) and decompress (op
+// 2) for 36-value blocks: 0 one lane per block (zccl_lane.hip, the default),
+// 1 workgroup tiles (slot layout).
+static std::atomic<int> g_layout[3] = {{0}, {0}, {0}};
 
 extern "C" int zccl_tune_layout(int op, int layout)
 {
-    if ((op != 1 && op != 2) || layout < 0 || layout > 1)
+    if (op < 0 || op > 2 || layout < 0 || layout > 1)
         return ZCCL_EINVAL;
     g_layout[op].store(layout);
     return ZCCL_OK;
@@ -146,12 +146,20 @@ extern "C" unsigned int zccl_tune_get_stage_bytes(void) { return get_stage_min_b
 // Tiling of operation op for geometry g.
 static Tiling op_tiling(int op, const Geom& g)
 {
+    if (op == 0 && g_layout[0].load() == 0 && lane_layout_ok(g))
+        return make_lane_wg_tiling(g);
     if ((op == 1 || op == 2) && g_layout[op].load() == 0 && lane_layout_ok(g))
         return make_lane_tiling(g);
     return make_tiling(g, op_threads(op));
 }
 
 static bool is_lane_tiling(const Geom& g, const Tiling& tl) { return lane_layout_ok(g) && tl.threads == 64 && tl.G == kLaneTileBlocks; }
+
+// The lane-layout compress tiling (a workgroup per kLaneWaves x 64 blocks).
+static bool is_lane_wg_tiling(const Geom& g, const Tiling& tl)
+{
+    return lane_layout_ok(g) && tl.threads == 64 * kLaneWaves && tl.G == kLaneWaves * kLaneTileBlocks;
+}
 
 // ------------------------------------------------------------ sizes
 
@@ -256,7 +264,7 @@ extern "C" int zccl_compress_dev(const float* x, size_t n, float eb, int L, int 
         return ZCCL_EINVAL;
     CompressArgs a;
     a.job.g = g;
-    a.job.tl = make_tiling(g, op_threads(0));
+    a.job.tl = op_tiling(0, g);
     int rc = check_ws(ws, ws_bytes, a.job.tl.ntiles);
     if (rc)
         return rc;
@@ -267,7 +275,10 @@ extern "C" int zccl_compress_dev(const float* x, size_t n, float eb, int L, int 
     a.job.io.index = (uint8_t*) index;
     a.job.io.size_out = d_size;
     a.ws = (WsHeader*) ws;
-    HIP_TRY(launch_compress(a, (hipStream_t) stream));
+    if (is_lane_wg_tiling(g, a.job.tl))
+        HIP_TRY(launch_compress_lane(a, (hipStream_t) stream));
+    else
+        HIP_TRY(launch_compress(a, (hipStream_t) stream));
     return ZCCL_OK;
 }
 
@@ -480,8 +491,8 @@ zccl_batch* batch_plan(BatchKind kind, const std::vector<Job>& jobs, uint32_t ch
     b->kind = kind;
     b->owned = nullptr;
     b->bound = false;
-    b->lane = (kind == BatchKind::Homsum || kind == BatchKind::Decompress) && nj > 0 &&
-              is_lane_tiling(jobs[0].g, jobs[0].tl);
+    b->lane = nj > 0 && (kind == BatchKind::Compress ? is_lane_wg_tiling(jobs[0].g, jobs[0].tl)
+                                                     : is_lane_tiling(jobs[0].g, jobs[0].tl));
     uint32_t threads = 64;
     size_t lds = 0;
     std::vector<unsigned long long> tile0(nj + 1, 0);
@@ -620,7 +631,7 @@ extern "C" zccl_batch* zccl_batch_compress_plan(int njobs, const size_t* nbEle, 
             *st = ZCCL_EINVAL;
             return nullptr;
         }
-        c.tl = make_tiling(c.g, op_threads(0));
+        c.tl = op_tiling(0, c.g);
         c.p = op_params(c.tl, inverse_bound(ebs[j]), ebs[j], 1);
         c.io.x = xs[j];
         c.io.out = outs[j];
@@ -704,6 +715,8 @@ extern "C" int zccl_batch_launch(const zccl_batch* b, void* ws, size_t ws_bytes,
     a.ws = (WsHeader*) ws;
     if (b->lane && b->kind == BatchKind::Homsum)
         HIP_TRY(launch_homsum_lane_batch(a, (hipStream_t) stream));
+    else if (b->lane && b->kind == BatchKind::Compress)
+        HIP_TRY(launch_compress_lane_batch(a, (hipStream_t) stream));
     else if (b->lane)
         HIP_TRY(launch_decompress_lane_batch(a, (hipStream_t) stream));
     else

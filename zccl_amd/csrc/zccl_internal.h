@@ -164,6 +164,21 @@ constexpr uint32_t kLaneWaves = 4;
 
 ZK_HD bool lane_layout_ok(const Geom& g) { return g.L == kLaneL && g.IG == 16; }
 
+// Compress in the lane layout: a workgroup of kLaneWaves waves per tile of
+// kLaneWaves x 64 blocks (one lookback per workgroup).
+ZK_HD Tiling make_lane_wg_tiling(const Geom& g)
+{
+    Tiling t;
+    t.TG = kLaneWaves * kLaneTileBlocks / g.IG;
+    t.G = kLaneWaves * kLaneTileBlocks;
+    t.threads = 64 * kLaneWaves;
+    t.tilesT = (uint32_t) ceil_div(g.gT, t.TG);
+    t.tilesR = g.gR ? (uint32_t) ceil_div(g.gR, t.TG) : 0u;
+    t.ntiles = g.T * t.tilesT + t.tilesR;
+    t.tile_bytes_max = t.G * block_bytes(32, g.L);
+    return t;
+}
+
 ZK_HD Tiling make_lane_tiling(const Geom& g)
 {
     Tiling t;

@@ -1199,6 +1199,153 @@ __device__ __forceinline__ void decompress_scan_lane(const Geom& g, const Tiling
     }
 }
 
+// ---------------------------------------------------------------- compress
+//
+// A workgroup of four waves owns a tile of 256 blocks (make_lane_wg_tiling),
+// wave w blocks 64w .. 64w + 63, one block per lane.  Each wave stages its
+// blocks' values and the one before them into its LDS buffer (16-byte
+// direct-to-LDS loads through a range-checked view of the input; reads past
+// the input return zeros), each lane reads its block's 36 values and the one
+// before, quantizes them (FP64 product, truncation), takes the deltas, the
+// sign words, the magnitudes and the width, and the wave scans the block
+// sizes.  The workgroup adds its waves' sizes, publishes the tile's
+// aggregate, encodes every block into the wave buffers (the lane encoders of
+// the sum), resolves the tile's stream offset by the decoupled lookback
+// (one chain per stream, as the slot kernel), and each wave stores its bytes.
+// Arithmetic: ZCCL/src/ZCCL_float.c:563,577-606 (quantize, delta, width),
+// ZCCL/src/ZCCL_TypeManager.c:12-91, 824-848 (packing, signs).
+constexpr uint32_t kCompBuf = 9728;  // a wave's values (<= 2305 floats) or a worst-case 64-block encode + reach
+static_assert(kCompBuf >= kLaneTileBlocks * kWorstBlock + 64 && kCompBuf >= 4 * (kLaneTileBlocks * L + 1) + 16,
+    "a compress wave buffer holds the staged values and the worst-case encode");
+static_assert(kCompBuf % 16 == 0, "compress wave buffers stay 16-byte aligned");
+
+struct CompShared {
+    uint32_t wtot[kLaneWaves];
+    unsigned long long ex;
+};
+
+__device__ __forceinline__ void compress_tile_lane(const Geom& g, const Tiling& tl, double inv, const float* xin,
+    uint8_t* out, uint64_t out_cap, uint8_t* oindex, unsigned long long* size_out, uint64_t t,
+    unsigned long long* status, WsHeader* ws, uint32_t nchains, uint32_t* obuf, CompShared& sh)
+{
+    const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const Tile x = tile_of(g, tl, (uint32_t) t);
+    const uint32_t jb = kLaneTileBlocks * wv + lane;  // block of the tile
+    const bool has = jb < x.nblk;
+    const uint32_t nj = has ? min(tile_block_len(g, x, jb), L) : 0u;
+    // the wave's values and the one before them (the segment's first value,
+    // the seed, before its first block)
+    const uint32_t nbw = x.nblk > kLaneTileBlocks * wv ? min(x.nblk - kLaneTileBlocks * wv, kLaneTileBlocks) : 0u;
+    const uint32_t nvals = nbw ? (nbw - 1) * L + min(tile_block_len(g, x, kLaneTileBlocks * wv + nbw - 1), L) : 0u;
+    const uint32_t v0 = x.e0 + kLaneTileBlocks * L * wv - 1;  // input index of LDS float 0
+    if (nbw) {
+        // (a view of the wave's values alone: its range check ends them)
+        const View vx = make_view((const uint8_t*) (xin + v0), 4ull * (nvals + 1));
+        const uint32_t nch = (4 * (nvals + 1) + 15) >> 4;
+        for (uint32_t run = 0; run * 64 < nch; run++) {
+            const uint32_t ch = run * 64 + lane;
+            if (ch < nch)
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(vx,
+                    (__attribute__((address_space(3))) void*) ((uint8_t*) obuf + run * 1024), 16, 16 * ch, 0, 0, 0);
+        }
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    // quantize, deltas, signs and magnitudes (values past nj are zero)
+    const float* xv = reinterpret_cast<const float*>(obuf) + lane * L;
+    uint32_t m[L];
+    uint32_t s = 0, s2 = 0, mx = 0;
+    {
+        uint32_t prev = (uint32_t) quant(xv[0], inv);
+#pragma unroll
+        for (uint32_t j = 0; j < L; j++) {
+            const uint32_t q = (uint32_t) quant(xv[j + 1], inv);
+            const uint32_t d = j < nj ? q - prev : 0u;
+            prev = q;
+            if (j < 32)
+                s = __builtin_amdgcn_alignbit(s, d, 31);  // (s << 1) | sign of value j
+            else
+                s2 = __builtin_amdgcn_alignbit(s2, d, 31);
+            m[j] = mag(d);
+            mx = max(mx, m[j]);
+        }
+    }
+    const uint32_t w = has ? dwidth(mx) : 0u;
+    const uint32_t bs = has ? (nj == L ? full_block_bytes(w) : lane_block_bytes(w, nj)) : 0u;
+    const uint32_t incl = wave_scan_dpp(bs);
+    const uint32_t excl = incl - bs;
+    const uint32_t wtot = __builtin_amdgcn_readlane(incl, 63);
+    if (lane == 0)
+        sh.wtot[wv] = wtot;
+    __syncthreads();  // every wave's values are read: the buffers are free for the encode
+    uint32_t woff = 0, total = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < kLaneWaves; i++) {
+        const uint32_t v = sh.wtot[i];
+        woff += i < wv ? v : 0u;
+        total += v;
+    }
+    if (threadIdx.x == 0)
+        lookback_publish(status, t, 0, total, 0);  // successors see the aggregate now
+    const bool fast = nj == L && w < 16;
+    const bool wide = __any(has && w >= 8);
+    const uint32_t nz = (wtot + 64 + 15) / 16;
+    for (uint32_t i = lane; i < nz; i += 64)
+        reinterpret_cast<uint4*>(obuf)[i] = make_uint4(0, 0, 0, 0);
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    // (two separate branches rather than if/else: the compiler would hoist
+    // what the two encoders share above an if/else and keep it live)
+    if (has && w && fast)
+        encode_fast(obuf, excl, w, s, s2, m, wide);
+    if (has && w && !fast)
+        encode_general(obuf, excl, w, nj, s, s2, m);
+    if (wv == 0) {
+        const unsigned long long ex = lookback(status, t, 0, total, 0, ws, true);
+        if (lane == 0)
+            sh.ex = ex;
+    }
+    __syncthreads();
+    const uint64_t P = 8ull * g.T + 4ull * (x.s + 1) + sh.ex;
+    const bool fits = P + total <= out_cap;
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    if (fits)
+        store_wave(obuf, out + P + woff, wtot);
+    else if (threadIdx.x == 0)
+        raise_error(ws, ZK_ERR_NOSPACE);
+    IndexView ox = index_view(g, oindex);
+    if (has)
+        ox.widths[x.g0 + jb] = (uint8_t) w;
+    if (has && (jb % g.IG) == 0)
+        ox.group_off[x.grp0 + jb / g.IG] = P + woff + excl;
+    if (threadIdx.x == 0) {
+        if (x.nblk == 0 && x.ngrp)
+            ox.group_off[x.grp0] = P;  // a segment of its seed alone keeps one group entry
+        if (x.lt == 0) {
+            const uint32_t seed = (uint32_t) quant(xin[x.seg_a], inv);
+            const uint64_t so = P - 4;
+            if (so + 4 <= out_cap) {
+                store_bytes(out + so, &seed, 4);
+                if (x.s < g.T) {
+                    const unsigned long long h = so - 8ull * g.T;
+                    store_bytes(out + 8ull * x.s, &h, 8);
+                }
+            }
+            ox.seed_off[x.s] = so;
+        }
+        if (t == tl.ntiles - 1) {
+            // the last tile ends the stream: if it fits, every tile fits
+            const unsigned long long sz = fits ? P + total : 0ull;
+            ox.meta[0] = sz;
+            if (size_out)
+                *size_out = sz;
+            chain_finished(ws, nchains);
+        }
+    }
+}
+
 }  // namespace
 
 // One fused sum (kernel arguments hold its geometry and pointers).
@@ -1306,6 +1453,68 @@ __global__ void __launch_bounds__(64 * kLaneWaves, kDecGroupsPerCU) k_decompress
 __global__ void __launch_bounds__(64 * kLaneWaves, kDecGroupsPerCU) k_decompress_lane_batch_out(BatchArgs b)
 {
     dec_pass<2>(b, b.ws);
+}
+
+// Compress in the lane layout: one stream, a uniform batch (shared geometry
+// and bound, per-job pointers, tiles of the jobs interleaved) or a mixed one.
+__global__ void __launch_bounds__(64 * kLaneWaves, 4) k_compress_lane(CompressArgs a)
+{
+    __shared__ CompShared sh;
+    uint64_t t;
+    unsigned long long* status;
+    if (!launch_status(a.ws, a.job.tl.ntiles, t, status))
+        return;
+    compress_tile_lane(a.job.g, a.job.tl, a.job.p.inv, a.job.io.x, a.job.io.out, a.job.io.out_cap, a.job.io.index,
+        a.job.io.size_out, t, status, a.ws, 1, zk_lds + (threadIdx.x >> 6) * (kCompBuf / 4), sh);
+}
+
+__global__ void __launch_bounds__(64 * kLaneWaves, 4) k_compress_lane_batch(BatchArgs b)
+{
+    __shared__ CompShared sh;
+    uint64_t tg;
+    unsigned long long* status;
+    if (!launch_status(b.ws, b.total_tiles, tg, status))
+        return;
+    uint32_t jidx, t;
+    if (b.interleave) {
+        jidx = (uint32_t) tg % b.njobs;
+        t = (uint32_t) tg / b.njobs;
+    } else {
+        jidx = 0;
+        t = (uint32_t) tg;
+    }
+    jidx = __builtin_amdgcn_readfirstlane(jidx);
+    t = __builtin_amdgcn_readfirstlane(t);
+    const CompressIO* io = (const CompressIO*) b.jobs + jidx;
+    compress_tile_lane(b.g, b.tl, b.p.inv, ldc(&io->x), ldc(&io->out), ldc(&io->out_cap), ldc(&io->index),
+        ldc(&io->size_out), t, status + (uint64_t) jidx * b.tl.ntiles * kStatusStride, b.ws, b.total_chains,
+        zk_lds + (threadIdx.x >> 6) * (kCompBuf / 4), sh);
+}
+
+__global__ void __launch_bounds__(64 * kLaneWaves, 4) k_compress_lane_mixed(BatchArgs b)
+{
+    __shared__ CompShared sh;
+    uint64_t tg;
+    unsigned long long* status;
+    if (!launch_status(b.ws, b.total_tiles, tg, status))
+        return;
+    uint32_t lo = 0, hi = b.njobs;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (ldc(&b.job_tile0[mid]) <= tg)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    const uint32_t jidx = __builtin_amdgcn_readfirstlane(lo);
+    const uint64_t tile0 = ldc(&b.job_tile0[jidx]);
+    const uint32_t t = __builtin_amdgcn_readfirstlane((uint32_t) (tg - tile0));
+    const CompressJob* job = (const CompressJob*) b.jobs + jidx;
+    const Geom g = job->g;
+    const Tiling tl = job->tl;
+    compress_tile_lane(g, tl, ldc(&job->p.inv), ldc(&job->io.x), ldc(&job->io.out), ldc(&job->io.out_cap),
+        ldc(&job->io.index), ldc(&job->io.size_out), t, status + tile0 * kStatusStride, b.ws, b.total_chains,
+        zk_lds + (threadIdx.x >> 6) * (kCompBuf / 4), sh);
 }
 
 // Scans: workgroup (s, j) takes segment s of job j.
@@ -1625,6 +1834,26 @@ hipError_t launch_decompress_lane_batch(const BatchArgs& b, hipStream_t s)
     hipLaunchKernelGGL(k_decompress_lane_batch_sums, grid, block, lane_lds_bytes(), s, b);
     hipLaunchKernelGGL(k_decompress_lane_batch_scan, dim3(b.max_segments, b.njobs), dim3(kDecScanThreads), 0, s, b);
     hipLaunchKernelGGL(k_decompress_lane_batch_out, grid, block, lane_lds_bytes(), s, b);
+    return hipGetLastError();
+}
+
+size_t compress_lane_lds_bytes() { return (size_t) kLaneWaves * kCompBuf; }
+
+hipError_t launch_compress_lane(const CompressArgs& a, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_compress_lane, dim3((uint32_t) a.job.tl.ntiles), dim3(64 * kLaneWaves),
+        compress_lane_lds_bytes(), s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_compress_lane_batch(const BatchArgs& b, hipStream_t s)
+{
+    if (b.uniform)
+        hipLaunchKernelGGL(k_compress_lane_batch, dim3((uint32_t) b.total_tiles), dim3(64 * kLaneWaves),
+            compress_lane_lds_bytes(), s, b);
+    else
+        hipLaunchKernelGGL(k_compress_lane_mixed, dim3((uint32_t) b.total_tiles), dim3(64 * kLaneWaves),
+            compress_lane_lds_bytes(), s, b);
     return hipGetLastError();
 }
 
