@@ -287,7 +287,8 @@ int zccl_tune_get_threads(int op);
 /* Layout of compress (op 0), the homomorphic sum (op 1) and decompress (op 2)
  * for 36-value blocks: 0 one lane per block (the default; a wave per 64-block
  * tile, compress a workgroup of four waves per 256-block tile); 1 workgroup
- * tiles of the zccl_tune_threads size.  Affects launches and
+ * tiles of the zccl_tune_threads size; 2 (sum only) one lane per block in
+ * single-pass workgroup tiles of four waves (no two-pass form).  Affects launches and
  * plans created afterwards, and zccl_num_tiles() never shrinks below either
  * layout's need; never the stream bytes.  No reference counterpart. */
 int zccl_tune_layout(int op, int layout);

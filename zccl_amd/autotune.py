@@ -8,7 +8,9 @@ Knobs (C ABI ``zccl_tune_*``; none changes the stream bytes):
                   lookback chain (ops compress, homsum, decompress);
 * ``layout``   -- 0 one lane per block (a wave per 64-block tile; compress a
                   workgroup of four waves per 256 blocks); 1 the
-                  workgroup-tiled kernels (every op, blockSize 36);
+                  workgroup-tiled kernels (every op, blockSize 36); 2 (sum
+                  only) one lane per block, single pass, a lookback tile per
+                  workgroup of four waves;
 * ``stage``    -- smallest operand stage of the slot-layout sum, in bytes;
 * ``two_pass`` -- sums of at least this many tiles take the two-pass form;
 * ``groups``   -- workgroups per CU of the persistent lane decompress.
@@ -38,7 +40,7 @@ CANDIDATES = {"compress": (256, 512), "homsum": (128, 256), "decompress": (256, 
 # where layout 1 runs
 SPACES: Dict[str, List[Dict[str, int]]] = {
     "compress": [{"layout": 0}] + [{"layout": 1, "threads": t} for t in (256, 512)],
-    "homsum": ([{"layout": 0, "two_pass": tp} for tp in (1024, 1 << 40)] +
+    "homsum": ([{"layout": 0, "two_pass": tp} for tp in (1024, 1 << 40)] + [{"layout": 2}] +
                [{"layout": 1, "threads": t, "stage": st} for t in (128, 256) for st in (0, 32768)]),
     "decompress": ([{"layout": 0, "groups": g} for g in (3, 4, 5)] +
                    [{"layout": 1, "threads": t} for t in (256, 512)]),

@@ -126,12 +126,13 @@ extern "C" int zccl_tune_get_threads(int op) { return op < 0 || op > 2 ? 0 : g_t
 
 // Layout of compress (op 0), the homomorphic sum (op 1) and decompress (op
 // 2) for 36-value blocks: 0 one lane per block (zccl_lane.hip, the default),
-// 1 workgroup tiles (slot layout).
+// 1 workgroup tiles (slot layout); for the sum, 2 one lane per block in
+// single-pass workgroup tiles of four waves.
 static std::atomic<int> g_layout[3] = {{0}, {0}, {0}};
 
 extern "C" int zccl_tune_layout(int op, int layout)
 {
-    if (op < 0 || op > 2 || layout < 0 || layout > 1)
+    if (op < 0 || op > 2 || layout < 0 || layout > (op == 1 ? 2 : 1))
         return ZCCL_EINVAL;
     g_layout[op].store(layout);
     return ZCCL_OK;
@@ -146,7 +147,7 @@ extern "C" unsigned int zccl_tune_get_stage_bytes(void) { return get_stage_min_b
 // Tiling of operation op for geometry g.
 static Tiling op_tiling(int op, const Geom& g)
 {
-    if (op == 0 && g_layout[0].load() == 0 && lane_layout_ok(g))
+    if (((op == 0 && g_layout[0].load() == 0) || (op == 1 && g_layout[1].load() == 2)) && lane_layout_ok(g))
         return make_lane_wg_tiling(g);
     if ((op == 1 || op == 2) && g_layout[op].load() == 0 && lane_layout_ok(g))
         return make_lane_tiling(g);
@@ -344,6 +345,8 @@ extern "C" int zccl_homsum_dev(const unsigned char* const* ins, const void* cons
         return rc;
     if (is_lane_tiling(a.job.g, a.job.tl))
         HIP_TRY(launch_homsum_lane(a, (hipStream_t) stream));
+    else if (is_lane_wg_tiling(a.job.g, a.job.tl))
+        HIP_TRY(launch_homsum_lane_wg(a, (hipStream_t) stream));
     else
         HIP_TRY(launch_homsum(a, (hipStream_t) stream));
     return ZCCL_OK;
@@ -391,6 +394,8 @@ extern "C" int zccl_homsum_dev_staged(const unsigned char* const* ins, const voi
         HIP_TRY(launch_homsum_lane_staged(a, (uint8_t*) stage, (hipStream_t) stream));
     else if (is_lane_tiling(a.job.g, a.job.tl))
         HIP_TRY(launch_homsum_lane(a, (hipStream_t) stream));
+    else if (is_lane_wg_tiling(a.job.g, a.job.tl))
+        HIP_TRY(launch_homsum_lane_wg(a, (hipStream_t) stream));
     else
         HIP_TRY(launch_homsum(a, (hipStream_t) stream));
     return ZCCL_OK;
@@ -470,7 +475,8 @@ struct zccl_batch {
     size_t tile0_off;                 // byte offset of the tile starts in `host`
     void* owned;                      // device copy owned by the plan (create()), else null
     bool bound;
-    bool lane;                        // homomorphic sum or decompress in the lane layout
+    bool lane;                        // compress, homomorphic sum or decompress in the lane layout
+    bool wg;                          // homomorphic sum in single-pass workgroup tiles (layout 2)
 };
 
 namespace {
@@ -491,8 +497,11 @@ zccl_batch* batch_plan(BatchKind kind, const std::vector<Job>& jobs, uint32_t ch
     b->kind = kind;
     b->owned = nullptr;
     b->bound = false;
+    b->wg = kind == BatchKind::Homsum && nj > 0 && is_lane_wg_tiling(jobs[0].g, jobs[0].tl);
     b->lane = nj > 0 && (kind == BatchKind::Compress ? is_lane_wg_tiling(jobs[0].g, jobs[0].tl)
-                                                     : is_lane_tiling(jobs[0].g, jobs[0].tl));
+                         : kind == BatchKind::Homsum
+                             ? is_lane_tiling(jobs[0].g, jobs[0].tl) || is_lane_wg_tiling(jobs[0].g, jobs[0].tl)
+                             : is_lane_tiling(jobs[0].g, jobs[0].tl));
     uint32_t threads = 64;
     size_t lds = 0;
     std::vector<unsigned long long> tile0(nj + 1, 0);
@@ -713,7 +722,9 @@ extern "C" int zccl_batch_launch(const zccl_batch* b, void* ws, size_t ws_bytes,
         return rc;
     BatchArgs a = b->args;
     a.ws = (WsHeader*) ws;
-    if (b->lane && b->kind == BatchKind::Homsum)
+    if (b->wg)
+        HIP_TRY(launch_homsum_lane_wg_batch(a, (hipStream_t) stream));
+    else if (b->lane && b->kind == BatchKind::Homsum)
         HIP_TRY(launch_homsum_lane_batch(a, (hipStream_t) stream));
     else if (b->lane && b->kind == BatchKind::Compress)
         HIP_TRY(launch_compress_lane_batch(a, (hipStream_t) stream));
@@ -729,7 +740,7 @@ extern "C" int zccl_batch_launch_staged(const zccl_batch* b, void* ws, size_t ws
 {
     if (!b || !b->bound)
         return ZCCL_EINVAL;
-    if (!(b->lane && b->kind == BatchKind::Homsum && b->args.uniform &&
+    if (!(b->lane && !b->wg && b->kind == BatchKind::Homsum && b->args.uniform &&
           stage_ok(stage, stage_bytes, b->args.total_tiles)))
         return zccl_batch_launch(b, ws, ws_bytes, stream);
     int rc = check_ws(ws, ws_bytes, b->args.total_tiles);

@@ -212,6 +212,8 @@ hipError_t launch_parse(const ParseArgs& a, hipStream_t s);
 size_t lane_lds_bytes();
 hipError_t launch_homsum_lane(const HomsumArgs& a, hipStream_t s);
 hipError_t launch_homsum_lane_batch(const BatchArgs& b, hipStream_t s);
+hipError_t launch_homsum_lane_wg(const HomsumArgs& a, hipStream_t s);
+hipError_t launch_homsum_lane_wg_batch(const BatchArgs& b, hipStream_t s);
 hipError_t launch_compress_lane(const CompressArgs& a, hipStream_t s);
 hipError_t launch_compress_lane_batch(const BatchArgs& b, hipStream_t s);
 hipError_t launch_decompress_lane(const DecompressArgs& a, hipStream_t s);

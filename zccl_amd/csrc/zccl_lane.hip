@@ -450,19 +450,36 @@ __device__ __forceinline__ void read_block_direct(View v, uint32_t o, uint32_t w
     }
 }
 
+// Shared state of a workgroup of lane waves that owns one lookback tile:
+// each wave's bytes, and the tile's stream offset.
+struct WgShared {
+    uint32_t wtot[kLaneWaves];
+    unsigned long long ex;
+};
+
+// Forms of the lane sum's output side.
+enum SumMode : int {
+    kSumWave = 0,    // single pass, a lookback tile per wave
+    kSumStaged = 1,  // pass one of the two-pass sum: no lookback; the tile's bytes go to
+                     // slot sidx of the stage area `area` of R tiles, its sizes and seed to
+                     // record sidx, and only the widths to the output index
+    kSumWg = 2,      // single pass, a lookback tile per workgroup of kLaneWaves waves
+};
+
 // One tile of a fused k-operand sum, executed by one wave.  obuf: the wave's
-// LDS buffer (kOutBuf bytes, 16-byte aligned).
-// Staged (pass one of the two-pass sum): no lookback; the tile's bytes go to
-// slot sidx of the stage area `area` of R tiles, its sizes and seed to record sidx,
-// and only the widths to the output index.
-template <bool Staged, typename Src>
+// LDS buffer (kOutBuf bytes, 16-byte aligned).  kSumWg: t is the wave's tile
+// of the lane tiling, t_wg the workgroup's lookback tile (the last of the
+// launch when wg_last), wsh the workgroup's shared state; every wave of the
+// workgroup calls this (a wave past its segment's blocks with an empty tile).
+template <int Mode, typename Src>
 __device__ __forceinline__ void homsum_tile_lane(const Geom& g, const Tiling& tl, int k, const Src& src,
     uint8_t* out, uint64_t out_cap, uint8_t* oindex, unsigned long long* size_out, uint64_t t,
     unsigned long long* status, WsHeader* ws, uint32_t nchains, uint32_t* obuf, uint8_t* area = nullptr,
-    uint64_t R = 0, uint64_t sidx = 0)
+    uint64_t R = 0, uint64_t sidx = 0, uint64_t t_wg = 0, bool wg_last = false, WgShared* wsh = nullptr,
+    const Tile* xe = nullptr)
 {
     const uint32_t lane = threadIdx.x & 63;
-    const Tile x = tile_of(g, tl, (uint32_t) t);
+    const Tile x = xe ? *xe : tile_of(g, tl, (uint32_t) t);
     const bool has = lane < x.nblk;
     const uint32_t nj = has ? min(tile_block_len(g, x, lane), L) : 0u;
     bool bad_byte = false, bad_width = false, bad_over = false;
@@ -679,12 +696,95 @@ __device__ __forceinline__ void homsum_tile_lane(const Geom& g, const Tiling& tl
     const uint32_t incl = wave_scan_dpp(bs);
     const uint32_t excl = incl - bs;
     const uint32_t total = __builtin_amdgcn_readlane(incl, 63);
-    if (!Staged && lane == 0)
+    if (Mode == kSumWave && lane == 0)
         lookback_publish(status, t, 0, total, 0);  // successors see the aggregate now
     const bool fast = nj == L && w < 16;
     const bool wide = __any(has && w >= 8);
 
-    if constexpr (Staged) {
+    if constexpr (Mode == kSumWg) {
+        // the workgroup's tile: its waves' bytes in wave order, one lookback
+        const uint32_t wv = threadIdx.x >> 6;
+        if (lane == 0)
+            wsh->wtot[wv] = total;
+        __syncthreads();
+        uint32_t woff = 0, gtot = 0;
+#pragma unroll
+        for (uint32_t i = 0; i < kLaneWaves; i++) {
+            const uint32_t v = wsh->wtot[i];
+            woff += i < wv ? v : 0u;
+            gtot += v;
+        }
+        if (threadIdx.x == 0)
+            lookback_publish(status, t_wg, 0, gtot, 0);  // successors see the aggregate now
+        const bool split = total + 64 > kOutBuf;
+        if (!split) {
+            const uint32_t nz = (total + 64 + 15) / 16;
+            for (uint32_t i = lane; i < nz; i += 64)
+                reinterpret_cast<uint4*>(obuf)[i] = make_uint4(0, 0, 0, 0);
+            if (has && w && fast)
+                encode_fast(obuf, excl, w, s, s2, acc, wide);
+            if (has && w && !fast)
+                encode_general(obuf, excl, w, nj, s, s2, acc);
+        }
+        if (wv == 0) {
+            const unsigned long long ex = lookback(status, t_wg, 0, gtot, 0, ws, true);
+            if (lane == 0)
+                wsh->ex = ex;
+        }
+        __syncthreads();
+        const uint64_t hdr = 8ull * g.T + 4ull * (x.s + 1);
+        const uint64_t P0 = hdr + wsh->ex;  // the workgroup tile's stream offset
+        const uint64_t P = P0 + woff;       // this wave's
+        const bool fits = P0 + gtot <= out_cap;
+        if (!split) {
+            if (fits)
+                store_wave(obuf, out + P, total);
+        } else {
+            // a wave too wide for the buffer: half a wave at a time, value by value
+            const uint32_t mid = __builtin_amdgcn_readlane(excl, 32);
+            for (uint32_t i = lane; i < (mid + 64 + 15) / 16; i += 64)
+                reinterpret_cast<uint4*>(obuf)[i] = make_uint4(0, 0, 0, 0);
+            if (lane < 32 && has && w)
+                encode_general(obuf, excl, w, nj, s, s2, acc);
+            if (fits)
+                store_wave(obuf, out + P, mid);
+            for (uint32_t i = lane; i < (total - mid + 64 + 15) / 16; i += 64)
+                reinterpret_cast<uint4*>(obuf)[i] = make_uint4(0, 0, 0, 0);
+            if (lane >= 32 && has && w)
+                encode_general(obuf, excl - mid, w, nj, s, s2, acc);
+            if (fits)
+                store_wave(obuf, out + P + mid, total - mid);
+        }
+        if (!fits && threadIdx.x == 0)
+            raise_error(ws, ZK_ERR_NOSPACE);
+        IndexView ox = index_view(g, oindex);
+        if (has)
+            ox.widths[x.g0 + lane] = (uint8_t) w;
+        if ((lane & 15) == 0 && lane / 16 < x.ngrp)
+            ox.group_off[x.grp0 + lane / 16] = P + excl;
+        if (lane == 0 && x.lt == 0) {
+            const uint64_t so = P0 - 4;
+            if (so + 4 <= out_cap) {
+                store_bytes(out + so, &seed, 4);
+                if (x.s < g.T) {
+                    const unsigned long long h = so - 8ull * g.T;
+                    store_bytes(out + 8ull * x.s, &h, 8);
+                }
+            }
+            ox.seed_off[x.s] = so;
+        }
+        if (threadIdx.x == 0 && wg_last) {
+            // the last tile ends the stream: if it fits, every tile fits
+            const unsigned long long sz = fits ? P0 + gtot : 0ull;
+            ox.meta[0] = sz;
+            if (size_out)
+                *size_out = sz;
+            chain_finished(ws, nchains);
+        }
+        return;
+    }
+
+    if constexpr (Mode == kSumStaged) {
         uint8_t* slot = area + (8 + sizeof(StageRec)) * R + kStageSlot * sidx;
         if (total + 64 <= kOutBuf) {
             const uint32_t nz = (total + 64 + 15) / 16;
@@ -1360,7 +1460,7 @@ __global__ void __launch_bounds__(64 * kLaneWaves, 4) k_homsum_lane(HomsumArgs a
     if (t >= a.job.tl.ntiles)
         return;
     const ArgIO src{a.job.io};
-    homsum_tile_lane<false>(a.job.g, a.job.tl, a.job.p.k, src, a.job.io.out, a.job.io.out_cap, a.job.io.index,
+    homsum_tile_lane<kSumWave>(a.job.g, a.job.tl, a.job.p.k, src, a.job.io.out, a.job.io.out_cap, a.job.io.index,
         a.job.io.size_out, t, status, a.ws, 1, zk_lds + wv * (kOutBuf / 4));
 }
 
@@ -1388,7 +1488,7 @@ __global__ void __launch_bounds__(64 * kLaneWaves, 5) k_homsum_lane_batch(BatchA
     t = __builtin_amdgcn_readfirstlane(t);
     const HomsumIO* io = (const HomsumIO*) b.jobs + jidx;
     const SumIO src{io};
-    homsum_tile_lane<false>(b.g, b.tl, b.p.k, src, ldc(&io->out), ldc(&io->out_cap), ldc(&io->index),
+    homsum_tile_lane<kSumWave>(b.g, b.tl, b.p.k, src, ldc(&io->out), ldc(&io->out_cap), ldc(&io->index),
         ldc(&io->size_out), t, status + (uint64_t) jidx * b.tl.ntiles * kStatusStride, b.ws, b.total_chains,
         zk_lds + wv * (kOutBuf / 4));
 }
@@ -1420,7 +1520,7 @@ __global__ void __launch_bounds__(64 * kLaneWaves, 4) k_homsum_lane_mixed(BatchA
     const Geom g = job->g;
     const Tiling tl = job->tl;
     const SumIO src{&job->io};
-    homsum_tile_lane<false>(g, tl, ldc(&job->p.k), src, ldc(&job->io.out), ldc(&job->io.out_cap), ldc(&job->io.index),
+    homsum_tile_lane<kSumWave>(g, tl, ldc(&job->p.k), src, ldc(&job->io.out), ldc(&job->io.out_cap), ldc(&job->io.index),
         ldc(&job->io.size_out), t, status + tile0 * kStatusStride, b.ws, b.total_chains, zk_lds + wv * (kOutBuf / 4));
 }
 
@@ -1546,6 +1646,104 @@ __global__ void __launch_bounds__(kDecScanThreads) k_decompress_lane_batch_scan(
     }
 }
 
+// Single-pass sum with a lookback tile per workgroup (layout 2): the
+// workgroup's tile comes from make_lane_wg_tiling, each wave takes one tile
+// of the lane tiling inside it.  The wave's lane tile (its index in t_lane),
+// or an empty tile past its segment's blocks.
+__device__ __forceinline__ Tile wg_wave_tile(const Geom& g, const Tiling& tl, const Tiling& tw, uint32_t t_wg,
+    uint32_t wv, uint32_t& t_lane)
+{
+    const Tile xw = tile_of(g, tw, t_wg);
+    const uint32_t base = xw.s < g.T ? xw.s * tl.tilesT : g.T * tl.tilesT;
+    const uint32_t seg_tiles = xw.s < g.T ? tl.tilesT : tl.tilesR;
+    const uint32_t lt = kLaneWaves * xw.lt + wv;
+    t_lane = base + min(lt, seg_tiles - 1);
+    Tile x = tile_of(g, tl, t_lane);
+    if (lt >= seg_tiles) {
+        x.nblk = 0;
+        x.ngrp = 0;
+        x.lt = lt;
+    }
+    return x;
+}
+
+__global__ void __launch_bounds__(64 * kLaneWaves, 4) k_homsum_lane_wg(HomsumArgs a)
+{
+    __shared__ WgShared sh;
+    uint64_t t_wg;
+    unsigned long long* status;
+    if (!launch_status(a.ws, a.job.tl.ntiles, t_wg, status))
+        return;
+    const uint32_t wv = threadIdx.x >> 6;
+    const Tiling tl = make_lane_tiling(a.job.g);
+    uint32_t t_lane;
+    const Tile x = wg_wave_tile(a.job.g, tl, a.job.tl, (uint32_t) t_wg, wv, t_lane);
+    const ArgIO src{a.job.io};
+    homsum_tile_lane<kSumWg>(a.job.g, tl, a.job.p.k, src, a.job.io.out, a.job.io.out_cap, a.job.io.index,
+        a.job.io.size_out, t_lane, status, a.ws, 1, zk_lds + wv * (kOutBuf / 4), nullptr, 0, 0, t_wg,
+        t_wg == a.job.tl.ntiles - 1, &sh, &x);
+}
+
+// (four workgroups per CU: at five the workgroup state spills)
+__global__ void __launch_bounds__(64 * kLaneWaves, 4) k_homsum_lane_wg_batch(BatchArgs b)
+{
+    __shared__ WgShared sh;
+    uint64_t tg;
+    unsigned long long* status;
+    if (!launch_status(b.ws, b.total_tiles, tg, status))
+        return;
+    uint32_t jidx, t;
+    if (b.interleave) {
+        jidx = (uint32_t) tg % b.njobs;
+        t = (uint32_t) tg / b.njobs;
+    } else {
+        jidx = 0;
+        t = (uint32_t) tg;
+    }
+    jidx = __builtin_amdgcn_readfirstlane(jidx);
+    t = __builtin_amdgcn_readfirstlane(t);
+    const uint32_t wv = threadIdx.x >> 6;
+    const Tiling tl = make_lane_tiling(b.g);
+    uint32_t t_lane;
+    const Tile x = wg_wave_tile(b.g, tl, b.tl, t, wv, t_lane);
+    const HomsumIO* io = (const HomsumIO*) b.jobs + jidx;
+    const SumIO src{io};
+    homsum_tile_lane<kSumWg>(b.g, tl, b.p.k, src, ldc(&io->out), ldc(&io->out_cap), ldc(&io->index),
+        ldc(&io->size_out), t_lane, status + (uint64_t) jidx * b.tl.ntiles * kStatusStride, b.ws, b.total_chains,
+        zk_lds + wv * (kOutBuf / 4), nullptr, 0, 0, t, t == b.tl.ntiles - 1, &sh, &x);
+}
+
+__global__ void __launch_bounds__(64 * kLaneWaves, 4) k_homsum_lane_wg_mixed(BatchArgs b)
+{
+    __shared__ WgShared sh;
+    uint64_t tg;
+    unsigned long long* status;
+    if (!launch_status(b.ws, b.total_tiles, tg, status))
+        return;
+    uint32_t lo = 0, hi = b.njobs;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (ldc(&b.job_tile0[mid]) <= tg)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    const uint32_t jidx = __builtin_amdgcn_readfirstlane(lo);
+    const uint64_t tile0 = ldc(&b.job_tile0[jidx]);
+    const uint32_t t = __builtin_amdgcn_readfirstlane((uint32_t) (tg - tile0));
+    const HomsumJob* job = (const HomsumJob*) b.jobs + jidx;
+    const Geom g = job->g;
+    const Tiling tw = job->tl;
+    const uint32_t wv = threadIdx.x >> 6;
+    const Tiling tl = make_lane_tiling(g);
+    uint32_t t_lane;
+    const Tile x = wg_wave_tile(g, tl, tw, t, wv, t_lane);
+    const SumIO src{&job->io};
+    homsum_tile_lane<kSumWg>(g, tl, ldc(&job->p.k), src, ldc(&job->io.out), ldc(&job->io.out_cap),
+        ldc(&job->io.index), ldc(&job->io.size_out), t_lane, status + tile0 * kStatusStride, b.ws, b.total_chains,
+        zk_lds + wv * (kOutBuf / 4), nullptr, 0, 0, t, t == tw.ntiles - 1, &sh, &x);
+}
+
 // ---------------------------------------------------------------- two-pass sum
 
 // The hardware hands workgroup b of a launch to XCD b % 8.  The index this
@@ -1567,7 +1765,7 @@ __global__ void __launch_bounds__(64 * kLaneWaves, 4) k_homsum_lane_staged(Homsu
     if (t >= a.job.tl.ntiles)
         return;
     const ArgIO src{a.job.io};
-    homsum_tile_lane<true>(a.job.g, a.job.tl, a.job.p.k, src, a.job.io.out, a.job.io.out_cap, a.job.io.index,
+    homsum_tile_lane<kSumStaged>(a.job.g, a.job.tl, a.job.p.k, src, a.job.io.out, a.job.io.out_cap, a.job.io.index,
         a.job.io.size_out, t, nullptr, a.ws, 1, zk_lds + wv * (kOutBuf / 4), stage, a.job.tl.ntiles, t);
 }
 
@@ -1585,7 +1783,7 @@ __global__ void __launch_bounds__(64 * kLaneWaves, 5) k_homsum_lane_batch_staged
         return;
     const HomsumIO* io = (const HomsumIO*) b.jobs + jidx;
     const SumIO src{io};
-    homsum_tile_lane<true>(b.g, b.tl, b.p.k, src, ldc(&io->out), ldc(&io->out_cap), ldc(&io->index),
+    homsum_tile_lane<kSumStaged>(b.g, b.tl, b.p.k, src, ldc(&io->out), ldc(&io->out_cap), ldc(&io->index),
         ldc(&io->size_out), t, nullptr, b.ws, 1, zk_lds + wv * (kOutBuf / 4), stage, b.total_tiles,
         (uint64_t) jidx * b.tl.ntiles + t);
 }
@@ -1834,6 +2032,24 @@ hipError_t launch_decompress_lane_batch(const BatchArgs& b, hipStream_t s)
     hipLaunchKernelGGL(k_decompress_lane_batch_sums, grid, block, lane_lds_bytes(), s, b);
     hipLaunchKernelGGL(k_decompress_lane_batch_scan, dim3(b.max_segments, b.njobs), dim3(kDecScanThreads), 0, s, b);
     hipLaunchKernelGGL(k_decompress_lane_batch_out, grid, block, lane_lds_bytes(), s, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_homsum_lane_wg(const HomsumArgs& a, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_homsum_lane_wg, dim3((uint32_t) a.job.tl.ntiles), dim3(64 * kLaneWaves), lane_lds_bytes(),
+        s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_homsum_lane_wg_batch(const BatchArgs& b, hipStream_t s)
+{
+    if (b.uniform)
+        hipLaunchKernelGGL(k_homsum_lane_wg_batch, dim3((uint32_t) b.total_tiles), dim3(64 * kLaneWaves),
+            lane_lds_bytes(), s, b);
+    else
+        hipLaunchKernelGGL(k_homsum_lane_wg_mixed, dim3((uint32_t) b.total_tiles), dim3(64 * kLaneWaves),
+            lane_lds_bytes(), s, b);
     return hipGetLastError();
 }
 
