@@ -208,6 +208,8 @@ hipError_t launch_homsum(const HomsumArgs& a, hipStream_t s);
 hipError_t launch_decompress(const DecompressArgs& a, hipStream_t s);
 hipError_t launch_batch(BatchKind kind, const BatchArgs& b, uint32_t threads, size_t lds, hipStream_t s);
 hipError_t launch_parse(const ParseArgs& a, hipStream_t s);
+// Compute units of the current device (queried once per device).
+uint32_t device_cu_count();
 // Lane layout (zccl_lane.hip): tilings from make_lane_tiling, 36-value blocks.
 size_t lane_lds_bytes();
 hipError_t launch_homsum_lane(const HomsumArgs& a, hipStream_t s);

@@ -1999,22 +1999,27 @@ bool set_decompress_groups_per_cu(int groups)
 
 int get_decompress_groups_per_cu() { return g_dec_groups.load(); }
 
+uint32_t device_cu_count()
+{
+    static std::atomic<int> cus[64];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64)
+        dev = 0;
+    int n = cus[dev].load();
+    if (n <= 0) {
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0)
+            n = 256;
+        cus[dev].store(n);
+    }
+    return (uint32_t) n;
+}
+
 // Workgroups of a persistent decompress pass: the tuned number per CU (at
 // most as many as the device holds at once), or one per four tiles when that
 // is fewer.
 static uint32_t dec_groups(uint64_t tiles)
 {
-    static int cus[64] = {0};
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64)
-        dev = 0;
-    if (!cus[dev]) {
-        int n = 0;
-        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0)
-            n = 256;
-        cus[dev] = n;
-    }
-    return (uint32_t) std::min<uint64_t>(ceil_div(tiles, kLaneWaves), (uint64_t) cus[dev] * g_dec_groups.load());
+    return (uint32_t) std::min<uint64_t>(ceil_div(tiles, kLaneWaves), (uint64_t) device_cu_count() * g_dec_groups.load());
 }
 
 hipError_t launch_decompress_lane(const DecompressArgs& a, hipStream_t s)
