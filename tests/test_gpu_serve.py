@@ -82,3 +82,64 @@ def test_results_same_alone_and_in_mix(mods):
     for r in [q for q in reqs if q.kind == "homsum"][:2]:
         want = O.homsum([s.to_host() for s in r.src], r.n, L, T)
         assert np.array_equal(r.out.to_host(), want), r.rid
+
+
+def _check_alone(codec, reqs, L, T):
+    for r in reqs:
+        if r.kind == "compress":
+            alone = codec.compress(r.x, r.eb, L, T)
+            assert torch.equal(r.out.data[: r.out.size], alone.data[: alone.size]), r.rid
+        elif r.kind == "decompress":
+            alone = codec.decompress(r.src[0], r.eb)
+            assert torch.equal(r.out.view(torch.int32), alone.view(torch.int32)), r.rid
+        else:
+            alone = codec.homsum(r.src)
+            assert torch.equal(r.out.data[: r.out.size], alone.data[: alone.size]), r.rid
+        del alone
+
+
+def test_full_size_range_and_budget(mods):
+    """The serving mix at its full request sizes: 64 KiB to 512 MiB per
+    request (plus three requests of exactly 512 MiB), groups of at most 8
+    jobs on 4 slots, the 200 GB budget with everything arriving at once, then
+    Poisson arrivals, then a budget that holds only the two largest results
+    at a time.  Every result equals the
+    same request run alone, and the reserved bytes never pass the budget."""
+    codec, serve, synth = mods
+    from zccl_amd import workloads
+
+    L, T = 36, 16
+    field = synth.grf3d(512, seed=11)  # 134 217 728 values = 512 MiB
+    reqs = workloads.serving_requests(20, 11, L, T, field)
+    reqs += workloads.serving_requests(3, 12, L, T, field, lo=512 << 20, hi=512 << 20)
+    for i, r in enumerate(reqs):
+        r.rid = i
+    sizes = sorted(r.n * 4 for r in reqs)
+    say("requests", len(reqs), {k: sum(r.kind == k for r in reqs) for k in serve.KINDS},
+        "bytes min", sizes[0], "max", sizes[-1])
+    assert sizes[0] >= 64 << 10 and sizes[-1] == 512 << 20
+    resident = sum(sum(s.data.numel() + s.index.numel() for s in (r.src or [])) for r in reqs)
+    resident += field.numel() * 4
+    ex = serve.Executor(L, T, slots=4)
+
+    def run(budget, arrivals, tag):
+        for r, a in zip(reqs, arrivals):
+            r.arrival, r.out = float(a), None
+        srv = serve.Server(ex, hbm_budget=budget, slots=4, max_jobs=8, resident_bytes=resident)
+        st = srv.run(reqs)
+        torch.cuda.synchronize()
+        say(tag, st.as_dict())
+        assert st.peak_reserved_bytes <= budget - resident
+        _check_alone(codec, reqs, L, T)
+        codec.check_errors()
+        return st
+
+    st = run(200e9, [0.0] * len(reqs), "closed loop, 200 GB")
+    assert 3 <= st.steps < len(reqs)  # groups of at most 8 jobs
+    rate = len(reqs) / st.makespan_s
+    run(200e9, serve.poisson_arrivals(len(reqs), 0.8 * rate, 13), "poisson 0.8")
+    need = sorted(ex.reserve(r) for r in reqs)
+    tight = resident + need[-1] + need[-2]
+    st = run(tight, [0.0] * len(reqs), f"budget {tight / 1e9:.2f} GB")
+    assert st.peak_reserved_bytes <= need[-1] + need[-2]
+    assert st.steps >= 3
