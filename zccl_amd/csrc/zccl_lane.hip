@@ -168,8 +168,10 @@ __device__ __forceinline__ uint32_t mad_i24(uint32_t a, uint32_t b, uint32_t c)
     return d;
 }
 
-template <bool wide>
-__device__ __forceinline__ void decode_fast(const Block& q, uint32_t (&acc)[L])
+// N < L: only the sum of the block's values is wanted; value j is added to
+// acc[j % N] (N short chains of adds instead of L values to add up).
+template <bool wide, uint32_t N = L>
+__device__ __forceinline__ void decode_fast(const Block& q, uint32_t (&acc)[N])
 {
     const uint32_t r = q.w & 7, hb = q.w & 8;  // hb: bits per high byte (0 or 8)
     const uint32_t s0 = __builtin_amdgcn_alignbyte(q.h[1], q.h[0], 1);  // sign bytes 0..3, memory order
@@ -204,7 +206,7 @@ __device__ __forceinline__ void decode_fast(const Block& q, uint32_t (&acc)[L])
             const uint32_t bit = j < 32 ? 8 * (j >> 3) + 7 - (j & 7) : 7 - (j - 32);
             const uint32_t sw = j < 32 ? ((bit & 1) ? so0 : se0) : ((bit & 1) ? so1 : se1);
             const int32_t f = __builtin_amdgcn_sbfe((int32_t) sw, bit & ~1u, 2);
-            acc[j] = mad_i24(m, (uint32_t) f, acc[j]);
+            acc[j % N] = mad_i24(m, (uint32_t) f, acc[j % N]);
         }
     }
 }
@@ -1207,35 +1209,48 @@ __device__ __forceinline__ void dec_run(const Args& a, WsHeader* ws, uint64_t fi
         bad_byte |= use && cc.has && (cc.w || cc.staged || cc.o + 4 <= cc.lim) && (q.h[0] & 0xFFu) != cc.w;
         bad_over |= cc.skip;
         q.w = use ? cc.w : 0u;
-        uint32_t acc[L];
-#pragma unroll
-        for (uint32_t j = 0; j < L; j++)
-            acc[j] = 0;
-        decode_fast<true>(q, acc);
         const bool gen = !fast && !cc.skip;
-        if (__any(gen)) {
-            if (gen)
-                bad_byte |= decode_general(cc.v, cc.o, cc.w, cc.nj, acc);
-        }
-        if (cc.nj < L) {
-#pragma unroll
-            for (uint32_t j = 0; j < L; j++)
-                if (j >= cc.nj)
-                    acc[j] = 0;
-        }
-        DecJob d;
-        dec_job(a, pc.idx, d);
-        const Geom& g = d.g;
-        const Tile x = tile_of(g, d.tl, d.t);
         if constexpr (Pass == 1) {
-            uint32_t b = 0;
+            // the block's sum only (a lane on the fast path has nj = L or
+            // adds zeros; the general decoder adds garbage past nj)
+            uint32_t part[2] = {0u, 0u};
+            decode_fast<true>(q, part);
+            uint32_t b = part[0] + part[1];
+            if (__any(gen)) {
+                uint32_t acc[L];
 #pragma unroll
-            for (uint32_t j = 0; j < L; j++)
-                b += acc[j];
+                for (uint32_t j = 0; j < L; j++)
+                    acc[j] = 0;
+                if (gen)
+                    bad_byte |= decode_general(cc.v, cc.o, cc.w, cc.nj, acc);
+#pragma unroll
+                for (uint32_t j = 0; j < L; j++)
+                    if (j < cc.nj)
+                        b += acc[j];
+            }
             const uint32_t inc = wave_scan_dpp(b);
             if (lane == 63)
                 sums[pc.idx] = inc;
         } else {
+            uint32_t acc[L];
+#pragma unroll
+            for (uint32_t j = 0; j < L; j++)
+                acc[j] = 0;
+            decode_fast<true>(q, acc);
+            if (__any(gen)) {
+                if (gen)
+                    bad_byte |= decode_general(cc.v, cc.o, cc.w, cc.nj, acc);
+            }
+            if (cc.nj < L) {
+#pragma unroll
+                for (uint32_t j = 0; j < L; j++)
+                    if (j >= cc.nj)
+                        acc[j] = 0;
+            }
+            DecJob d;
+            dec_job(a, pc.idx, d);
+            const Geom& g = d.g;
+            const Tile x = tile_of(g, d.tl, d.t);
 #pragma unroll
             for (uint32_t j = 1; j < L; j++)
                 acc[j] += acc[j - 1];
