@@ -2091,15 +2091,20 @@ __global__ void k_parse_finish(ParseArgs a, ParseLayout pl)
 __global__ void k_case_count(const uint8_t* w1, const uint8_t* w2, uint64_t nblocks,
     unsigned long long* counts)
 {
-    uint32_t c[4] = {0, 0, 0, 0};
+    // (four counters in registers: an array indexed by the case would be
+    // placed in memory, which the compiler maps to LDS)
+    uint32_t c0 = 0, c1 = 0, c2 = 0, c3 = 0;
     for (uint64_t i = blockIdx.x * (uint64_t) blockDim.x + threadIdx.x; i < nblocks;
          i += (uint64_t) gridDim.x * blockDim.x) {
-        uint32_t a = w1[i] != 0, b = w2[i] != 0;
-        c[a * 2 + b] += 1;
+        const uint32_t k = 2u * (w1[i] != 0) + (w2[i] != 0);
+        c0 += k == 0;
+        c1 += k == 1;
+        c2 += k == 2;
+        c3 += k == 3;
     }
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-        uint32_t v = c[k];
+        uint32_t v = k == 0 ? c0 : k == 1 ? c1 : k == 2 ? c2 : c3;
 #pragma unroll
         for (int d = 32; d >= 1; d >>= 1)
             v += __shfl_xor(v, d, 64);

@@ -1839,10 +1839,14 @@ __device__ __forceinline__ void copy_wave(uint8_t* dst, const uint8_t* src, uint
             if (ca >= d0 && ca + 16 <= d1) {
                 *(uint4*) ca = x[u];
             } else {
-                // x[u] holds source bytes from destination byte sb on
+                // x[u] holds source bytes from destination byte sb on (byte
+                // indices fixed at compile time: a variable one would put
+                // the chunks in memory, which the compiler maps to LDS)
                 const uintptr_t sb = ca > d0 ? ca : d0, e = ca + 16 < d1 ? ca + 16 : d1;
-                for (uintptr_t p = sb; p < e; p++)
-                    *(uint8_t*) p = (uint8_t) chunk_byte(x[u], (uint32_t) (p - sb));
+#pragma unroll
+                for (uint32_t i = 0; i < 16; i++)
+                    if (sb + i < e)
+                        *(uint8_t*) (sb + i) = (uint8_t) chunk_byte(x[u], i);
             }
         }
     }
