@@ -1663,13 +1663,20 @@ __device__ __forceinline__ uint32_t parse_next_nonzero(const uint32_t* buf, cons
 // blocks so far plus what that lane walked from the offset on.  Walks merge
 // within a few blocks, so the later candidates take a few steps instead of
 // the whole tile.
-__global__ void __launch_bounds__(256) k_parse_spec(ParseArgs a, ParseLayout pl)
+// Two waves per tile: wave 0's recorded walks take the first 64 candidates,
+// and both waves the remaining ones (128 per round, one round for blocks of
+// up to 36 values), so more tiles fit on a CU (the walks are chains of
+// dependent LDS reads: tiles in flight, not instructions, set the pace).
+constexpr uint32_t kParseSpecThreads = 128;
+constexpr uint32_t kParseSpecWaves = kParseSpecThreads / 64;
+
+__global__ void __launch_bounds__(kParseSpecThreads) k_parse_spec(ParseArgs a, ParseLayout pl)
 {
     __shared__ uint32_t buf[kParseTile / 4 + 5];  // one dword past the staged bytes for the 8-byte window
     __shared__ uint32_t nzb[kParseNzWords];
     __shared__ uint32_t seen[kParseTile];          // offset -> wave-0 lane << 16 | blocks before it, or ~0
     __shared__ uint32_t s_cnt[64], s_ex[64];       // wave 0's walks: blocks, exit (kParseNoExit: dead)
-    __shared__ uint32_t s_lo[4], s_hi[4];
+    __shared__ uint32_t s_lo[kParseSpecWaves], s_hi[kParseSpecWaves];
     const uint32_t tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
     const uint64_t gt = blockIdx.x;
     const uint32_t c = (uint32_t) (gt / pl.tpc), t = (uint32_t) (gt % pl.tpc);
@@ -1680,8 +1687,8 @@ __global__ void __launch_bounds__(256) k_parse_spec(ParseArgs a, ParseLayout pl)
     if (ts >= ce)
         return;  // past the chunk's region
     const uint64_t te = ts + kParseTile < ce ? ts + kParseTile : ce;
-    const uint32_t ndw = parse_stage<256>(a, ts, te, buf, tid);
-    for (uint32_t i = tid; i < kParseTile; i += 256)
+    const uint32_t ndw = parse_stage<kParseSpecThreads>(a, ts, te, buf, tid);
+    for (uint32_t i = tid; i < kParseTile; i += kParseSpecThreads)
         seen[i] = ~0u;
     __syncthreads();
     if (wv == 0)
@@ -1764,7 +1771,7 @@ __global__ void __launch_bounds__(256) k_parse_spec(ParseArgs a, ParseLayout pl)
         keep(lane, ex, cnt);
     }
     __syncthreads();
-    for (uint32_t e0 = 64 + 64 * wv; e0 < pl.maxb; e0 += 256) {
+    for (uint32_t e0 = 64 + 64 * wv; e0 < pl.maxb; e0 += kParseSpecThreads) {
         uint32_t cnt;
         const uint32_t ex = walk(e0 + lane, false, cnt);
         keep(e0 + lane, ex, cnt);
@@ -1780,8 +1787,12 @@ __global__ void __launch_bounds__(256) k_parse_spec(ParseArgs a, ParseLayout pl)
     }
     __syncthreads();
     if (tid == 0) {
-        lo = min(min(s_lo[0], s_lo[1]), min(s_lo[2], s_lo[3]));
-        hi = max(max(s_hi[0], s_hi[1]), max(s_hi[2], s_hi[3]));
+        for (uint32_t w = 1; w < kParseSpecWaves; w++) {
+            s_lo[0] = min(s_lo[0], s_lo[w]);
+            s_hi[0] = max(s_hi[0], s_hi[w]);
+        }
+        lo = s_lo[0];
+        hi = s_hi[0];
         ((uint32_t*) ((char*) a.ws + pl.conv_off))[gt] = lo == hi ? lo : kParseDead;
     }
 }
@@ -2247,7 +2258,7 @@ hipError_t launch_parse(const ParseArgs& a, hipStream_t s)
         const uint32_t groups = (uint32_t) ceil_div(pl.ntiles, 4);
         const uint64_t words = (pl.bytes - pl.flag_off) / 8;
         hipLaunchKernelGGL(k_parse_begin, dim3(1), dim3(64), 0, s, a, pl);
-        hipLaunchKernelGGL(k_parse_spec, dim3((uint32_t) pl.ntiles), dim3(256), 0, s, a, pl);
+        hipLaunchKernelGGL(k_parse_spec, dim3((uint32_t) pl.ntiles), dim3(kParseSpecThreads), 0, s, a, pl);
         hipLaunchKernelGGL(k_parse_resolve, dim3(a.g.T), dim3(kParseResolveThreads), 0, s, a, pl);
         hipLaunchKernelGGL(k_parse_emit, dim3(groups), dim3(256), 0, s, a, pl);
         hipLaunchKernelGGL(k_parse_finish, dim3(1 + (uint32_t) std::min<uint64_t>(ceil_div(words, 256), 4096)), dim3(256),
