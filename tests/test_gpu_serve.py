@@ -103,8 +103,9 @@ def test_full_size_range_and_budget(mods):
     request (plus three requests of exactly 512 MiB), groups of at most 8
     jobs on 4 slots, the 200 GB budget with everything arriving at once, then
     Poisson arrivals, then a budget that holds only the two largest results
-    at a time.  Every result equals the
-    same request run alone, and the reserved bytes never pass the budget."""
+    at a time, first in arrival order, then with bypass of a blocked head.
+    Every result equals the same request run alone, and the reserved bytes
+    never pass the budget."""
     codec, serve, synth = mods
     from zccl_amd import workloads
 
@@ -122,10 +123,10 @@ def test_full_size_range_and_budget(mods):
     resident += field.numel() * 4
     ex = serve.Executor(L, T, slots=4)
 
-    def run(budget, arrivals, tag):
+    def run(budget, arrivals, tag, bypass=0):
         for r, a in zip(reqs, arrivals):
             r.arrival, r.out = float(a), None
-        srv = serve.Server(ex, hbm_budget=budget, slots=4, max_jobs=8, resident_bytes=resident)
+        srv = serve.Server(ex, hbm_budget=budget, slots=4, max_jobs=8, resident_bytes=resident, bypass=bypass)
         st = srv.run(reqs)
         torch.cuda.synchronize()
         say(tag, st.as_dict())
@@ -143,3 +144,6 @@ def test_full_size_range_and_budget(mods):
     st = run(tight, [0.0] * len(reqs), f"budget {tight / 1e9:.2f} GB")
     assert st.peak_reserved_bytes <= need[-1] + need[-2]
     assert st.steps >= 3
+    # the same budget with smaller requests admitted past a blocked head
+    st = run(tight, [0.0] * len(reqs), f"budget {tight / 1e9:.2f} GB, bypass 4", bypass=4)
+    assert st.peak_reserved_bytes <= need[-1] + need[-2]

@@ -112,3 +112,39 @@ def test_resident_state_counts_against_budget():
     reqs = [Request(i, "compress", 1000, 1e-3, 0.0, src=[None]) for i in range(10)]
     st, ex = run(reqs, hbm_budget=20_000, resident_bytes=10_000, slots=4, max_jobs=10)
     assert st.peak_reserved_bytes <= 10_000
+
+
+def _blocked_head_case():
+    """A large request at the head behind one that holds most of the budget,
+    followed by small ones that fit beside the first."""
+    big = Request(0, "compress", 1_000_000, 1e-3, 0.0, src=[object()])
+    head = Request(1, "compress", 1_000_000, 1e-3, 0.0, src=[object()])
+    small = [Request(2 + i, "compress", 10_000, 1e-3, 0.0, src=[object()]) for i in range(12)]
+    return [big, head] + small, 5 * 1_000_000 + 5 * 10_000 * 4
+
+
+def test_fifo_without_bypass_holds_later_requests_behind_the_head():
+    reqs, budget = _blocked_head_case()
+    st, ex = run(reqs, hbm_budget=budget, slots=4, max_jobs=4)
+    head = reqs[1]
+    assert all(r.t_admit >= head.t_admit for r in reqs[2:])
+
+
+def test_bypass_admits_fitting_requests_past_a_blocked_head_and_bounds_it():
+    reqs, budget = _blocked_head_case()
+    st, ex = run(reqs, hbm_budget=budget, slots=4, max_jobs=4, bypass=2)
+    head = reqs[1]
+    early = [r for r in reqs[2:] if r.t_admit < head.t_admit]
+    assert early, "small requests went ahead of the blocked head"
+    # at most `bypass` groups of at most max_jobs requests pass the head
+    assert len(early) <= 2 * 4
+    groups_before = {tuple(g) for s, t, e, g in ex.log if t < head.t_admit and 0 not in g}
+    assert len(groups_before) <= 2
+    assert st.peak_reserved_bytes <= budget
+    for r in reqs:
+        assert r.arrival <= r.t_admit <= r.t_done
+
+
+def test_bypass_rejects_negative():
+    with pytest.raises(ValueError):
+        Server(SimExecutor(), hbm_budget=1e9, bypass=-1)

@@ -9,7 +9,9 @@ CPU.  The loop here:
 * admits arrived requests in arrival order while their output memory fits an
   HBM budget and an execution slot (a HIP stream with its own lookback
   workspace) is free; a request that does not fit waits at the head of the
-  queue until completions release memory;
+  queue until completions release memory, and with ``bypass`` > 0 later
+  requests that fit are admitted past it, at most ``bypass`` groups in a
+  row, after which the head is served before anything else (no starvation);
 * launches each admitted group as at most three batch launches (one per
   operation kind), so many small requests share one kernel;
 * keeps up to ``slots`` groups in flight: requests that arrive while others
@@ -100,9 +102,10 @@ class Server:
 
     def __init__(self, executor, hbm_budget: float, slots: int = 4, max_jobs: int = 256,
                  resident_bytes: int = 0, clock: Callable[[], float] = time.perf_counter,
-                 idle: Optional[Callable[[float], None]] = None):
-        if slots < 1 or max_jobs < 1:
-            raise ValueError("need at least one slot and one job per step")
+                 idle: Optional[Callable[[float], None]] = None, bypass: int = 0):
+        if slots < 1 or max_jobs < 1 or bypass < 0:
+            raise ValueError("need at least one slot and one job per step, and bypass >= 0")
+        self.bypass = bypass
         self.ex = executor
         # resident state (inputs, operands already in HBM) counts against the budget
         self.budget = float(hbm_budget) - float(resident_bytes)
@@ -124,6 +127,7 @@ class Server:
         peak = 0
         nsteps = 0
         max_inflight = 0
+        passed = 0  # groups admitted in a row past a head that did not fit
         i = 0
         t0 = self.clock()
         while i < len(reqs) or waiting or inflight:
@@ -152,6 +156,23 @@ class Server:
                     waiting.popleft()
                     reserved += r.out_bytes
                     group.append(r)
+                if waiting and len(group) < self.max_jobs and passed < self.bypass:
+                    # the head does not fit: later requests that do go ahead of it
+                    head = waiting.popleft()
+                    keep: collections.deque = collections.deque([head])
+                    while waiting and len(group) < self.max_jobs:
+                        r = waiting.popleft()
+                        if used + reserved + r.out_bytes <= self.budget:
+                            reserved += r.out_bytes
+                            group.append(r)
+                        else:
+                            keep.append(r)
+                    keep.extend(waiting)
+                    waiting = keep
+                    if group:
+                        passed += 1
+                elif group:
+                    passed = 0
                 if group:
                     slot = free.pop()
                     t = self.clock() - t0

@@ -18,7 +18,7 @@ from __future__ import annotations
 
 import math
 import time
-from typing import Dict, List
+from typing import Dict, List, Optional
 
 import numpy as np
 import torch
@@ -220,25 +220,46 @@ def _reset(reqs):
         r.t_admit = r.t_done = float("nan")
 
 
-def serving(nreq: int = 400, L: int = 36, T: int = 16, slots: int = 4, max_jobs: int = 64,
-            budget: float = 200e9, loads=(0.5, 0.8, 0.95), seed: int = 5, side: int = 512) -> Dict[str, object]:
+def serving(nreq: int = 400, L: int = 36, T: int = 16, slots: Optional[int] = None,
+            max_jobs: Optional[int] = None, budget: float = 200e9, loads=(0.5, 0.8, 0.95), seed: int = 5,
+            side: int = 512, bypass: int = 4, slot_choices=(2, 4, 8),
+            job_choices=(16, 64, 256)) -> Dict[str, object]:
+    """The serving mix at the given budget.  Slots and jobs per launch not
+    given are chosen by the closed-loop makespan over slot_choices x
+    job_choices (the same requests, everything arriving at once); the loads
+    then run at the chosen pair."""
     field = synth.grf3d(side, seed=seed)
     reqs = serving_requests(nreq, seed, L, T, field)
     resident = sum(sum(s.data.numel() + s.index.numel() for s in (r.src or [])) for r in reqs)
     resident += field.numel() * 4
     _log(f"serving: {nreq} requests, resident {resident / 1e9:.1f} GB")
-    ex = serve.Executor(L, T, slots=slots)
-    out: Dict[str, object] = {"requests": nreq, "slots": slots, "max_jobs_per_launch": max_jobs,
-                              "hbm_budget_GB": budget / 1e9, "resident_GB": round(resident / 1e9, 2)}
+    s_opts = (slots,) if slots else tuple(slot_choices)
+    j_opts = (max_jobs,) if max_jobs else tuple(job_choices)
+    ex = serve.Executor(L, T, slots=max(s_opts))
 
     # closed loop: everything arrives at t = 0 (capacity)
     for r in reqs:
         r.arrival = 0.0
-    srv = serve.Server(ex, hbm_budget=budget, slots=slots, max_jobs=max_jobs, resident_bytes=resident)
-    srv.run(reqs)  # warm-up: allocations, plan paths
-    _reset(reqs)
-    torch.cuda.synchronize()
-    cap = srv.run(reqs)
+    sweep = {}
+    best = None
+    for s_ in s_opts:
+        for j_ in j_opts:
+            srv_ = serve.Server(ex, hbm_budget=budget, slots=s_, max_jobs=j_, resident_bytes=resident,
+                                bypass=bypass)
+            srv_.run(reqs)  # warm-up: allocations, plan paths
+            _reset(reqs)
+            torch.cuda.synchronize()
+            st_ = srv_.run(reqs)
+            _reset(reqs)
+            torch.cuda.synchronize()
+            sweep[f"slots={s_},max_jobs={j_}"] = round(st_.throughput_GBps, 1)
+            if best is None or st_.makespan_s < best[2].makespan_s:
+                best = (s_, j_, st_, srv_)
+    slots, max_jobs, cap, srv = best
+    out: Dict[str, object] = {"requests": nreq, "slots": slots, "max_jobs_per_launch": max_jobs, "bypass": bypass,
+                              "hbm_budget_GB": budget / 1e9, "resident_GB": round(resident / 1e9, 2)}
+    if len(sweep) > 1:
+        out["tuned"] = {"closed_loop_GBps": sweep, "chosen": {"slots": slots, "max_jobs": max_jobs}}
     out["closed_loop"] = cap.as_dict()
     rate = nreq / cap.makespan_s
     # plain per-call: one request at a time, each its own launch, completed
