@@ -1610,7 +1610,7 @@ __device__ __forceinline__ void parse_fail(const ParseArgs& a, const ParseLayout
 
 __device__ __forceinline__ uint32_t parse_block_bytes(uint32_t w, uint32_t L)
 {
-    return w ? 1u + ((L + 7u) >> 3) + (w >> 3) * L + (((w & 7u) * L + 7u) >> 3) : 1u;
+    return w ? 1u + ((L + 7u) >> 3) + ((w * L + 7u) >> 3) : 1u;  // (w >> 3) L + ceil((w & 7) L / 8) = ceil(w L / 8)
 }
 
 // One bit per nonzero dword of a wave's staged bytes (ndw dwords), built with
@@ -1733,8 +1733,7 @@ __global__ void __launch_bounds__(kParseSpecThreads) k_parse_spec(ParseArgs a, P
             const uint32_t w = (uint32_t) x & 0xFFu;
             const uint32_t span = 8u - (q & 3);
             const uint32_t zr = x ? (uint32_t) __builtin_ctzll(x) >> 3 : span;
-            const uint32_t wc = min(w, 32u);
-            const uint32_t bsz = hb + __umul24(wc >> 3, a.g.L) + ((__umul24(wc & 7u, a.g.L) + 7u) >> 3);
+            const uint32_t bsz = hb + ((__umul24(w, a.g.L) + 7u) >> 3);  // (used only for 0 < w <= 32)
             const uint32_t z = min(zr, len - min(pos, len));
             const bool bad = act && w > 32;
             dead |= bad;
@@ -2001,7 +2000,7 @@ __global__ void __launch_bounds__(256) k_parse_emit(ParseArgs a, ParseLayout pl)
         if (j == nb - 1)
             pos += block_bytes(w, last_len);
         else
-            pos += hb + __umul24(w >> 3, g.L) + ((__umul24(w & 7u, g.L) + 7u) >> 3);
+            pos += hb + ((__umul24(w, g.L) + 7u) >> 3);
         j++;
         if (++k == 64)
             flush();
