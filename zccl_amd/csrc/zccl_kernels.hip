@@ -1972,7 +1972,8 @@ __global__ void __launch_bounds__(256) k_parse_emit(ParseArgs a, ParseLayout pl)
         }
         k = 0;
     };
-    while (pos < len && j < nb) {
+    // every block but the chunk's last (a run of zero bytes may end the chunk)
+    while (pos < len && j + 1 < nb) {
         const uint32_t w = __builtin_amdgcn_readfirstlane(tb[pos]);
         if (w > 32) {
             bad = true;
@@ -1997,13 +1998,26 @@ __global__ void __launch_bounds__(256) k_parse_emit(ParseArgs a, ParseLayout pl)
             my_j = j;
             my_w = w;
         }
-        if (j == nb - 1)
-            pos += block_bytes(w, last_len);
-        else
-            pos += hb + ((__umul24(w, g.L) + 7u) >> 3);
+        pos += hb + ((__umul24(w, g.L) + 7u) >> 3);
         j++;
         if (++k == 64)
             flush();
+    }
+    // the chunk's last block (last_len values), if it starts in this tile
+    if (!bad && pos < len && j + 1 == nb) {
+        const uint32_t w = __builtin_amdgcn_readfirstlane(tb[pos]);
+        if (w > 32) {
+            bad = true;
+        } else {
+            if (lane == k) {
+                my_pos = pos;
+                my_j = j;
+                my_w = w;
+            }
+            pos += block_bytes(w, last_len);
+            j++;
+            k++;  // (at most 64: the loop flushed at 64)
+        }
     }
     flush();
     if (lane != 0)
