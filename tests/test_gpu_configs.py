@@ -126,7 +126,7 @@ def test_mixed_fields_512cube_properties(codec, eb):
         torch.cuda.empty_cache()
 
 
-def test_config2_full_size_bit_exact(codec):
+def test_eight_512cube_fields_full_size_bit_exact(codec):
     """The headline workload at full size: 8 ranks x 512^3 FP32 GRF (k^-3,
     shared seed) + N(0, 0.05 std) per rank, relative eb 1e-3, L = 36, T = 16.
     All 64 segment streams (batched compress), the 8 fused sums of the
@@ -138,7 +138,7 @@ def test_config2_full_size_bit_exact(codec):
 
     P, side, L, T = 8, 512, 36, 16
     count = side ** 3
-    say("config 2 fields")
+    say("8 x 512^3 fields")
     base = synth.grf3d(side, seed=20261202)
     xs = [synth.perturb(base, 0.05, seed=1000 + r) for r in range(P)]
     del base
@@ -158,7 +158,7 @@ def test_config2_full_size_bit_exact(codec):
     codec.check_errors()
     for j in range(P):
         assert torch.equal(per_call[j].data[: per_call[j].size], outs[j].data[: outs[j].size]), j
-    say("config 2 oracle", eb)
+    say("8 x 512^3 oracle", eb)
     yh = y.cpu().numpy()
     for j in range(P):
         a, n = ar.displs[j], ar.cnts[j]
@@ -179,7 +179,7 @@ def test_config2_full_size_bit_exact(codec):
     assert bool(((y.double() - truth).abs() <= P * eb + 2.0 ** -21 * absum).all())
 
 
-def test_config4_uneven_batch_graph_vs_oracle(codec):
+def test_uneven_4096_buffers_graph_vs_oracle(codec):
     """The batched form: 16 simulated ranks x 32 buffers of log-uniform sizes
     (4 KB..8 MB of gradient-like values: N(0, 1e-2), 1 % outliers at 100x,
     distinct per rank), relative eb 1e-3 per buffer.  One compress launch of
@@ -191,7 +191,7 @@ def test_config4_uneven_batch_graph_vs_oracle(codec):
 
     L, T, ranks, nslot = 36, 16, 16, 32
     sizes = [max(n, T) for n in workloads.log_uniform_sizes(nslot, 4096, 8 << 20, 77)]
-    say("config 4 sizes", min(sizes), max(sizes), sum(sizes) * ranks * 4 / 1e9, "GB")
+    say("uneven batch sizes", min(sizes), max(sizes), sum(sizes) * ranks * 4 / 1e9, "GB")
     flat, views = workloads.gradient_buffers([n for _ in range(ranks) for n in sizes], 4097)
     xs = [[views[r * nslot + i] for i in range(nslot)] for r in range(ranks)]
     ebs = workloads.per_buffer_eb([torch.cat([xs[r][i] for r in range(ranks)]) for i in range(nslot)], 1e-3)
@@ -226,7 +226,7 @@ def test_config4_uneven_batch_graph_vs_oracle(codec):
     assert not hp.staged
     ws = codec.Workspace(max(cp.tiles, hp.tiles))
     run_graph(cp, hp, ws, [o for r in range(ranks) for o in outs[r]], sums)
-    say("config 4 oracle (mixed)")
+    say("uneven batch oracle (mixed)")
     for i, n in enumerate(sizes):
         want = []
         for r in range(ranks):
@@ -237,7 +237,7 @@ def test_config4_uneven_batch_graph_vs_oracle(codec):
 
     # uniform jobs above the two-pass threshold
     n2, njob = 2_400_017, 2
-    say("config 4 uniform two-pass batch")
+    say("uneven batch, uniform two-pass batch")
     flat2, v2 = workloads.gradient_buffers([n2] * (ranks * njob), 4098)
     eb2 = workloads.per_buffer_eb([flat2], 1e-3)[0]
     outs2 = [codec.empty_stream(n2, L, T) for _ in range(ranks * njob)]
